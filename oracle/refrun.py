@@ -1,0 +1,115 @@
+"""ctypes driver for the reference build in ``oracle/_ref/libfgtref.so``.
+
+Test infrastructure only: tests use it (in the build container, where the
+reference sources exist) to build trees with the reference tree builder and
+to run the reference ``boxfgt`` on them, producing the golden fixtures under
+``tests/golden/``.  ``bench.py``'s CPU-baseline leg may time it.
+
+Reference interfaces driven here:
+  * ``vol_tree_mem`` / ``vol_tree_build`` (src/common/tree_vol_coeffs_nd.f:83,444)
+  * ``boxfgt`` (src/bfgt/boxfgt.f:5-8), 28 by-reference arguments.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SO = os.path.join(HERE, "_ref", "libfgtref.so")
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(f"reference build missing: {REF_SO} (make -C oracle ref)")
+        _lib = C.CDLL(REF_SO)
+    return _lib
+
+
+@dataclass
+class Tree:
+    """Reference tree arrays (1-based box ids inside ``itree``)."""
+    ndim: int
+    norder: int
+    ipoly: int
+    iperiod: int
+    nboxes: int
+    nlevels: int
+    ltree: int
+    itree: np.ndarray      # int32 (ltree,)
+    iptr: np.ndarray       # int32 (8,)
+    centers: np.ndarray    # f64 (nboxes, ndim)  == Fortran centers(ndim,nboxes)
+    boxsize: np.ndarray    # f64 (nlevels+1,)
+    fvals: np.ndarray      # f64 (nboxes, npbox) == Fortran fvals(1,npbox,nboxes)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def gauss_params(centers, widths, amps) -> np.ndarray:
+    """Pack Gaussians (c[ndim], s[ndim], a) per source into the dpars layout."""
+    centers = np.atleast_2d(np.asarray(centers, np.float64))
+    widths = np.atleast_2d(np.asarray(widths, np.float64))
+    amps = np.asarray(amps, np.float64).reshape(-1, 1)
+    return np.ascontiguousarray(np.hstack([centers, widths, amps]).ravel())
+
+
+def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng) -> Tree:
+    L = lib()
+    dp = np.ascontiguousarray(dpars, np.float64)
+    nb, nl, lt = C.c_int(), C.c_int(), C.c_int()
+    L.refglue_tree_mem(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod),
+                       C.c_double(eps), C.c_int(norder), C.c_int(ng), _p(dp),
+                       C.byref(nb), C.byref(nl), C.byref(lt))
+    nboxes, nlevels, ltree = nb.value, nl.value, lt.value
+    npbox = norder ** ndim
+    itree = np.zeros(ltree, np.int32)
+    iptr = np.zeros(8, np.int32)
+    centers = np.zeros((nboxes, ndim), np.float64)
+    boxsize = np.zeros(nlevels + 1, np.float64)
+    fvals = np.zeros((nboxes, npbox), np.float64)
+    L.refglue_tree_build(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod),
+                         C.c_double(eps), C.c_int(norder), C.c_int(ng), _p(dp),
+                         C.c_int(nboxes), C.c_int(nlevels), C.c_int(ltree),
+                         _p(itree), _p(iptr), _p(centers), _p(boxsize), _p(fvals))
+    return Tree(ndim, norder, ipoly, iperiod, nboxes, nlevels, ltree, itree,
+                iptr, centers, boxsize, fvals)
+
+
+def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None):
+    """Run the reference boxfgt (ifpgh=1, ifnewtree=0, no extra targets).
+
+    Returns (pot[nboxes, npbox], timeinfo[20]).
+    """
+    L = lib()
+    npbox = tree.norder ** tree.ndim
+    f = np.ascontiguousarray(tree.fvals if fvals is None else fvals, np.float64)
+    pot = np.zeros_like(f)
+    grad = np.zeros(1, np.float64)
+    hess = np.zeros(1, np.float64)
+    targs = np.zeros(tree.ndim, np.float64)
+    pote = np.zeros(nd, np.float64)
+    timeinfo = np.zeros(100, np.float64)
+    itree = tree.itree.copy()   # the reference may write colleagues back (periodic, npwlevel<=1)
+    ints = [C.c_int(v) for v in (nd, tree.ndim)]
+    L.boxfgt_(C.byref(ints[0]), C.byref(ints[1]), C.byref(C.c_double(delta)),
+              C.byref(C.c_double(eps)), C.byref(C.c_int(tree.ipoly)),
+              C.byref(C.c_int(tree.iperiod)), C.byref(C.c_int(tree.norder)),
+              C.byref(C.c_int(npbox)), C.byref(C.c_int(tree.nboxes)),
+              C.byref(C.c_int(tree.nlevels)), C.byref(C.c_int(tree.ltree)),
+              _p(itree), _p(tree.iptr), _p(tree.centers), _p(tree.boxsize),
+              _p(f), C.byref(C.c_int(1)), _p(pot), _p(grad), _p(hess),
+              C.byref(C.c_int(0)), C.byref(C.c_int(0)), _p(targs),
+              C.byref(C.c_int(0)), _p(pote), _p(grad), _p(hess), _p(timeinfo))
+    return pot, timeinfo[:20].copy()
