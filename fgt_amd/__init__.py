@@ -1,0 +1,208 @@
+"""fgt_amd -- MI355X box Gauss transform behind the reference's ``boxfgt`` interface.
+
+The compute path is ``libfgt_amd.so`` (C++ host planner + HIP kernels for
+gfx950), reached through its C ABI (``include/fgt.h``).  This module mirrors
+the reference's Fortran interface for Python callers:
+
+* :func:`boxfgt` -- same arguments and meaning as ``boxfgt``
+  (src/bfgt/boxfgt.f:5-8): accumulates the potential into ``pot``.
+* :class:`Solver` -- device-resident plan for repeated solves on one tree.
+* :func:`vol_tree_build` -- host tree builder producing the arrays of the
+  reference's ``vol_tree_mem``/``vol_tree_build``.
+
+There is no CPU fallback: if the library (or a GPU) is missing, calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfgt_amd.so")
+
+_lib = None
+
+
+class FgtError(RuntimeError):
+    """Raised when the library reports a nonzero status."""
+
+
+class _SolverStats(C.Structure):
+    _fields_ = [("npwlevel", C.c_int), ("npw", C.c_int), ("nexp", C.c_int),
+                ("nlevstart", C.c_int), ("nslots", C.c_int),
+                ("periodic_small", C.c_int), ("all_pw", C.c_int),
+                ("nleaves", C.c_longlong), ("npw_leaves", C.c_longlong),
+                ("ndirect_pairs", C.c_longlong), ("nout", C.c_longlong),
+                ("expansion_bytes", C.c_longlong), ("stage_ms", C.c_double * 8)]
+
+
+class _PlanInfo(C.Structure):
+    _fields_ = [("npwlevel", C.c_int), ("npw", C.c_int), ("nexp", C.c_int),
+                ("nlevstart", C.c_int), ("nslots", C.c_int),
+                ("periodic_small", C.c_int), ("all_pw", C.c_int),
+                ("nleaves", C.c_longlong), ("npw_leaves", C.c_longlong),
+                ("ndirect_pairs", C.c_longlong), ("n_m2l_pairs", C.c_longlong),
+                ("n_parent_child", C.c_longlong)]
+
+
+def lib():
+    """Load libfgt_amd.so (built in-tree by ``python -m fgt_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FgtError(f"{LIB_PATH} is missing: build it with `python -m fgt_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.fgt_solver_create.restype = C.c_int
+        L.fgt_solver_fvals_device.restype = C.c_void_p
+        L.fgt_solver_pot_device.restype = C.c_void_p
+        L.fgt_plan_direct_pairs.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+def last_status():
+    buf = C.create_string_buffer(512)
+    st = lib().fgt_last_status(buf, 512)
+    return st, buf.value.decode(errors="replace")
+
+
+def _check(rc):
+    if rc != 0:
+        st, msg = last_status()
+        raise FgtError(f"fgt status {rc}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _i(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
+           ltree, itree, iptr, centers, boxsize, fvals, ifpgh, pot, grad=None,
+           hess=None, ifnewtree=0, ntarg=0, targs=None, ifpghtarg=0, pote=None,
+           grade=None, hesse=None, timeinfo=None):
+    """Box Gauss transform, argument for argument as the reference ``boxfgt``.
+
+    Arrays use the reference's Fortran memory order; with C-ordered numpy
+    arrays that means ``centers[nboxes, ndim]``, ``fvals[nboxes, npbox(, nd)]``
+    and ``pot`` of the same shape as ``fvals`` (accumulated, ``+=``).
+    Raises :class:`FgtError` for unsupported options (ifpgh > 1,
+    ifpghtarg > 0, ifnewtree = 1) instead of silently returning.
+    """
+    itree = np.ascontiguousarray(itree, np.int32)
+    iptr = np.ascontiguousarray(iptr, np.int32)
+    centers = np.ascontiguousarray(centers, np.float64)
+    boxsize = np.ascontiguousarray(boxsize, np.float64)
+    fvals = np.ascontiguousarray(fvals, np.float64)
+    if not (isinstance(pot, np.ndarray) and pot.dtype == np.float64 and pot.flags.c_contiguous):
+        raise TypeError("pot must be a C-contiguous float64 array (it is accumulated in place)")
+    if pot.size != nd * npbox * nboxes or fvals.size != nd * npbox * nboxes:
+        raise ValueError("fvals/pot must hold nd*npbox*nboxes values")
+    dummy = np.zeros(1, np.float64)
+    ti = np.zeros(20, np.float64) if timeinfo is None else timeinfo
+    rc = lib().fgt_boxfgt(_i(nd), _i(ndim), _d(delta), _d(eps), _i(ipoly), _i(iperiod),
+                          _i(norder), _i(npbox), _i(nboxes), _i(nlevels), _i(ltree),
+                          _p(itree), _p(iptr), _p(centers), _p(boxsize), _p(fvals),
+                          _i(ifpgh), _p(pot), _p(dummy if grad is None else grad),
+                          _p(dummy if hess is None else hess), _i(ifnewtree), _i(ntarg),
+                          _p(dummy if targs is None else targs), _i(ifpghtarg),
+                          _p(dummy if pote is None else pote), _p(dummy if grade is None else grade),
+                          _p(dummy if hesse is None else hesse), _p(ti))
+    _check(rc)
+    return pot
+
+
+class Solver:
+    """Device-resident box FGT on one tree (plan once, solve many times)."""
+
+    def __init__(self, nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels,
+                 ltree, itree, iptr, centers, boxsize):
+        self._h = C.c_void_p()
+        self.nd, self.ndim, self.norder, self.nboxes = nd, ndim, norder, nboxes
+        self.npbox = norder ** ndim
+        self._keep = [np.ascontiguousarray(itree, np.int32), np.ascontiguousarray(iptr, np.int32),
+                      np.ascontiguousarray(centers, np.float64), np.ascontiguousarray(boxsize, np.float64)]
+        it, ip, ce, bs = self._keep
+        _check(lib().fgt_solver_create(C.c_int(nd), C.c_int(ndim), C.c_double(delta), C.c_double(eps),
+                                       C.c_int(ipoly), C.c_int(iperiod), C.c_int(norder),
+                                       C.c_int(nboxes), C.c_int(nlevels), C.c_int(ltree),
+                                       _p(it), _p(ip), _p(ce), _p(bs), C.byref(self._h)))
+
+    @classmethod
+    def from_tree(cls, tree, delta, eps, nd=1):
+        return cls(nd, tree.ndim, delta, eps, tree.ipoly, tree.iperiod, tree.norder,
+                   tree.nboxes, tree.nlevels, tree.ltree, tree.itree, tree.iptr,
+                   tree.centers, tree.boxsize)
+
+    def set_fvals(self, fvals):
+        f = np.ascontiguousarray(fvals, np.float64)
+        if f.size != self.nd * self.npbox * self.nboxes:
+            raise ValueError("fvals size mismatch")
+        _check(lib().fgt_solver_set_fvals(self._h, _p(f)))
+
+    def execute(self, stream=None, timed=False):
+        _check(lib().fgt_solver_execute(self._h, C.c_void_p(stream or 0), C.c_int(1 if timed else 0)))
+
+    def get_pot(self, out=None, accumulate=False):
+        if out is None:
+            out = np.zeros((self.nboxes, self.npbox) if self.nd == 1 else (self.nboxes, self.npbox, self.nd))
+        _check(lib().fgt_solver_get_pot(self._h, _p(out), C.c_int(1 if accumulate else 0)))
+        return out
+
+    def stats(self):
+        s = _SolverStats()
+        _check(lib().fgt_solver_stats_get(self._h, C.byref(s)))
+        d = {k: getattr(s, k) for k, _ in _SolverStats._fields_ if k != "stage_ms"}
+        d["stage_ms"] = list(s.stage_ms)
+        return d
+
+    @property
+    def fvals_device_ptr(self):
+        return lib().fgt_solver_fvals_device(self._h)
+
+    @property
+    def pot_device_ptr(self):
+        return lib().fgt_solver_pot_device(self._h)
+
+    def close(self):
+        if self._h:
+            lib().fgt_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_summary(tree, delta, eps, nd=1):
+    """Host-only plan scalars (cutoff level, npw, list sizes) for a tree."""
+    info = _PlanInfo()
+    t = tree
+    rc = lib().fgt_plan_summary(C.c_int(nd), C.c_int(t.ndim), C.c_double(delta), C.c_double(eps),
+                                C.c_int(t.ipoly), C.c_int(t.iperiod), C.c_int(t.norder),
+                                C.c_int(t.nboxes), C.c_int(t.nlevels), C.c_int(t.ltree),
+                                _p(np.ascontiguousarray(t.itree, np.int32)),
+                                _p(np.ascontiguousarray(t.iptr, np.int32)),
+                                _p(np.ascontiguousarray(t.centers, np.float64)),
+                                _p(np.ascontiguousarray(t.boxsize, np.float64)), C.byref(info))
+    if rc != 0:
+        raise FgtError(f"plan failed with status {rc}")
+    return {k: getattr(info, k) for k, _ in _PlanInfo._fields_}
+
+
+from .tree import Tree, vol_tree_build  # noqa: E402  (host tree builder)
+
+__all__ = ["boxfgt", "Solver", "FgtError", "plan_summary", "Tree", "vol_tree_build", "lib",
+           "last_status"]

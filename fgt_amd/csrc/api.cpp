@@ -1,0 +1,396 @@
+// api.cpp -- C ABI (include/fgt.h): the boxfgt drop-in and the
+// device-resident solver.  Host orchestration of the stage kernels.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fgt.h"
+#include "kernels.h"
+#include "plan.h"
+
+namespace {
+
+thread_local int g_status = 0;
+thread_local std::string g_msg;
+
+void set_status(int st, const std::string& m) {
+    g_status = st;
+    g_msg = m;
+}
+
+void hcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A device buffer owned by the solver.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        if (b) hcheck(hipMalloc(&p, b), "hipMalloc");
+    }
+    template <class T>
+    void upload(const std::vector<T>& v) {
+        alloc(sizeof(T) * v.size());
+        if (!v.empty()) hcheck(hipMemcpy(p, v.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct fgt_solver {
+    fgt::Plan plan;
+    fgt::Geom geom{};
+    fgt::DevTables tabs{};
+    // tables
+    DBuf d_p2pw, d_pw2pot, d_wms, d_wm2l, d_one, d_loctab, d_locband;
+    // data
+    DBuf d_fvals, d_pot, d_expn;
+    // lists
+    DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
+    DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
+    DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
+    DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code;
+    std::vector<int> leaf_level_off;   // per level, offsets into the leaf lists
+    double stage_ms[8] = {0};
+};
+
+namespace {
+
+// Map problem-axis sign bits / offsets / table codes onto the (x, y, z) axes
+// of the kernels: 3-D (x,y,z); 2-D (x,_,z); 1-D (_,_,z).
+int map_sign(int ndim, int s) {
+    if (ndim == 3) return s;
+    if (ndim == 2) return (s & 1) | ((s & 2) << 1);
+    return (s & 1) << 2;
+}
+int real_mask(int ndim) { return ndim == 3 ? 7 : (ndim == 2 ? 5 : 4); }
+int map_m2l(int ndim, int idx) {
+    if (ndim == 3) return idx;
+    if (ndim == 2) return (idx % 3) + 3 * 1 + 9 * (idx / 3);
+    return 1 + 3 * 1 + 9 * idx;
+}
+int map_code(int ndim, int c) {
+    const int a0 = c & 15, a1 = (c >> 4) & 15, a2 = (c >> 8) & 15;
+    if (ndim == 3) return a0 | (a1 << 4) | (a2 << 8);
+    if (ndim == 2) return a0 | (6 << 4) | (a1 << 8);
+    return 6 | (6 << 4) | (a0 << 8);
+}
+
+std::vector<double2> to_d2(const std::vector<fgt::cplx>& v) {
+    std::vector<double2> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = make_double2(v[i].real(), v[i].imag());
+    return o;
+}
+
+void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
+    std::string err;
+    const int rc = fgt::make_plan(in, S.plan, &err);
+    if (rc != 0) throw std::invalid_argument(err);
+    const fgt::Plan& P = S.plan;
+    const int ndim = P.ndim, n = P.n;
+    fgt::Geom& g = S.geom;
+    g.nd = P.nd;
+    g.ndim = ndim;
+    g.nx = ndim == 1 ? 1 : n;
+    g.ny = ndim == 3 ? n : 1;
+    g.nz = n;
+    g.px = ndim == 1 ? 1 : P.npw;
+    g.py = ndim == 3 ? P.npw : 1;
+    g.pz = P.nh;
+    g.npw = P.npw;
+    g.npbox = P.npbox;
+    g.nexp = P.nexp;
+
+    // device limits for the LDS-staged kernels
+    int dev = 0;
+    hcheck(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hcheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    const size_t lds_p2pw = sizeof(double2) * ((size_t)P.npw * n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
+    const size_t lds_pw2pot = sizeof(double2) * ((size_t)n * P.npw + (size_t)g.nx * g.py);
+    if (P.do_pw && (lds_p2pw > prop.sharedMemPerBlock || lds_pw2pot > prop.sharedMemPerBlock))
+        throw std::invalid_argument("plane-wave count too large for the LDS-staged kernels");
+
+    S.d_fvals.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
+    S.d_pot.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
+
+    S.d_loctab.upload(P.loctab);
+    S.d_locband.upload(P.locband);
+    std::vector<double2> one{make_double2(1.0, 0.0)};
+    S.d_one.upload(one);
+    if (P.do_pw) {
+        S.d_p2pw.upload(to_d2(P.p2pw));
+        S.d_pw2pot.upload(to_d2(P.pw2pot));
+        S.d_wms.upload(to_d2(P.wms));
+        S.d_wm2l.upload(to_d2(P.wm2l));
+        S.d_expn.alloc(sizeof(double2) * 2 * (size_t)P.nd * P.nexp * P.nslots);
+    }
+    fgt::DevTables& t = S.tabs;
+    t.p2pw = S.d_p2pw.as<double2>();
+    t.pw2pot = S.d_pw2pot.as<double2>();
+    t.wms = S.d_wms.as<double2>();
+    t.wm2l = S.d_wm2l.as<double2>();
+    t.one = S.d_one.as<double2>();
+    t.loctab = S.d_loctab.as<double>();
+    t.locband = S.d_locband.as<int>();
+    t.n = n;
+    t.npw = P.npw;
+
+    // work lists
+    S.d_leaf_box.upload(P.leaf_box);
+    S.d_leaf_lev.upload(P.leaf_lev);
+    S.d_leaf_slot.upload(P.leaf_slot);
+    S.d_par_slot.upload(P.par_slot);
+    S.d_par_first.upload(P.par_first);
+    S.d_par_count.upload(P.par_count);
+    S.d_pc_parent.upload(P.pc_parent);
+    S.d_pc_child.upload(P.pc_child);
+    std::vector<int> sg(P.pc_sign.size()), sgs(P.pc_sign.size());
+    for (size_t i = 0; i < sg.size(); ++i) {
+        sg[i] = map_sign(ndim, P.pc_sign[i]);
+        sgs[i] = sg[i] ^ real_mask(ndim);
+    }
+    S.d_pc_sign.upload(sg);
+    S.d_pc_sign_split.upload(sgs);
+    S.d_m2l_tgt.upload(P.m2l_tgt);
+    S.d_m2l_off.upload(P.m2l_off);
+    S.d_m2l_src.upload(P.m2l_src);
+    std::vector<int> mi(P.m2l_idx.size());
+    for (size_t i = 0; i < mi.size(); ++i) mi[i] = map_m2l(ndim, P.m2l_idx[i]);
+    S.d_m2l_idx.upload(mi);
+    S.d_dir_tgt.upload(P.dir_tgt);
+    S.d_dir_lev.upload(P.dir_lev);
+    S.d_dir_off.upload(P.dir_off);
+    S.d_dir_src.upload(P.dir_src);
+    std::vector<int> dc(P.dir_code.size());
+    for (size_t i = 0; i < dc.size(); ++i) dc[i] = map_code(ndim, P.dir_code[i]);
+    S.d_dir_code.upload(dc);
+}
+
+void execute(fgt_solver& S, hipStream_t st, bool timed) {
+    const fgt::Plan& P = S.plan;
+    const fgt::Geom& g = S.geom;
+    const fgt::DevTables& t = S.tabs;
+    hipEvent_t ev[8];
+    if (timed)
+        for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
+    auto mark = [&](int i) { if (timed) hcheck(hipEventRecord(ev[i], st), "hipEventRecord"); };
+    hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
+    mark(0);
+    if (P.do_pw) {
+        // STEP 1: every PW leaf, all levels in one launch
+        fgt::launch_p2pw(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                         S.d_leaf_slot.as<int>(), S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
+        mark(1);
+        // STEP 2: fine to coarse
+        for (int l = P.nlevels - 1; l >= P.nlevstart; --l) {
+            const int a = P.par_level_off[l], b = P.par_level_off[l + 1];
+            fgt::launch_merge(g, t, b - a, S.d_par_slot.as<int>() + a, S.d_par_first.as<int>() + a,
+                              S.d_par_count.as<int>() + a, S.d_pc_child.as<int>(), S.d_pc_sign.as<int>(),
+                              P.nlevels - l, S.d_expn.as<double2>(), st);
+        }
+        mark(2);
+        // STEP 3
+        fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
+                        S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);
+        mark(3);
+        // STEP 4: coarse to fine
+        for (int l = P.nlevstart; l <= P.nlevels - 1; ++l) {
+            const int a = P.pc_level_off[l], b = P.pc_level_off[l + 1];
+            fgt::launch_split(g, t, b - a, S.d_pc_parent.as<int>() + a, S.d_pc_child.as<int>() + a,
+                              S.d_pc_sign_split.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+        }
+        mark(4);
+        // STEP 5
+        fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                           S.d_leaf_slot.as<int>(), S.d_expn.as<double2>(), S.d_pot.as<double>(), st);
+        mark(5);
+    } else {
+        for (int i = 1; i <= 5; ++i) mark(i);
+    }
+    // STEP 6
+    fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
+                       S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
+                       S.d_fvals.as<double>(), S.d_pot.as<double>(), st);
+    mark(6);
+    if (timed) {
+        hcheck(hipEventSynchronize(ev[6]), "hipEventSynchronize");
+        for (int i = 0; i < 6; ++i) {
+            float ms = 0;
+            hcheck(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "hipEventElapsedTime");
+            S.stage_ms[i] = ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        set_status(0, "");
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        set_status(2, e.what());
+        return 2;
+    } catch (const std::bad_alloc&) {
+        set_status(5, "host allocation failed");
+        return 5;
+    } catch (const std::exception& e) {
+        set_status(6, e.what());
+        return 6;
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgt_last_status(char* msg, int len) {
+    if (msg && len > 0) {
+        std::strncpy(msg, g_msg.c_str(), (size_t)len - 1);
+        msg[len - 1] = '\0';
+    }
+    return g_status;
+}
+
+int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder,
+                      int nboxes, int nlevels, int ltree, const int* itree, const int* iptr,
+                      const double* centers, const double* boxsize, fgt_solver** out) {
+    if (!out) return 2;
+    *out = nullptr;
+    return guarded([&] {
+        if (!itree || !iptr || !centers || !boxsize) throw std::invalid_argument("null tree array");
+        int npbox = 1;
+        for (int i = 0; i < ndim && i < 3; ++i) npbox *= norder;
+        fgt::PlanInput in{nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree,
+                          itree, iptr, centers, boxsize};
+        auto S = std::make_unique<fgt_solver>();
+        build_solver(*S, in);
+        *out = S.release();
+    });
+}
+
+int fgt_solver_set_fvals(fgt_solver* s, const double* fvals) {
+    return guarded([&] {
+        if (!s || !fvals) throw std::invalid_argument("null argument");
+        hcheck(hipMemcpy(s->d_fvals.p, fvals, s->d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
+    });
+}
+
+int fgt_solver_execute(fgt_solver* s, void* stream, int timed) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null solver");
+        execute(*s, static_cast<hipStream_t>(stream), timed != 0);
+    });
+}
+
+int fgt_solver_get_pot(fgt_solver* s, double* pot, int accumulate) {
+    return guarded([&] {
+        if (!s || !pot) throw std::invalid_argument("null argument");
+        const size_t cnt = s->d_pot.bytes / sizeof(double);
+        if (!accumulate) {
+            hcheck(hipMemcpy(pot, s->d_pot.p, s->d_pot.bytes, hipMemcpyDeviceToHost), "download pot");
+            return;
+        }
+        std::vector<double> tmp(cnt);
+        hcheck(hipMemcpy(tmp.data(), s->d_pot.p, s->d_pot.bytes, hipMemcpyDeviceToHost), "download pot");
+        for (size_t i = 0; i < cnt; ++i) pot[i] += tmp[i];
+    });
+}
+
+int fgt_solver_stats_get(const fgt_solver* s, fgt_solver_stats* o) {
+    if (!s || !o) return 2;
+    const fgt::Plan& P = s->plan;
+    std::memset(o, 0, sizeof *o);
+    o->npwlevel = P.npwlevel;
+    o->npw = P.npw;
+    o->nexp = P.nexp;
+    o->nlevstart = P.nlevstart;
+    o->nslots = P.nslots;
+    o->periodic_small = P.periodic_small;
+    o->all_pw = P.all_pw;
+    o->nleaves = P.nleaves;
+    o->npw_leaves = P.npw_leaves;
+    o->ndirect_pairs = P.ndirect_pairs;
+    o->nout = (long long)P.nd * P.npbox * P.nleaves;
+    o->expansion_bytes = (long long)s->d_expn.bytes;
+    for (int i = 0; i < 8; ++i) o->stage_ms[i] = s->stage_ms[i];
+    return 0;
+}
+
+void* fgt_solver_fvals_device(fgt_solver* s) { return s ? s->d_fvals.p : nullptr; }
+void* fgt_solver_pot_device(fgt_solver* s) { return s ? s->d_pot.p : nullptr; }
+
+void fgt_solver_destroy(fgt_solver* s) { delete s; }
+
+int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* iperiod, int* norder,
+               int* npbox, int* nboxes, int* nlevels, int* ltree, int* itree, int* iptr, double* centers,
+               double* boxsize, double* fvals, int* ifpgh, double* pot, double* grad, double* hess,
+               int* ifnewtree, int* ntarg, double* targs, int* ifpghtarg, double* pote, double* grade,
+               double* hesse, double* timeinfo) {
+    (void)grad; (void)hess; (void)ntarg; (void)targs; (void)pote; (void)grade; (void)hesse;
+    if (timeinfo)
+        for (int i = 0; i < 20; ++i) timeinfo[i] = 0.0;
+    return guarded([&] {
+        if (!nd || !ndim || !delta || !eps || !ipoly || !iperiod || !norder || !npbox || !nboxes ||
+            !nlevels || !ltree || !itree || !iptr || !centers || !boxsize || !fvals || !ifpgh || !pot ||
+            !ifnewtree || !ifpghtarg)
+            throw std::invalid_argument("null argument");
+        if (*ifpgh != 1) throw std::invalid_argument("only ifpgh = 1 (potential) is supported");
+        if (*ifpghtarg != 0) throw std::invalid_argument("extra targets (ifpghtarg >= 1) are not supported");
+        if (*ifnewtree != 0) throw std::invalid_argument("ifnewtree = 1 is not supported");
+        const double t0 = now_s();
+        fgt::PlanInput in{*nd, *ndim, *delta, *eps, *ipoly, *iperiod, *norder, *npbox, *nboxes, *nlevels,
+                          *ltree, itree, iptr, centers, boxsize};
+        fgt_solver S;
+        build_solver(S, in);
+        hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
+        const double t1 = now_s();
+        execute(S, nullptr, timeinfo != nullptr);
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        const double t2 = now_s();
+        const size_t cnt = S.d_pot.bytes / sizeof(double);
+        std::vector<double> tmp(cnt);
+        hcheck(hipMemcpy(tmp.data(), S.d_pot.p, S.d_pot.bytes, hipMemcpyDeviceToHost), "download pot");
+        for (size_t i = 0; i < cnt; ++i) pot[i] += tmp[i];
+        const double t3 = now_s();
+        if (timeinfo) {
+            timeinfo[0] = t1 - t0;
+            for (int i = 0; i < 6; ++i) timeinfo[1 + i] = S.stage_ms[i] * 1e-3;
+            timeinfo[7] = t3 - t2;
+        }
+        (void)t2;
+    });
+}
+
+void boxfgt_(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* iperiod, int* norder,
+             int* npbox, int* nboxes, int* nlevels, int* ltree, int* itree, int* iptr, double* centers,
+             double* boxsize, double* fvals, int* ifpgh, double* pot, double* grad, double* hess,
+             int* ifnewtree, int* ntarg, double* targs, int* ifpghtarg, double* pote, double* grade,
+             double* hesse, double* timeinfo) {
+    (void)fgt_boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree, itree, iptr,
+                     centers, boxsize, fvals, ifpgh, pot, grad, hess, ifnewtree, ntarg, targs, ifpghtarg, pote,
+                     grade, hesse, timeinfo);
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// kernels.h -- launch interface of the box-FGT HIP kernels (gfx950).
+//
+// Geometry is expressed with three axes (x, y, z); the plane-wave half
+// spectrum always lives on z.  A 3-D problem uses (n,n,n) nodes and
+// (npw,npw,nh) frequencies; a 2-D problem maps (x,y) -> (x,z) with a unit
+// y axis; a 1-D problem maps x -> z.  Unit axes carry the 1x1 identity
+// table and unit phases, so one set of kernels serves every dimension.
+//
+// Expansion layout in HBM (one "slot" = mp then loc, nd densities each):
+//   E[ind][k3][k1][k2], k2 fastest  (k3 < Pz = nh, k1 < Px, k2 < Py)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace fgt {
+
+struct Geom {
+    int nd;
+    int nx, ny, nz;      // nodes per axis
+    int px, py, pz;      // frequencies per axis (pz = half spectrum)
+    int npw;             // full 1-D frequency count (odd => zero-frequency halving)
+    int npbox;           // nx*ny*nz
+    long long nexp;      // px*py*pz
+    int ndim;
+};
+
+// Device-resident tables (all pointers are device pointers).
+struct DevTables {
+    // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)
+    const double2* p2pw;
+    // per level l: PW->value  U[l][i][k] (i < n, k < npw)
+    const double2* pw2pot;
+    // merge/split 1-D phases wms[klev-1][k], M2L 1-D phases wm2l[o][k]
+    const double2* wms;
+    const double2* wm2l;
+    // identity table for unit axes (one complex 1.0)
+    const double2* one;
+    // near-field masked tables loc[l][slot][k][j], bands [l][slot][k][2]
+    const double* loctab;
+    const int* locband;
+    int n;     // norder
+    int npw;
+};
+
+// Stage launchers (stream-ordered; all arguments device pointers).
+void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
+                 const int* lev, const int* slot, const double* fvals,
+                 double2* expn, hipStream_t s);
+void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
+                  const int* par_first, const int* par_count, const int* pc_child,
+                  const int* pc_sign, int klev, double2* expn, hipStream_t s);
+void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
+                const int* off, const int* src, const int* idx, double2* expn,
+                hipStream_t s);
+void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
+                  const int* pc_child, const int* pc_sign, int klev, double2* expn,
+                  hipStream_t s);
+void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
+                   const int* lev, const int* slot, const double2* expn,
+                   double* pot, hipStream_t s);
+void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
+                   const int* lev, const int* off, const int* src, const int* code,
+                   const double* fvals, double* pot, hipStream_t s);
+
+}  // namespace fgt

@@ -1,0 +1,375 @@
+// kernels.hip -- box-FGT stage kernels for gfx950 (FP64).
+//
+// Stage map (reference loops they replace):
+//   k_p2pw    STEP 1, values -> PW per PW leaf    (boxfgt.f:710-726,
+//             g3d_tens_prod_to_pw bfgt_pwrouts.f:153-222)
+//   k_merge   STEP 2, child mp -> parent mp       (boxfgt.f:746-765)
+//   k_m2l     STEP 3, mp -> loc at cutoff level   (boxfgt.f:787-804)
+//   k_split   STEP 4, parent loc -> child loc     (boxfgt.f:827-844)
+//   k_pw2pot  STEP 5, loc -> node values          (boxfgt.f:865-881,
+//             g3d_pw2pot bfgt_pwrouts.f:719-790)
+//   k_direct  STEP 6, banded near field           (boxfgt.f:904-928,
+//             g3d_tens_prod_to_potloc_fast bfgt_local.f:1322-1618)
+#include "kernels.h"
+
+#include <cstdio>
+#include <stdexcept>
+
+namespace fgt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+    // c + a*b
+    return make_double2(c.x + a.x * b.x - a.y * b.y, c.y + a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
+// Separable phase of element (k1,k2,k3) for sign mask s (bit0 x, bit1 y,
+// bit2 z; bit set = conjugate), formed as (x*y)*z like the reference tables.
+__device__ __forceinline__ double2 ms_phase(const double2* w1, int s, int k1, int k2, int k3,
+                                            const Geom& g) {
+    double2 wx = g.px > 1 ? w1[k1] : make_double2(1.0, 0.0);
+    double2 wy = g.py > 1 ? w1[k2] : make_double2(1.0, 0.0);
+    double2 wz = w1[k3];
+    if (s & 1) wx = conjd(wx);
+    if (s & 2) wy = conjd(wy);
+    if (s & 4) wz = conjd(wz);
+    return cmul(cmul(wx, wy), wz);
+}
+
+__device__ __forceinline__ double2 m2l_phase(const double2* w, int npw, int idx, int k1, int k2,
+                                             int k3, const Geom& g) {
+    const int ox = idx % 3, oy = (idx / 3) % 3, oz = idx / 9;
+    double2 wx = g.px > 1 ? w[ox * npw + k1] : make_double2(1.0, 0.0);
+    double2 wy = g.py > 1 ? w[oy * npw + k2] : make_double2(1.0, 0.0);
+    double2 wz = w[oz * npw + k3];
+    return cmul(cmul(wx, wy), wz);
+}
+
+// ---------------------------------------------------------------- STEP 1
+__global__ void __launch_bounds__(kBlock) k_p2pw(Geom g, DevTables t, const int* __restrict__ box,
+                                                 const int* __restrict__ lev,
+                                                 const int* __restrict__ slot,
+                                                 const double* __restrict__ fvals,
+                                                 double2* __restrict__ expn) {
+    extern __shared__ double2 sm[];
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], s = slot[task];
+    const int n = t.n, npw = t.npw;
+    double2* sT = sm;                    // [npw][n]
+    double2* ff = sT + npw * n;          // [m2][m1]  nx*ny
+    double2* ff2 = ff + g.nx * g.ny;     // [k2][m1]  py*nx
+    const double2* T = t.p2pw + (size_t)l * npw * n;
+    for (int i = threadIdx.x; i < npw * n; i += blockDim.x) sT[i] = T[i];
+    const int nxy = g.nx * g.ny;
+    const int m1 = threadIdx.x % g.nx, m2 = threadIdx.x / g.nx;
+    double freg[16];
+    const double one = 1.0;
+    for (int ind = 0; ind < g.nd; ++ind) {
+        __syncthreads();
+        if ((int)threadIdx.x < nxy)
+            for (int m3 = 0; m3 < g.nz; ++m3)
+                freg[m3] = fvals[((size_t)b * g.npbox + m1 + g.nx * (m2 + g.ny * m3)) * g.nd + ind];
+        double2* E = expn + ((size_t)s * 2 * g.nd + ind) * g.nexp;
+        for (int k3 = 0; k3 < g.pz; ++k3) {
+            if ((int)threadIdx.x < nxy) {
+                double2 acc = make_double2(0.0, 0.0);
+                for (int m3 = 0; m3 < g.nz; ++m3) {
+                    const double2 tz = sT[k3 * n + m3];
+                    acc.x += tz.x * freg[m3];
+                    acc.y += tz.y * freg[m3];
+                }
+                ff[m2 * g.nx + m1] = acc;
+            }
+            __syncthreads();
+            for (int o = threadIdx.x; o < g.py * g.nx; o += blockDim.x) {
+                const int a1 = o % g.nx, k2 = o / g.nx;
+                double2 acc = make_double2(0.0, 0.0);
+                if (g.ny == 1) {
+                    acc = ff[a1];
+                } else {
+                    for (int b2 = 0; b2 < g.ny; ++b2) acc = cfma(sT[k2 * n + b2], ff[b2 * g.nx + a1], acc);
+                }
+                ff2[k2 * g.nx + a1] = acc;
+            }
+            __syncthreads();
+            double2* Ek = E + (size_t)k3 * g.px * g.py;
+            for (int o = threadIdx.x; o < g.px * g.py; o += blockDim.x) {
+                const int k2 = o % g.py, k1 = o / g.py;
+                double2 acc = make_double2(0.0, 0.0);
+                if (g.nx == 1) {
+                    acc = ff2[k2];
+                } else {
+                    for (int a1 = 0; a1 < g.nx; ++a1) acc = cfma(sT[k1 * n + a1], ff2[k2 * g.nx + a1], acc);
+                }
+                Ek[o] = acc;
+            }
+            __syncthreads();
+        }
+    }
+    (void)one;
+}
+
+// ---------------------------------------------------------------- STEP 2
+__global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int* __restrict__ par_slot,
+                                                  const int* __restrict__ par_first,
+                                                  const int* __restrict__ par_count,
+                                                  const int* __restrict__ pc_child,
+                                                  const int* __restrict__ pc_sign, int klev,
+                                                  double2* __restrict__ expn) {
+    const int task = blockIdx.y;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)g.nd * g.nexp;
+    if (e >= tot) return;
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
+    const int first = par_first[task], cnt = par_count[task];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < cnt; ++i) {
+        const int cs = pc_child[first + i];
+        const double2 v = expn[(size_t)cs * 2 * tot + e];
+        acc = cfma(v, ms_phase(w1, pc_sign[first + i], k1, k2, k3, g), acc);
+    }
+    expn[(size_t)par_slot[task] * 2 * tot + e] = acc;
+}
+
+// ---------------------------------------------------------------- STEP 3
+__global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, const int* __restrict__ tgt,
+                                                const int* __restrict__ off,
+                                                const int* __restrict__ src,
+                                                const int* __restrict__ idx,
+                                                double2* __restrict__ expn) {
+    const int task = blockIdx.y;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)g.nd * g.nexp;
+    if (e >= tot) return;
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    const int ts = tgt[task];
+    double2 acc = expn[(size_t)ts * 2 * tot + e];
+    for (int j = off[task]; j < off[task + 1]; ++j) {
+        const double2 v = expn[(size_t)src[j] * 2 * tot + e];
+        acc = cfma(v, m2l_phase(t.wm2l, t.npw, idx[j], k1, k2, k3, g), acc);
+    }
+    expn[((size_t)ts * 2 + 1) * tot + e] = acc;
+}
+
+// ---------------------------------------------------------------- STEP 4
+__global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
+                                                  const int* __restrict__ pc_child,
+                                                  const int* __restrict__ pc_sign, int klev,
+                                                  double2* __restrict__ expn) {
+    const int entry = blockIdx.y;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)g.nd * g.nexp;
+    if (e >= tot) return;
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
+    const double2 v = expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e];
+    expn[((size_t)pc_child[entry] * 2 + 1) * tot + e] = cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g));
+}
+
+// ---------------------------------------------------------------- STEP 5
+__global__ void __launch_bounds__(kBlock) k_pw2pot(Geom g, DevTables t, const int* __restrict__ box,
+                                                   const int* __restrict__ lev,
+                                                   const int* __restrict__ slot,
+                                                   const double2* __restrict__ expn,
+                                                   double* __restrict__ pot) {
+    extern __shared__ double2 sm[];
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], s = slot[task];
+    const int n = t.n, npw = t.npw;
+    double2* sU = sm;                 // [n][npw]
+    double2* ff = sU + n * npw;       // [kk1][k2]  nx*py
+    const double2* U = t.pw2pot + (size_t)l * n * npw;
+    for (int i = threadIdx.x; i < n * npw; i += blockDim.x) sU[i] = U[i];
+    const int nxy = g.nx * g.ny;
+    const int kk1 = threadIdx.x % g.nx, kk2 = threadIdx.x / g.nx;
+    const bool halve_last = (npw % 2) == 1;
+    for (int ind = 0; ind < g.nd; ++ind) {
+        double acc[16];
+        for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+        const double2* E = expn + ((size_t)s * 2 * g.nd + g.nd + ind) * g.nexp;
+        for (int k3 = 0; k3 < g.pz; ++k3) {
+            __syncthreads();
+            const double2* Ek = E + (size_t)k3 * g.px * g.py;
+            for (int o = threadIdx.x; o < g.nx * g.py; o += blockDim.x) {
+                const int k2 = o % g.py, a1 = o / g.py;
+                double2 v = make_double2(0.0, 0.0);
+                if (g.nx == 1) {
+                    v = Ek[k2];
+                } else {
+                    for (int k1 = 0; k1 < g.px; ++k1) v = cfma(sU[a1 * npw + k1], Ek[(size_t)k1 * g.py + k2], v);
+                }
+                ff[a1 * g.py + k2] = v;
+            }
+            __syncthreads();
+            if ((int)threadIdx.x < nxy) {
+                double2 v = make_double2(0.0, 0.0);
+                if (g.ny == 1) {
+                    v = ff[kk1 * g.py];
+                } else {
+                    for (int k2 = 0; k2 < g.py; ++k2) v = cfma(sU[kk2 * npw + k2], ff[kk1 * g.py + k2], v);
+                }
+                const double wgt = (halve_last && k3 == g.pz - 1) ? 0.5 : 1.0;
+                for (int kk3 = 0; kk3 < g.nz; ++kk3) {
+                    const double2 u = sU[kk3 * npw + k3];
+                    acc[kk3] += (u.x * v.x - u.y * v.y) * wgt;
+                }
+            }
+        }
+        if ((int)threadIdx.x < nxy)
+            for (int kk3 = 0; kk3 < g.nz; ++kk3)
+                pot[((size_t)b * g.npbox + kk1 + g.nx * (kk2 + g.ny * kk3)) * g.nd + ind] += 2.0 * acc[kk3];
+    }
+}
+
+// ---------------------------------------------------------------- STEP 6
+// One block per target leaf; sources of its list are processed in order and
+// accumulated in registers; per source the contraction runs x, y (per z
+// slab, through LDS) and z (in registers).
+__global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const int* __restrict__ tgt,
+                                                   const int* __restrict__ lev,
+                                                   const int* __restrict__ off,
+                                                   const int* __restrict__ src,
+                                                   const int* __restrict__ code,
+                                                   const double* __restrict__ fvals,
+                                                   double* __restrict__ pot) {
+    extern __shared__ double dsm[];
+    const int task = blockIdx.x;
+    const int tb = tgt[task], l = lev[task];
+    const int n = t.n;
+    const int nxy = g.nx * g.ny;
+    double* sf = dsm;                  // npbox
+    double* h = sf + g.npbox;          // [j2][k1]  nx*ny
+    double* sA = h + nxy;              // 3 tables [k][j]  n*n each
+    int* sB = (int*)(sA + 3 * n * n);  // 3 bands [k][2]
+    const size_t nn = (size_t)n * n;
+    const double* tabl = t.loctab + (size_t)l * 13 * nn;
+    const int* bandl = t.locband + (size_t)l * 13 * 2 * n;
+    const int k1 = threadIdx.x % g.nx, k2 = threadIdx.x / g.nx;
+    for (int ind = 0; ind < g.nd; ++ind) {
+        double acc[16];
+        for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+        for (int j = off[task]; j < off[task + 1]; ++j) {
+            const int sb = src[j], c = code[j];
+            __syncthreads();
+            for (int i = threadIdx.x; i < g.npbox; i += blockDim.x)
+                sf[i] = fvals[((size_t)sb * g.npbox + i) * g.nd + ind];
+            for (int a = 0; a < 3; ++a) {
+                const int slot = (c >> (4 * a)) & 15;
+                const double* A = tabl + (size_t)slot * nn;
+                const int* B = bandl + (size_t)slot * 2 * n;
+                for (int i = threadIdx.x; i < n * n; i += blockDim.x) sA[a * nn + i] = A[i];
+                for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) sB[a * 2 * n + i] = B[i];
+            }
+            __syncthreads();
+            const double* Ax = sA;
+            const double* Ay = sA + nn;
+            const double* Az = sA + 2 * nn;
+            for (int j3 = 0; j3 < g.nz; ++j3) {
+                for (int o = threadIdx.x; o < nxy; o += blockDim.x) {
+                    const int a1 = o % g.nx, j2 = o / g.nx;
+                    double v = 0.0;
+                    const double* fs = sf + g.nx * (j2 + g.ny * j3);
+                    if (g.nx == 1) {
+                        v = fs[0];
+                    } else {
+                        const int lo = sB[2 * a1], hi = sB[2 * a1 + 1];
+                        for (int j1 = lo; j1 <= hi; ++j1) v += Ax[a1 * n + j1] * fs[j1];
+                    }
+                    h[j2 * g.nx + a1] = v;
+                }
+                __syncthreads();
+                if ((int)threadIdx.x < nxy) {
+                    double v = 0.0;
+                    if (g.ny == 1) {
+                        v = h[k1];
+                    } else {
+                        const int lo = sB[2 * n + 2 * k2], hi = sB[2 * n + 2 * k2 + 1];
+                        for (int j2 = lo; j2 <= hi; ++j2) v += Ay[k2 * n + j2] * h[j2 * g.nx + k1];
+                    }
+                    for (int k3 = 0; k3 < g.nz; ++k3) acc[k3] += Az[k3 * n + j3] * v;
+                }
+                __syncthreads();
+            }
+        }
+        if ((int)threadIdx.x < nxy)
+            for (int k3 = 0; k3 < g.nz; ++k3)
+                pot[((size_t)tb * g.npbox + k1 + g.nx * (k2 + g.ny * k3)) * g.nd + ind] += acc[k3];
+    }
+}
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+        throw std::runtime_error(buf);
+    }
+}
+
+unsigned blocks_for(long long tot) { return (unsigned)((tot + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                 const int* slot, const double* fvals, double2* expn, hipStream_t s) {
+    if (ntask <= 0) return;
+    const size_t lds = sizeof(double2) * ((size_t)t.npw * t.n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
+    hipLaunchKernelGGL(k_p2pw, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, fvals, expn);
+    check(hipGetLastError(), "k_p2pw launch");
+}
+
+void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
+                  const int* par_first, const int* par_count, const int* pc_child, const int* pc_sign,
+                  int klev, double2* expn, hipStream_t s) {
+    if (ntask <= 0) return;
+    hipLaunchKernelGGL(k_merge, dim3(blocks_for((long long)g.nd * g.nexp), ntask), dim3(kBlock), 0, s, g,
+                       t, par_slot, par_first, par_count, pc_child, pc_sign, klev, expn);
+    check(hipGetLastError(), "k_merge launch");
+}
+
+void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* off,
+                const int* src, const int* idx, double2* expn, hipStream_t s) {
+    if (ntask <= 0) return;
+    hipLaunchKernelGGL(k_m2l, dim3(blocks_for((long long)g.nd * g.nexp), ntask), dim3(kBlock), 0, s, g, t,
+                       tgt, off, src, idx, expn);
+    check(hipGetLastError(), "k_m2l launch");
+}
+
+void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
+                  const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s) {
+    if (nentry <= 0) return;
+    hipLaunchKernelGGL(k_split, dim3(blocks_for((long long)g.nd * g.nexp), nentry), dim3(kBlock), 0, s, g,
+                       t, pc_parent, pc_child, pc_sign, klev, expn);
+    check(hipGetLastError(), "k_split launch");
+}
+
+void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                   const int* slot, const double2* expn, double* pot, hipStream_t s) {
+    if (ntask <= 0) return;
+    const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
+    hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, pot);
+    check(hipGetLastError(), "k_pw2pot launch");
+}
+
+void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
+                   const int* off, const int* src, const int* code, const double* fvals, double* pot,
+                   hipStream_t s) {
+    if (ntask <= 0) return;
+    const size_t lds = sizeof(double) * ((size_t)g.npbox + (size_t)g.nx * g.ny + 3 * (size_t)t.n * t.n) +
+                       sizeof(int) * 6 * (size_t)t.n;
+    hipLaunchKernelGGL(k_direct, dim3(ntask), dim3(kBlock), lds, s, g, t, tgt, lev, off, src, code, fvals,
+                       pot);
+    check(hipGetLastError(), "k_direct launch");
+}
+
+}  // namespace fgt

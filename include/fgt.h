@@ -1,0 +1,96 @@
+/*
+ * fgt.h -- C ABI of the MI355X box Gauss transform (libfgt_amd.so).
+ *
+ * Drop-in boundary: the Fortran external subroutine boxfgt of
+ * flatironinstitute/fgt (src/bfgt/boxfgt.f:5-8, documented at README.md:61-63).
+ * The tree is built on the host beforehand, by the reference's
+ * vol_tree_mem/vol_tree_build (src/common/tree_vol_coeffs_nd.f:83-85,
+ * 444-446) or by fgt_vol_tree_mem/fgt_vol_tree_build below, which produce
+ * the same arrays.
+ *
+ * All integers are 32-bit (default Fortran INTEGER), all reals FP64, all
+ * arrays are in Fortran (column-major) layout, box ids in itree are 1-based.
+ */
+#ifndef FGT_AMD_FGT_H
+#define FGT_AMD_FGT_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * boxfgt_ -- replaces boxfgt (src/bfgt/boxfgt.f:5-280).  Same 28 by-reference
+ * arguments, gfortran/flang default mangling, so a Fortran caller links it
+ * in place of the reference.  pot(nd,npbox,nboxes) is accumulated (+=) at
+ * the tensor nodes of every leaf, as the reference does; the caller zeroes it.
+ * Supported: ifpgh = 1, ifpghtarg = 0, ifnewtree = 0.  Anything else leaves
+ * the outputs untouched and records a nonzero status (fgt_last_status); the
+ * reference has no error channel (it neither checks nor reports).
+ * timeinfo(1..20) is zeroed; (1) plan+upload, (2..7) STEP 1..6 device
+ * seconds, (8) download.
+ */
+void boxfgt_(int *nd, int *ndim, double *delta, double *eps, int *ipoly,
+             int *iperiod, int *norder, int *npbox, int *nboxes, int *nlevels,
+             int *ltree, int *itree, int *iptr, double *centers,
+             double *boxsize, double *fvals, int *ifpgh, double *pot,
+             double *grad, double *hess, int *ifnewtree, int *ntarg,
+             double *targs, int *ifpghtarg, double *pote, double *grade,
+             double *hesse, double *timeinfo);
+
+/* fgt_boxfgt -- the same entry point under a bind(C) name, for the
+ * iso_c_binding interface block in INTEGRATION.md (replaces the same
+ * src/bfgt/boxfgt.f:5-8).  Returns the status as well. */
+int fgt_boxfgt(int *nd, int *ndim, double *delta, double *eps, int *ipoly,
+               int *iperiod, int *norder, int *npbox, int *nboxes,
+               int *nlevels, int *ltree, int *itree, int *iptr,
+               double *centers, double *boxsize, double *fvals, int *ifpgh,
+               double *pot, double *grad, double *hess, int *ifnewtree,
+               int *ntarg, double *targs, int *ifpghtarg, double *pote,
+               double *grade, double *hesse, double *timeinfo);
+
+/* Status of the last call made on this thread: 0 = ok.  Copies the message
+ * (NUL-terminated, truncated to len) into msg when msg != NULL.  Added error
+ * channel; the reference has none (boxfgt.f:191 ignores its allocation
+ * status). */
+int fgt_last_status(char *msg, int len);
+
+/*
+ * Device-resident solver: the same computation as boxfgt_ split into plan
+ * (host lists and tables, device upload), data upload, execution and
+ * download, so that repeated solves on one tree keep everything in HBM.
+ * It has no single counterpart in the reference: it is bfgtmain
+ * (src/bfgt/boxfgt.f:285-1435) with its STEP 0 hoisted out of the solve.
+ */
+typedef struct fgt_solver fgt_solver;
+
+typedef struct {
+    int npwlevel, npw, nexp, nlevstart, nslots;
+    int periodic_small, all_pw;
+    long long nleaves, npw_leaves, ndirect_pairs, nout; /* nout = nd*npbox*nleaves */
+    long long expansion_bytes;
+    double stage_ms[8];   /* last execute: STEP 1..6 in [0..5] (when timed) */
+} fgt_solver_stats;
+
+int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly,
+                      int iperiod, int norder, int nboxes, int nlevels,
+                      int ltree, const int *itree, const int *iptr,
+                      const double *centers, const double *boxsize,
+                      fgt_solver **out);
+/* fvals(nd,npbox,nboxes) host -> device. */
+int fgt_solver_set_fvals(fgt_solver *s, const double *fvals);
+/* Runs STEPs 1-6 on `stream` (a hipStream_t; NULL = default stream) into the
+ * device potential (overwritten, not accumulated).  timed != 0 records
+ * per-stage events (synchronises at the end). */
+int fgt_solver_execute(fgt_solver *s, void *stream, int timed);
+/* device potential -> host; accumulate != 0 adds into pot (reference +=). */
+int fgt_solver_get_pot(fgt_solver *s, double *pot, int accumulate);
+int fgt_solver_stats_get(const fgt_solver *s, fgt_solver_stats *out);
+/* Device pointers of the resident fvals / pot buffers (for callers that
+ * fill or read HBM directly). */
+void *fgt_solver_fvals_device(fgt_solver *s);
+void *fgt_solver_pot_device(fgt_solver *s);
+void fgt_solver_destroy(fgt_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,36 @@
+/*
+ * fgt_tree.h -- host adaptive-tree builder of libfgt_amd.so.
+ *
+ * Replaces the reference's two-pass builder vol_tree_mem / vol_tree_build
+ * (src/common/tree_vol_coeffs_nd.f:83-437, 444-687) for a source given as a
+ * sum of anisotropic Gaussians, with the same refinement rule (L2 tail of
+ * the tensor-product coefficients, iptype = 2, eta = 0), the same 2:1 level
+ * restriction (vol_tree_fix_lr, 1242-1557) and the same output arrays
+ * (itree/iptr/centers/boxsize/fvals) that boxfgt_ consumes.
+ */
+#ifndef FGT_AMD_FGT_TREE_H
+#define FGT_AMD_FGT_TREE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fgt_tree_builder fgt_tree_builder;
+
+/* First pass: builds the tree in an internal buffer and reports its sizes
+ * (vol_tree_mem).  gauss: ng rows of (c[ndim], s[ndim], a); the source is
+ * f(y) = sum a exp(-sum_k (y_k - c_k)^2 / s_k), periodised over the 3^ndim
+ * nearest images when iperiod = 1.  The root box is [-boxlen/2, boxlen/2]^ndim. */
+int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder,
+                     double boxlen, int ng, const double *gauss,
+                     fgt_tree_builder **out, int *nboxes, int *nlevels,
+                     int *ltree);
+/* Second pass: copies the arrays out (vol_tree_build) and frees the builder.
+ * itree(ltree), iptr(8), centers(ndim,nboxes), boxsize(0:nlevels),
+ * fvals(npbox,nboxes). */
+int fgt_vol_tree_build(fgt_tree_builder *b, int *itree, int *iptr,
+                       double *centers, double *boxsize, double *fvals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
