@@ -1,0 +1,199 @@
+"""Benchmark: box Gauss transform throughput (output nodes/s) on one MI355X.
+
+Workload (default): 3-D periodic box FGT at FP64, Legendre nodes, norder=16,
+eps=1e-9, delta=1e-4; source = 20 anisotropic Gaussians with seeded centres
+in the unit box, per-axis widths log-uniform in [5e-4, 5e-3], amplitudes
+uniform in [-1, 1]; adaptive tree refined to eps by the library's builder
+(same arrays as the reference's vol_tree_build).  A "step" is one full
+transform (STEPs 1-6) of the resident tree: fvals are in HBM and the
+potential is written to HBM inside the timed region; the plan (tables,
+lists) is built once before it.
+
+Prints ONE JSON line on stdout (everything else goes to stderr).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), dense
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(seed=1, ng=20, n=16, eps=1e-9, delta=1e-4, iperiod=1):
+    import fgt_amd
+    from fgt_amd.tree import gauss_source
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.5, 0.5, (ng, 3))
+    s = np.exp(rng.uniform(np.log(5e-4), np.log(5e-3), (ng, 3)))
+    a = rng.uniform(-1.0, 1.0, ng)
+    t0 = time.time()
+    tree = fgt_amd.vol_tree_build(3, n, 0, iperiod, eps, gauss_source(c, s, a))
+    log(f"tree: nboxes={tree.nboxes} nlevels={tree.nlevels} leaves={len(tree.leaves)} "
+        f"build {time.time() - t0:.1f}s")
+    return tree, dict(n=n, eps=eps, delta=delta, iperiod=iperiod, ng=ng, seed=seed)
+
+
+def stage_flops_bytes(tree, st, n, nd=1):
+    """Algorithmic FLOPs / HBM bytes per stage launch (one transform).
+
+    STEP 1 and STEP 5 (per PW leaf, 3-D):  4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
+    flops (real x complex first stage, complex x complex after); bytes: read
+    fvals (8 n^3) + write the expansion (16 nexp) for STEP 1, read the
+    expansion + read/write pot for STEP 5.
+    """
+    npw, nexp = st["npw"], st["nexp"]
+    nh = (npw + 1) // 2
+    L = st["npw_leaves"]
+    f15 = 4 * n ** 3 * nh + 8 * n * n * npw * nh + 8 * n * npw * npw * nh
+    out = {
+        "p2pw": (L * nd * f15, L * nd * (8 * n ** 3 + 16 * nexp)),
+        "pw2pot": (L * nd * f15, L * nd * (16 * nexp + 16 * n ** 3)),
+    }
+    return out
+
+
+def cpu_baseline(tree, cfg):
+    """Reference boxfgt (oracle/_ref, OpenMP) on the same tree and fvals."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        from oracle import refrun
+        if not refrun.available():
+            raise RuntimeError("reference build missing")
+        refrun.lib().refglue_silence()
+        kind = "reference"
+
+        class T:
+            pass
+        rt = T()
+        for k in ("ndim", "norder", "ipoly", "iperiod", "nboxes", "nlevels", "ltree", "itree", "iptr",
+                  "centers", "boxsize", "fvals"):
+            setattr(rt, k, getattr(tree, k))
+
+        def run():
+            return refrun.boxfgt(rt, cfg["delta"], cfg["eps"])
+    except Exception as e:  # pragma: no cover - fallback when the reference .so is absent
+        log(f"cpu baseline: reference unavailable ({e}); timing the C restatement")
+        from oracle import oracle
+        kind = "port"
+
+        def run():
+            return oracle.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, tree.iperiod, tree.norder, tree.nboxes,
+                                 tree.nlevels, tree.itree, tree.iptr, tree.centers, tree.boxsize, tree.fvals)
+    # keep the Fortran runtime's stdout off our JSON line
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            run()
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+    return min(times), kind, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              f"sharding of the box FGT is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fgt_amd
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.init()
+
+    tree, cfg = make_workload()
+    n = cfg["n"]
+    t0 = time.time()
+    s = fgt_amd.Solver(1, 3, cfg["delta"], cfg["eps"], 0, cfg["iperiod"], n, tree.nboxes, tree.nlevels,
+                       tree.ltree, tree.itree, tree.iptr, tree.centers, tree.boxsize)
+    s.set_fvals(tree.fvals)
+    torch.cuda.synchronize()
+    log(f"plan+upload {time.time() - t0:.2f}s; stats {s.stats()}")
+
+    for _ in range(args.warmup):
+        s.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.execute()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    st = s.stats()
+    nout = st["nout"]
+    value = nout / (ms * 1e-3)
+
+    # per-stage device time (HIP events on the launch stream), one more step
+    s.execute(timed=True)
+    stage_ms = st2 = s.stats()["stage_ms"]
+    names = ["p2pw", "merge", "m2l", "split", "pw2pot", "direct"]
+    stage = dict(zip(names, stage_ms[:6]))
+    log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage.items()))
+    fb = stage_flops_bytes(tree, st2, n)
+    dom = max(("p2pw", "pw2pot"), key=lambda k: stage[k])
+    flops, nbytes = fb[dom]
+    achieved = flops / (stage[dom] * 1e-3) / 1e12
+    roofline = {"kernel": f"k_{dom}", "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
+                "stage_ms": {k: round(v, 4) for k, v in stage.items()}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        secs, kind, threads = cpu_baseline(tree, cfg)
+        cpu = {"value": round(nout / secs, 1), "unit": "output nodes/s", "cores": threads, "kind": kind,
+               "sample": f"the full workload ({nout} output nodes), one boxfgt call, best of 2, "
+                         f"{secs:.2f}s"}
+        log(f"cpu baseline: {secs:.2f}s ({kind}, {threads} threads)")
+
+    out = {
+        "metric": "output nodes/sec (3-D box FGT, plain FP64 drop-in, one problem)",
+        "value": round(value, 1),
+        "unit": "output nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian sources)",
+        "config": {"workload": "C2: 3-D periodic boxfgt FP64, Legendre, norder=16, eps=1e-9, delta=1e-4, "
+                               "20 anisotropic Gaussians (seed 1)",
+                   "nboxes": tree.nboxes, "nleaves": int(st["nleaves"]), "nlevels": tree.nlevels,
+                   "output_nodes": nout, "npwlevel": st["npwlevel"], "npw": st["npw"],
+                   "pw_leaves": int(st["npw_leaves"]), "direct_pairs": int(st["ndirect_pairs"])},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

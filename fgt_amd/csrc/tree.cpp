@@ -80,13 +80,71 @@ struct Builder {
         for (int j = 0; j < mc; ++j) child.push_back(-1);
         for (int k = 0; k < ndim; ++k) centers.push_back(cen[k]);
         iflag.push_back(0);
+        bsize_of.push_back(bs);
         fvals.resize(static_cast<size_t>(b + 1) * npbox);
-        double y[3];
-        for (int l = 0; l < npbox; ++l) {
-            for (int k = 0; k < ndim; ++k) y[k] = cen[k] + grid[ndim * l + k] * bs;
-            fvals[static_cast<size_t>(b) * npbox + l] = source(y);
-        }
         return b;
+    }
+
+    // Node values of every box created since the last call, boxes in
+    // parallel.  Gaussian images whose exponent exceeds 750 on the whole box
+    // are skipped: exp underflows to exactly +0 there, so the sums are
+    // unchanged bit for bit.
+    std::vector<double> bsize_of;
+    int evaluated = 0;
+    void flush() {
+        const int n = nb();
+        const int stride = 2 * ndim + 1;
+        int nimg = 1;
+        for (int k = 0; k < ndim; ++k) nimg *= iperiod ? 3 : 1;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int b = evaluated; b < n; ++b) {
+            const double bs = bsize_of[b];
+            const double* cen = &centers[static_cast<size_t>(ndim) * b];
+            std::vector<int> live;
+            live.reserve(static_cast<size_t>(ng) * nimg);
+            for (int g = 0; g < ng; ++g) {
+                const double* p = &gauss[static_cast<size_t>(g) * stride];
+                for (int im = 0; im < nimg; ++im) {
+                    int code = im;
+                    double lb = 0.0;
+                    for (int k = 0; k < ndim; ++k) {
+                        double shift = 0.0;
+                        if (iperiod) {
+                            shift = static_cast<double>(code % 3 - 1) * boxlen;
+                            code /= 3;
+                        }
+                        const double d = std::fabs(cen[k] - p[k] - shift) - bs / 2;
+                        if (d > 0) lb += d * d / p[ndim + k];
+                    }
+                    live.push_back(lb < 750.0 ? 1 : 0);
+                }
+            }
+            double y[3];
+            for (int l = 0; l < npbox; ++l) {
+                for (int k = 0; k < ndim; ++k) y[k] = cen[k] + grid[ndim * l + k] * bs;
+                double acc = 0.0;
+                for (int g = 0; g < ng; ++g) {
+                    const double* p = &gauss[static_cast<size_t>(g) * stride];
+                    for (int im = 0; im < nimg; ++im) {
+                        if (!live[static_cast<size_t>(g) * nimg + im]) continue;
+                        int code = im;
+                        double rr = 0.0;
+                        for (int k = 0; k < ndim; ++k) {
+                            double shift = 0.0;
+                            if (iperiod) {
+                                shift = static_cast<double>(code % 3 - 1) * boxlen;
+                                code /= 3;
+                            }
+                            const double d = y[k] - p[k] - shift;
+                            rr += d * d / p[ndim + k];
+                        }
+                        acc += p[2 * ndim] * std::exp(-rr);
+                    }
+                }
+                fvals[static_cast<size_t>(b) * npbox + l] = acc;
+            }
+        }
+        evaluated = n;
     }
 
     // children of b at level lev+1 (box size bs); sign of axis k for child j
@@ -104,7 +162,8 @@ struct Builder {
     }
 
     // sum over tensor nodes of f^2 w (scaled) -- the box's squared L2 mass
-    double box_l2sq(int b, double rsc) const {
+    double box_l2sq(int b, double rsc) {
+        flush();
         double s = 0;
         for (int l = 0; l < npbox; ++l) {
             const double f = fvals[static_cast<size_t>(b) * npbox + l];
@@ -114,7 +173,8 @@ struct Builder {
     }
 
     // L2 norm of the tail coefficients of box b
-    double tail_error(int b) const {
+    double tail_error(int b) {
+        flush();
         const int n = norder;
         std::vector<double> a(fvals.begin() + static_cast<size_t>(b) * npbox,
                               fvals.begin() + static_cast<size_t>(b + 1) * npbox);
@@ -185,6 +245,7 @@ struct Builder {
     // Renumber so that each level's boxes are contiguous: per level the
     // boxes of [lfirst,llast] then those of the tail range (vol_tree_reorg).
     void reorg(const std::vector<int>& tfirst, const std::vector<int>& tlast) {
+        flush();
         const int n = nb();
         std::vector<int> newid(n, -1);
         std::vector<int> order;
@@ -217,6 +278,7 @@ struct Builder {
         perm_i(iflag, 1);
         perm_d(centers, ndim);
         perm_d(fvals, npbox);
+        perm_d(bsize_of, 1);
         for (auto& p : parent)
             if (p >= 0) p = newid[p];
         for (auto& c : child)
@@ -416,6 +478,7 @@ struct Builder {
         nlevels = lev;
         computecoll();
         if (nlevels >= 2) fix_lr();
+        flush();
     }
 };
 
