@@ -148,7 +148,8 @@ def main():
 
     # per-stage device time (HIP events on the launch stream), one more step
     s.execute(timed=True)
-    stage_ms = st2 = s.stats()["stage_ms"]
+    st2 = s.stats()
+    stage_ms = st2["stage_ms"]
     names = ["p2pw", "merge", "m2l", "split", "pw2pot", "direct"]
     stage = dict(zip(names, stage_ms[:6]))
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage.items()))
