@@ -2,6 +2,7 @@
 // device-resident solver.  Host orchestration of the stage kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -57,14 +58,14 @@ struct fgt_solver {
     fgt::Geom geom{};
     fgt::DevTables tabs{};
     // tables
-    DBuf d_p2pw, d_pw2pot, d_wms, d_wm2l, d_one, d_loctab, d_locband;
+    DBuf d_p2pw, d_pw2pot, d_pw2potT, d_wms, d_wm2l, d_one, d_loctab, d_locband;
     // data
     DBuf d_fvals, d_pot, d_expn;
     // lists
     DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
-    DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code;
+    DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     std::vector<int> leaf_level_off;   // per level, offsets into the leaf lists
     double stage_ms[8] = {0};
 };
@@ -136,6 +137,12 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     if (P.do_pw) {
         S.d_p2pw.upload(to_d2(P.p2pw));
         S.d_pw2pot.upload(to_d2(P.pw2pot));
+        std::vector<fgt::cplx> tr(P.pw2pot.size());
+        const size_t per = (size_t)n * P.npw;
+        for (size_t l = 0; l * per < P.pw2pot.size(); ++l)
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < P.npw; ++k) tr[l * per + (size_t)k * n + i] = P.pw2pot[l * per + (size_t)i * P.npw + k];
+        S.d_pw2potT.upload(to_d2(tr));
         S.d_wms.upload(to_d2(P.wms));
         S.d_wm2l.upload(to_d2(P.wm2l));
         S.d_expn.alloc(sizeof(double2) * 2 * (size_t)P.nd * P.nexp * P.nslots);
@@ -143,6 +150,8 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     fgt::DevTables& t = S.tabs;
     t.p2pw = S.d_p2pw.as<double2>();
     t.pw2pot = S.d_pw2pot.as<double2>();
+    t.pw2potT = S.d_pw2potT.as<double2>();
+    t.lds_limit = prop.sharedMemPerBlock;
     t.wms = S.d_wms.as<double2>();
     t.wm2l = S.d_wm2l.as<double2>();
     t.one = S.d_one.as<double2>();
@@ -176,10 +185,30 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     S.d_dir_tgt.upload(P.dir_tgt);
     S.d_dir_lev.upload(P.dir_lev);
     S.d_dir_off.upload(P.dir_off);
-    S.d_dir_src.upload(P.dir_src);
-    std::vector<int> dc(P.dir_code.size());
-    for (size_t i = 0; i < dc.size(); ++i) dc[i] = map_code(ndim, P.dir_code[i]);
+    // near-field entries: per target, sorted by (z, y, x) table slot so the
+    // 3-D kernel can share the y and z contractions within a group
+    std::vector<int> ds(P.dir_src.size()), dc(P.dir_code.size()), df(P.dir_code.size(), 0);
+    for (size_t t2 = 0; t2 + 1 < P.dir_off.size(); ++t2) {
+        const int a = P.dir_off[t2], b = P.dir_off[t2 + 1];
+        std::vector<int> idx(b - a);
+        for (int i = a; i < b; ++i) idx[i - a] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            const int cx = map_code(ndim, P.dir_code[x]), cy = map_code(ndim, P.dir_code[y]);
+            return (cx >> 4) < (cy >> 4);
+        });
+        for (int i = a; i < b; ++i) {
+            ds[i] = P.dir_src[idx[i - a]];
+            dc[i] = map_code(ndim, P.dir_code[idx[i - a]]);
+        }
+        for (int i = a; i < b; ++i) {
+            const bool last = (i + 1 == b);
+            if (last || (dc[i] >> 4) != (dc[i + 1] >> 4)) df[i] |= 1;
+            if (last || (dc[i] >> 8) != (dc[i + 1] >> 8)) df[i] |= 2;
+        }
+    }
+    S.d_dir_src.upload(ds);
     S.d_dir_code.upload(dc);
+    S.d_dir_flags.upload(df);
 }
 
 void execute(fgt_solver& S, hipStream_t st, bool timed) {
@@ -226,7 +255,7 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
     // STEP 6
     fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
                        S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
-                       S.d_fvals.as<double>(), S.d_pot.as<double>(), st);
+                       S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), S.d_pot.as<double>(), st);
     mark(6);
     if (timed) {
         hcheck(hipEventSynchronize(ev[6]), "hipEventSynchronize");

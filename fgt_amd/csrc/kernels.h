@@ -28,8 +28,10 @@ struct Geom {
 struct DevTables {
     // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)
     const double2* p2pw;
-    // per level l: PW->value  U[l][i][k] (i < n, k < npw)
+    // per level l: PW->value  U[l][i][k] (i < n, k < npw), and transposed
+    // UT[l][k][i] (the register-blocked 3-D kernels read it wave-uniformly)
     const double2* pw2pot;
+    const double2* pw2potT;
     // merge/split 1-D phases wms[klev-1][k], M2L 1-D phases wm2l[o][k]
     const double2* wms;
     const double2* wm2l;
@@ -40,7 +42,15 @@ struct DevTables {
     const int* locband;
     int n;     // norder
     int npw;
+    size_t lds_limit;  // device LDS bytes per block
 };
+
+// Register-blocked 3-D STEP 1 / STEP 5 kernels (norder 8, 12, 16); return
+// false (nothing launched) when the shape is not covered.
+bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                      const int* slot, const double2* expn, double* pot, hipStream_t s, size_t lds_limit);
 
 // Stage launchers (stream-ordered; all arguments device pointers).
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
@@ -58,8 +68,13 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
                    const int* lev, const int* slot, const double2* expn,
                    double* pot, hipStream_t s);
+// flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
+// last of its z group; entries must then be grouped in that order.
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                    const int* lev, const int* off, const int* src, const int* code,
-                   const double* fvals, double* pot, hipStream_t s);
+                   const int* flags, const double* fvals, double* pot, hipStream_t s);
+bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
+                      const int* off, const int* src, const int* code, const int* flags,
+                      const double* fvals, double* pot, hipStream_t s);
 
 }  // namespace fgt

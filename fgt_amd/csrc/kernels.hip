@@ -323,6 +323,10 @@ unsigned blocks_for(long long tot) { return (unsigned)((tot + kBlock - 1) / kBlo
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                  const int* slot, const double* fvals, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
+    if (launch_p2pw_3d(g, t, ntask, box, lev, slot, fvals, expn, s, t.lds_limit)) {
+        check(hipGetLastError(), "k_p2pw3 launch");
+        return;
+    }
     const size_t lds = sizeof(double2) * ((size_t)t.npw * t.n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
     hipLaunchKernelGGL(k_p2pw, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, fvals, expn);
     check(hipGetLastError(), "k_p2pw launch");
@@ -356,15 +360,23 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                    const int* slot, const double2* expn, double* pot, hipStream_t s) {
     if (ntask <= 0) return;
+    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, expn, pot, s, t.lds_limit)) {
+        check(hipGetLastError(), "k_pw2pot3 launch");
+        return;
+    }
     const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
     hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, pot);
     check(hipGetLastError(), "k_pw2pot launch");
 }
 
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
-                   const int* off, const int* src, const int* code, const double* fvals, double* pot,
-                   hipStream_t s) {
+                   const int* off, const int* src, const int* code, const int* flags, const double* fvals,
+                   double* pot, hipStream_t s) {
     if (ntask <= 0) return;
+    if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s)) {
+        check(hipGetLastError(), "k_direct3 launch");
+        return;
+    }
     const size_t lds = sizeof(double) * ((size_t)g.npbox + (size_t)g.nx * g.ny + 3 * (size_t)t.n * t.n) +
                        sizeof(int) * 6 * (size_t)t.n;
     hipLaunchKernelGGL(k_direct, dim3(ntask), dim3(kBlock), lds, s, g, t, tgt, lev, off, src, code, fvals,

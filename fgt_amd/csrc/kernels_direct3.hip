@@ -1,0 +1,144 @@
+// kernels_direct3.hip -- 3-D near-field (STEP 6) kernel for gfx950 (FP64).
+//
+// Replaces the per-pair banded contraction of g3d_tens_prod_to_potloc_fast
+// (src/bfgt/bfgt_local.f:1322-1618) summed over a target's list-1 sources
+// (boxfgt.f:904-928).  Sources of one target are pre-sorted on the host by
+// their (z, y, x) table slots; since the near-field operator is the
+// separable product A_z (x) A_y (x) A_x, sources that share the y and z
+// tables are summed after their x contraction, and y-results that share the
+// z table are summed before it:
+//     pot += sum_z A_z ( sum_{y | z} A_y ( sum_{x | y,z} A_x f_s ) )
+// which needs one x contraction per source, one y contraction per (y,z)
+// group and one z contraction per z group.
+// One 256-thread block per target leaf; the 1-D tables (masked to their
+// significant band, zero elsewhere) are wave-uniform and read into SGPRs;
+// the partial sums live in registers between LDS transposes.
+#include "kernels.h"
+
+namespace fgt {
+
+namespace {
+
+// flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group
+template <int N>
+__global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restrict__ loctab,
+                                                const int* __restrict__ tgt, const int* __restrict__ lev,
+                                                const int* __restrict__ off, const int* __restrict__ src,
+                                                const int* __restrict__ code, const int* __restrict__ flags,
+                                                const double* __restrict__ fvals, double* __restrict__ pot) {
+    extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
+    constexpr int NN = N * N;
+    const int task = blockIdx.x;
+    const int tb = tgt[task], l = lev[task];
+    const double* __restrict__ tabl = loctab + (size_t)l * 13 * NN;
+    const int tid = threadIdx.x;
+    const bool act = tid < NN;
+    // x stage: thread = (j2, j3); y stage: thread = (k1, j3); z stage: thread = (k1, k2)
+    const int a = tid % N, bq = tid / N;
+    const int nd = g.nd;
+    const int j0 = off[task], j1e = off[task + 1];
+    for (int ind = 0; ind < nd; ++ind) {
+        double X[N], Y[N], acc[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) X[i] = Y[i] = acc[i] = 0.0;
+        for (int j = j0; j < j1e; ++j) {
+            const int sb = src[j], c = code[j], fl = flags[j];
+            const double* __restrict__ Ax = tabl + (size_t)(c & 15) * NN;
+            // ---- x: X[k1] += sum_j1 Ax[k1][j1] f[j1, j2=a, j3=bq]
+            if (act) {
+                double f[N];
+                const double* fs = fvals + ((size_t)sb * g.npbox + (size_t)N * tid) * nd + ind;
+#pragma unroll
+                for (int i = 0; i < N; ++i) f[i] = fs[(size_t)i * nd];
+#pragma unroll
+                for (int k1 = 0; k1 < N; ++k1) {
+                    double s = X[k1];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) s += Ax[k1 * N + i] * f[i];
+                    X[k1] = s;
+                }
+            }
+            if (fl & 1) {
+                // ---- y: Y[k2] += sum_j2 Ay[k2][j2] X[k1, j2, j3]
+                const double* __restrict__ Ay = tabl + (size_t)((c >> 4) & 15) * NN;
+                __syncthreads();
+                if (act) {
+#pragma unroll
+                    for (int k1 = 0; k1 < N; ++k1) dsm[(bq * N + a) * N + k1] = X[k1];   // [j3][j2][k1]
+                }
+                __syncthreads();
+                if (act) {
+                    double xv[N];   // X[k1=a, j2, j3=bq]
+#pragma unroll
+                    for (int j2 = 0; j2 < N; ++j2) xv[j2] = dsm[(bq * N + j2) * N + a];
+#pragma unroll
+                    for (int k2 = 0; k2 < N; ++k2) {
+                        double s = Y[k2];
+#pragma unroll
+                        for (int j2 = 0; j2 < N; ++j2) s += Ay[k2 * N + j2] * xv[j2];
+                        Y[k2] = s;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < N; ++i) X[i] = 0.0;
+                if (fl & 2) {
+                    // ---- z: acc[k3] += sum_j3 Az[k3][j3] Y[k1, k2, j3]
+                    const double* __restrict__ Az = tabl + (size_t)((c >> 8) & 15) * NN;
+                    __syncthreads();
+                    if (act) {
+#pragma unroll
+                        for (int k2 = 0; k2 < N; ++k2) dsm[(bq * N + k2) * N + a] = Y[k2];   // [j3][k2][k1]
+                    }
+                    __syncthreads();
+                    if (act) {
+                        double yv[N];   // Y[k1=a, k2=bq, j3]
+#pragma unroll
+                        for (int j3 = 0; j3 < N; ++j3) yv[j3] = dsm[(j3 * N + bq) * N + a];
+#pragma unroll
+                        for (int k3 = 0; k3 < N; ++k3) {
+                            double s = acc[k3];
+#pragma unroll
+                            for (int j3 = 0; j3 < N; ++j3) s += Az[k3 * N + j3] * yv[j3];
+                            acc[k3] = s;
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < N; ++i) Y[i] = 0.0;
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int k3 = 0; k3 < N; ++k3)
+                pot[((size_t)tb * g.npbox + a + N * (bq + N * k3)) * nd + ind] += acc[k3];
+        }
+        __syncthreads();
+    }
+}
+
+template <int N>
+bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev, const int* off,
+                 const int* src, const int* code, const int* flags, const double* fvals, double* pot,
+                 hipStream_t s) {
+    const size_t lds = sizeof(double) * (size_t)N * N * N;
+    if (lds > t.lds_limit) return false;
+    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, tgt, lev, off, src, code,
+                       flags, fvals, pot);
+    return true;
+}
+
+}  // namespace
+
+bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
+                      const int* off, const int* src, const int* code, const int* flags,
+                      const double* fvals, double* pot, hipStream_t s) {
+    if (g.ndim != 3 || ntask <= 0 || !flags) return false;
+    switch (t.n) {
+        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
+        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
+        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
+        default: return false;
+    }
+}
+
+}  // namespace fgt

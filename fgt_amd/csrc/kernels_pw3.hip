@@ -1,0 +1,267 @@
+// kernels_pw3.hip -- register-blocked 3-D plane-wave kernels for gfx950 (FP64).
+//
+// STEP 1 (values -> PW, g3d_tens_prod_to_pw, bfgt_pwrouts.f:153-222) and
+// STEP 5 (PW -> values, g3d_pw2pot, bfgt_pwrouts.f:719-790) for norder
+// N in {8, 12, 16}, one 256-thread block (4 waves) per leaf.
+//
+// Design: every contraction is arranged so that the 1-D table operand is
+// wave-uniform (its address depends only on loop counters and the wave
+// index), which lets it come from the scalar cache into SGPRs; the other
+// operand lives in VGPRs, so each v_fma_f64 has one scalar and one vector
+// source and LDS only carries the transposes between stages.  The half
+// (z) axis is processed in rounds of four slabs, one per wave.
+#include "kernels.h"
+
+namespace fgt {
+
+namespace {
+
+__device__ __forceinline__ double2 zfma(double2 t, double2 f, double2 c) {
+    return make_double2(c.x + t.x * f.x - t.y * f.y, c.y + t.x * f.y + t.y * f.x);
+}
+
+// ---------------------------------------------------------------- STEP 1
+// LDS: sff [m2][s][m1] (N*4*N complex), sff2 [s][m1][k2] (4*N*npw complex)
+template <int N>
+__global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab, int npw,
+                                              const int* __restrict__ box, const int* __restrict__ lev,
+                                              const int* __restrict__ slot,
+                                              const double* __restrict__ fvals,
+                                              double2* __restrict__ expn) {
+    extern __shared__ double2 sm[];
+    double2* sff = sm;
+    double2* sff2 = sff + 4 * N * N;
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], sl = slot[task];
+    const double2* __restrict__ T = tab + (size_t)l * npw * N;   // T[k][m]
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int nh = g.pz, nd = g.nd;
+    for (int ind = 0; ind < nd; ++ind) {
+        double fr[N];
+        if (tid < N * N) {
+#pragma unroll
+            for (int m3 = 0; m3 < N; ++m3)
+                fr[m3] = fvals[((size_t)b * g.npbox + tid + N * N * m3) * nd + ind];
+        }
+        double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
+        for (int r = 0; r < nh; r += 4) {
+            // z: ff[s][m1][m2] = sum_m3 T[k3_s][m3] f[m1,m2,m3], thread = (m1,m2)
+            if (tid < N * N) {
+                const int m1 = tid % N, m2 = tid / N;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int k3 = r + s;
+                    if (k3 < nh) {
+                        double ar = 0.0, ai = 0.0;
+#pragma unroll
+                        for (int m3 = 0; m3 < N; ++m3) {
+                            const double2 t = T[k3 * N + m3];
+                            ar += t.x * fr[m3];
+                            ai += t.y * fr[m3];
+                        }
+                        sff[(m2 * 4 + s) * N + m1] = make_double2(ar, ai);
+                    }
+                }
+            }
+            __syncthreads();
+            // y: ff2[s][m1][k2] = sum_m2 T[k2][m2] ff[s][m1][m2]; lane = (m1, s), wave strides k2
+            {
+                const int m1 = lane & 15, s = lane >> 4;
+                const bool act = (m1 < N) && (r + s < nh);
+                double2 ff[N];
+#pragma unroll
+                for (int m2 = 0; m2 < N; ++m2)
+                    ff[m2] = act ? sff[(m2 * 4 + s) * N + m1] : make_double2(0.0, 0.0);
+                for (int k2 = wave; k2 < npw; k2 += 4) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(T[k2 * N + m2], ff[m2], acc);
+                    if (act) sff2[(s * N + m1) * npw + k2] = acc;
+                }
+            }
+            __syncthreads();
+            // x: E[k3][k1][k2] = sum_m1 T[k1][m1] ff2[s][m1][k2]; wave = slab, lane = k2
+            {
+                const int k3 = r + wave;
+                if (k3 < nh) {
+                    double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    for (int c = 0; c < npw; c += 64) {
+                        const int k2 = c + lane;
+                        const bool act = k2 < npw;
+                        double2 f2[N];
+#pragma unroll
+                        for (int m1 = 0; m1 < N; ++m1)
+                            f2[m1] = act ? sff2[(wave * N + m1) * npw + k2] : make_double2(0.0, 0.0);
+                        for (int k1 = 0; k1 < npw; ++k1) {
+                            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(T[k1 * N + m1], f2[m1], acc);
+                            if (act) Ek[(size_t)k1 * npw + k2] = acc;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------- STEP 5
+// tabT[l][k][i] = pw2pot table transposed (frequency k, node i).
+// LDS: sff [k2][s][16] (npw*64 complex), sU [i][k] (N*npw complex).
+template <int N>
+__global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restrict__ tabT, int npw,
+                                                const int* __restrict__ box, const int* __restrict__ lev,
+                                                const int* __restrict__ slot,
+                                                const double2* __restrict__ expn,
+                                                double* __restrict__ pot) {
+    extern __shared__ double2 sm[];
+    double2* sff = sm;
+    double2* sU = sff + (size_t)npw * 64;
+    constexpr int NQ = N / 4;
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], sl = slot[task];
+    const double2* __restrict__ UT = tabT + (size_t)l * npw * N;   // UT[k][i] = U[i][k]
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int nh = g.pz, nd = g.nd;
+    for (int i = tid; i < N * npw; i += blockDim.x) {
+        const int k = i / N, ii = i % N;
+        sU[ii * npw + k] = UT[i];
+    }
+    const int kk1 = lane & 15, s = lane >> 4;
+    const bool halve = (npw & 1) != 0;
+    for (int ind = 0; ind < nd; ++ind) {
+        double G[NQ][N];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int k = 0; k < N; ++k) G[q][k] = 0.0;
+        const double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
+        for (int r = 0; r < nh; r += 4) {
+            __syncthreads();
+            // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2
+            {
+                const int k3 = r + wave;
+                if (k3 < nh) {
+                    const double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    for (int c = 0; c < npw; c += 64) {
+                        const int k2 = c + lane;
+                        const bool act = k2 < npw;
+                        double2 acc[N];
+#pragma unroll
+                        for (int k = 0; k < N; ++k) acc[k] = make_double2(0.0, 0.0);
+                        int k1 = 0;
+                        for (; k1 + 4 <= npw; k1 += 4) {
+                            double2 e[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                e[u] = act ? Ek[(size_t)(k1 + u) * npw + k2] : make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                                for (int k = 0; k < N; ++k) acc[k] = zfma(UT[(k1 + u) * N + k], e[u], acc[k]);
+                        }
+                        for (; k1 < npw; ++k1) {
+                            const double2 e = act ? Ek[(size_t)k1 * npw + k2] : make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int k = 0; k < N; ++k) acc[k] = zfma(UT[k1 * N + k], e, acc[k]);
+                        }
+                        if (act)
+#pragma unroll
+                            for (int k = 0; k < N; ++k) sff[(k2 * 4 + wave) * 16 + k] = acc[k];
+                    }
+                }
+            }
+            __syncthreads();
+            // y: ff2[s][kk1][kk2] = sum_k2 U[kk2][k2] ff[k2][s][kk1]; lane = (kk1, s),
+            // wave owns kk2 in [wave*NQ, wave*NQ+NQ); then z: G += Re(U[kk3][k3] ff2)
+            const int k3 = r + s;
+            if (kk1 < N && k3 < nh) {
+                double2 y[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
+                for (int k2 = 0; k2 < npw; ++k2) {
+                    const double2 f = sff[(k2 * 4 + s) * 16 + kk1];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) y[q] = zfma(UT[k2 * N + wave * NQ + q], f, y[q]);
+                }
+                const double w = (halve && k3 == nh - 1) ? 0.5 : 1.0;
+#pragma unroll
+                for (int kk3 = 0; kk3 < N; ++kk3) {
+                    const double2 u = sU[kk3 * npw + k3];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) G[q][kk3] += (u.x * y[q].x - u.y * y[q].y) * w;
+                }
+            }
+        }
+        // reduce the four slab groups (lanes kk1 + 16 s) and store
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                double v = G[q][k];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                G[q][k] = v;
+            }
+        if (s == 0 && kk1 < N) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                for (int kk3 = 0; kk3 < N; ++kk3) {
+                    const int kk2 = wave * NQ + q;
+                    pot[((size_t)b * g.npbox + kk1 + N * (kk2 + N * kk3)) * nd + ind] += 2.0 * G[q][kk3];
+                }
+        }
+    }
+}
+
+template <int N>
+bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
+               const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
+    const size_t lds = sizeof(double2) * (4 * (size_t)N * N + 4 * (size_t)N * t.npw);
+    if (lds > lds_limit) return false;
+    hipLaunchKernelGGL(k_p2pw3<N>, dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot, fvals,
+                       expn);
+    return true;
+}
+
+template <int N>
+bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
+                 const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
+    const size_t lds = sizeof(double2) * ((size_t)t.npw * 64 + (size_t)N * t.npw);
+    if (lds > lds_limit) return false;
+    hipLaunchKernelGGL(k_pw2pot3<N>, dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.npw, box, lev, slot, expn,
+                       pot);
+    return true;
+}
+
+}  // namespace
+
+bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
+    if (g.ndim != 3 || ntask <= 0) return false;
+    switch (t.n) {
+        case 8: return try_p2pw3<8>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
+        case 12: return try_p2pw3<12>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
+        case 16: return try_p2pw3<16>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
+        default: return false;
+    }
+}
+
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                      const int* slot, const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
+    if (g.ndim != 3 || ntask <= 0) return false;
+    switch (t.n) {
+        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
+        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
+        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
+        default: return false;
+    }
+}
+
+}  // namespace fgt
