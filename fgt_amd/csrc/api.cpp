@@ -58,7 +58,7 @@ struct fgt_solver {
     fgt::Geom geom{};
     fgt::DevTables tabs{};
     // tables
-    DBuf d_p2pw, d_pw2pot, d_pw2potT, d_wms, d_wm2l, d_one, d_loctab, d_locband;
+    DBuf d_p2pw, d_pw2pot, d_pw2potT, d_pw2potS, d_wms, d_wm2l, d_one, d_loctab, d_locband;
     // data
     DBuf d_fvals, d_pot, d_expn;
     // lists
@@ -66,7 +66,11 @@ struct fgt_solver {
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
-    std::vector<int> leaf_level_off;   // per level, offsets into the leaf lists
+    // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
+    bool fast3d = false;
+    DBuf d_s5_src, d_s5_sel;
+    DBuf d_pcs_parent, d_pcs_child, d_pcs_sign;   // splits into non-leaf children only
+    std::vector<int> pcs_level_off;
     double stage_ms[8] = {0};
 };
 
@@ -152,6 +156,68 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     t.pw2pot = S.d_pw2pot.as<double2>();
     t.pw2potT = S.d_pw2potT.as<double2>();
     t.lds_limit = prop.sharedMemPerBlock;
+    t.n = n;
+    t.npw = P.npw;
+    S.fast3d = P.do_pw && fgt::pw2pot_3d_supported(g, t);
+    if (S.fast3d) {
+        const int nl = P.nlevels, np = P.npw;
+        const size_t per = (size_t)n * np;
+        std::vector<fgt::cplx> ts(2 * per * (nl + 1), fgt::cplx(0.0));
+        for (int l = P.nlevstart + 1; l <= nl; ++l) {
+            const int klev = nl - (l - 1);
+            const fgt::cplx* w = &P.wms[(size_t)(klev - 1) * np];
+            for (int k = 0; k < np; ++k)
+                for (int i = 0; i < n; ++i) {
+                    const fgt::cplx u = P.pw2pot[l * per + (size_t)i * np + k];
+                    ts[(2 * (size_t)l + 0) * per + (size_t)k * n + i] = u * w[k];
+                    ts[(2 * (size_t)l + 1) * per + (size_t)k * n + i] = u * std::conj(w[k]);
+                }
+        }
+        S.d_pw2potS.upload(to_d2(ts));
+        const fgt::TreeView& T = P.tree;
+        std::vector<int> src(P.leaf_box.size()), sel(P.leaf_box.size());
+        for (size_t i = 0; i < P.leaf_box.size(); ++i) {
+            const int b = P.leaf_box[i], l = P.leaf_lev[i];
+            if (l > P.nlevstart) {
+                const int par = T.parent[b];
+                int sg = 0;
+                for (int a = 0; a < 3; ++a)
+                    if (T.centers[3 * b + a] - T.centers[3 * par + a] < 0) sg |= 1 << a;
+                src[i] = P.slot_of_box[par];
+                sel[i] = 1 | (sg << 1);
+            } else {
+                src[i] = P.leaf_slot[i];
+                sel[i] = 0;
+            }
+        }
+        S.d_s5_src.upload(src);
+        S.d_s5_sel.upload(sel);
+        // split entries, keeping non-leaf children only
+        std::vector<int> pp, pcc, psg;
+        S.pcs_level_off.assign(nl + 2, 0);
+        for (int l = 0; l <= nl; ++l) {
+            S.pcs_level_off[l] = (int)pp.size();
+            if (l < P.nlevstart || l >= nl) continue;
+            for (int b = T.lfirst[l]; b <= T.llast[l]; ++b) {
+                if (T.nchild[b] == 0) continue;
+                for (int i = 0; i < T.nchild[b]; ++i) {
+                    const int c = T.child[(size_t)T.mc * b + i];
+                    if (T.nchild[c] == 0) continue;
+                    int k = 0;
+                    for (int a = 0; a < 3; ++a)
+                        if (T.centers[3 * c + a] - T.centers[3 * b + a] < 0) k |= 1 << a;
+                    pp.push_back(P.slot_of_box[b]);
+                    pcc.push_back(P.slot_of_box[c]);
+                    psg.push_back(k);
+                }
+            }
+        }
+        S.pcs_level_off[nl + 1] = (int)pp.size();
+        S.d_pcs_parent.upload(pp);
+        S.d_pcs_child.upload(pcc);
+        S.d_pcs_sign.upload(psg);
+    }
+    t.pw2potS = S.d_pw2potS.as<double2>();
     t.wms = S.d_wms.as<double2>();
     t.wm2l = S.d_wm2l.as<double2>();
     t.one = S.d_one.as<double2>();
@@ -238,16 +304,29 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
                         S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);
         mark(3);
-        // STEP 4: coarse to fine
+        // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
+        // leaves read their parent's expansion in STEP 5 through phased tables)
         for (int l = P.nlevstart; l <= P.nlevels - 1; ++l) {
-            const int a = P.pc_level_off[l], b = P.pc_level_off[l + 1];
-            fgt::launch_split(g, t, b - a, S.d_pc_parent.as<int>() + a, S.d_pc_child.as<int>() + a,
-                              S.d_pc_sign_split.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+            if (S.fast3d) {
+                const int a = S.pcs_level_off[l], b = S.pcs_level_off[l + 1];
+                fgt::launch_split(g, t, b - a, S.d_pcs_parent.as<int>() + a, S.d_pcs_child.as<int>() + a,
+                                  S.d_pcs_sign.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+            } else {
+                const int a = P.pc_level_off[l], b = P.pc_level_off[l + 1];
+                fgt::launch_split(g, t, b - a, S.d_pc_parent.as<int>() + a, S.d_pc_child.as<int>() + a,
+                                  S.d_pc_sign_split.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+            }
         }
         mark(4);
         // STEP 5
-        fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                           S.d_leaf_slot.as<int>(), S.d_expn.as<double2>(), S.d_pot.as<double>(), st);
+        if (S.fast3d)
+            fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                               S.d_s5_src.as<int>(), S.d_s5_sel.as<int>(), S.d_expn.as<double2>(),
+                               S.d_pot.as<double>(), st);
+        else
+            fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                               S.d_leaf_slot.as<int>(), nullptr, S.d_expn.as<double2>(), S.d_pot.as<double>(),
+                               st);
         mark(5);
     } else {
         for (int i = 1; i <= 5; ++i) mark(i);

@@ -32,6 +32,9 @@ struct DevTables {
     // UT[l][k][i] (the register-blocked 3-D kernels read it wave-uniformly)
     const double2* pw2pot;
     const double2* pw2potT;
+    // pw2potT pre-multiplied by the parent->child split phase of each level:
+    // pw2potS[l][sign][k][i], sign 0 = +, 1 = conjugate (levels > nlevstart)
+    const double2* pw2potS;
     // merge/split 1-D phases wms[klev-1][k], M2L 1-D phases wm2l[o][k]
     const double2* wms;
     const double2* wm2l;
@@ -49,8 +52,11 @@ struct DevTables {
 // false (nothing launched) when the shape is not covered.
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                     const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
+// sel: per task 0 (own loc) or 1 | split-sign bits << 1 (evaluate the parent's loc)
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const double2* expn, double* pot, hipStream_t s, size_t lds_limit);
+                      const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s,
+                      size_t lds_limit);
+bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 
 // Stage launchers (stream-ordered; all arguments device pointers).
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
@@ -66,7 +72,7 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
-                   const int* lev, const int* slot, const double2* expn,
+                   const int* lev, const int* slot, const int* sel, const double2* expn,
                    double* pot, hipStream_t s);
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.

@@ -358,9 +358,9 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
 }
 
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                   const int* slot, const double2* expn, double* pot, hipStream_t s) {
+                   const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s) {
     if (ntask <= 0) return;
-    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, expn, pot, s, t.lds_limit)) {
+    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, sel, expn, pot, s, t.lds_limit)) {
         check(hipGetLastError(), "k_pw2pot3 launch");
         return;
     }

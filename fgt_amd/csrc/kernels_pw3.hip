@@ -12,9 +12,19 @@
 // (z) axis is processed in rounds of four slabs, one per wave.
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace fgt {
 
 namespace {
+
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+// range of tasks so that tasks sharing data (children of one parent,
+// neighbouring targets) meet in the same L2.
+__device__ __forceinline__ int xcd_task(int b, int nb) {
+    const int x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + min(x, rem) + i;
+}
 
 __device__ __forceinline__ double2 zfma(double2 t, double2 f, double2 c) {
     return make_double2(c.x + t.x * f.x - t.y * f.y, c.y + t.x * f.y + t.y * f.x);
@@ -111,26 +121,37 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // ---------------------------------------------------------------- STEP 5
 // tabT[l][k][i] = pw2pot table transposed (frequency k, node i).
 // LDS: sff [k2][s][16] (npw*64 complex), sU [i][k] (N*npw complex).
+// sel per task: 0 = evaluate the leaf's own local expansion with the plain
+// tables; otherwise bit0 set and bits 1..3 = split signs (x,y,z): evaluate the
+// parent's local expansion with the tables pre-multiplied by the parent->child
+// phase (tabS[l][sign][k][i]) -- STEP 4 folded into STEP 5 for leaves.
 template <int N>
-__global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restrict__ tabT, int npw,
+__global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
+                                                const double2* __restrict__ tabS, int npw,
                                                 const int* __restrict__ box, const int* __restrict__ lev,
-                                                const int* __restrict__ slot,
+                                                const int* __restrict__ slot, const int* __restrict__ sel,
                                                 const double2* __restrict__ expn,
                                                 double* __restrict__ pot) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
-    double2* sU = sff + (size_t)npw * 64;
+    const size_t sff_len = max((size_t)npw * 64, (size_t)(4 * 4 * 16 * N) / 2);
+    double2* sU = sff + sff_len;
     constexpr int NQ = N / 4;
-    const int task = blockIdx.x;
-    const int b = box[task], l = lev[task], sl = slot[task];
-    const double2* __restrict__ UT = tabT + (size_t)l * npw * N;   // UT[k][i] = U[i][k]
+    constexpr int NH = N / 2;
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    const int b = box[task], l = lev[task], sl = slot[task], se = sel[task];
+    // UT[k][i] = U[i][k] for each axis
+    const size_t per = (size_t)npw * N;
+    const double2* __restrict__ UX = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 1) & 1)) * per : tabT + l * per;
+    const double2* __restrict__ UY = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 2) & 1)) * per : tabT + l * per;
+    const double2* __restrict__ UZ = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 3) & 1)) * per : tabT + l * per;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int nh = g.pz, nd = g.nd;
     for (int i = tid; i < N * npw; i += blockDim.x) {
         const int k = i / N, ii = i % N;
-        sU[ii * npw + k] = UT[i];
+        sU[ii * npw + k] = UZ[i];
     }
     const int kk1 = lane & 15, s = lane >> 4;
     const bool halve = (npw & 1) != 0;
@@ -143,7 +164,9 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
         const double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
         for (int r = 0; r < nh; r += 4) {
             __syncthreads();
-            // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2
+            // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2.
+            // The N outputs are formed in two halves to keep register pressure
+            // low (the G accumulators stay live across this stage).
             {
                 const int k3 = r + wave;
                 if (k3 < nh) {
@@ -151,28 +174,23 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
                     for (int c = 0; c < npw; c += 64) {
                         const int k2 = c + lane;
                         const bool act = k2 < npw;
-                        double2 acc[N];
 #pragma unroll
-                        for (int k = 0; k < N; ++k) acc[k] = make_double2(0.0, 0.0);
-                        int k1 = 0;
-                        for (; k1 + 4 <= npw; k1 += 4) {
-                            double2 e[4];
+                        for (int h = 0; h < 2; ++h) {
+                            double2 acc[NH];
 #pragma unroll
-                            for (int u = 0; u < 4; ++u)
-                                e[u] = act ? Ek[(size_t)(k1 + u) * npw + k2] : make_double2(0.0, 0.0);
+                            for (int k = 0; k < NH; ++k) acc[k] = make_double2(0.0, 0.0);
+                            double2 e = act ? Ek[k2] : make_double2(0.0, 0.0);
+                            for (int k1 = 0; k1 < npw; ++k1) {
+                                const double2 en = (act && k1 + 1 < npw) ? Ek[(size_t)(k1 + 1) * npw + k2]
+                                                                         : make_double2(0.0, 0.0);
 #pragma unroll
-                            for (int u = 0; u < 4; ++u)
+                                for (int k = 0; k < NH; ++k) acc[k] = zfma(UX[k1 * N + h * NH + k], e, acc[k]);
+                                e = en;
+                            }
+                            if (act)
 #pragma unroll
-                                for (int k = 0; k < N; ++k) acc[k] = zfma(UT[(k1 + u) * N + k], e[u], acc[k]);
+                                for (int k = 0; k < NH; ++k) sff[(k2 * 4 + wave) * 16 + h * NH + k] = acc[k];
                         }
-                        for (; k1 < npw; ++k1) {
-                            const double2 e = act ? Ek[(size_t)k1 * npw + k2] : make_double2(0.0, 0.0);
-#pragma unroll
-                            for (int k = 0; k < N; ++k) acc[k] = zfma(UT[k1 * N + k], e, acc[k]);
-                        }
-                        if (act)
-#pragma unroll
-                            for (int k = 0; k < N; ++k) sff[(k2 * 4 + wave) * 16 + k] = acc[k];
                     }
                 }
             }
@@ -187,7 +205,7 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
                 for (int k2 = 0; k2 < npw; ++k2) {
                     const double2 f = sff[(k2 * 4 + s) * 16 + kk1];
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) y[q] = zfma(UT[k2 * N + wave * NQ + q], f, y[q]);
+                    for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
                 }
                 const double w = (halve && k3 == nh - 1) ? 0.5 : 1.0;
 #pragma unroll
@@ -198,25 +216,28 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
                 }
             }
         }
-        // reduce the four slab groups (lanes kk1 + 16 s) and store
+        // reduce the four slab groups through LDS, one kk2 column at a time
+        // (keeps the live register set small) and accumulate into pot
+        double* red = reinterpret_cast<double*>(sm);   // [wave][s][kk3][16]
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+        for (int q = 0; q < NQ; ++q) {
+            __syncthreads();
+            if (kk1 < N)
 #pragma unroll
-            for (int k = 0; k < N; ++k) {
-                double v = G[q][k];
-                v += __shfl_xor(v, 16);
-                v += __shfl_xor(v, 32);
-                G[q][k] = v;
-            }
-        if (s == 0 && kk1 < N) {
+                for (int k = 0; k < N; ++k) red[((wave * 4 + s) * N + k) * 16 + kk1] = G[q][k];
+            __syncthreads();
+            for (int o = tid; o < 4 * 16 * N; o += blockDim.x) {
+                const int k1o = o & 15, k = (o >> 4) % N, w2 = o / (16 * N);
+                if (k1o < N) {
+                    double v = 0.0;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-#pragma unroll
-                for (int kk3 = 0; kk3 < N; ++kk3) {
-                    const int kk2 = wave * NQ + q;
-                    pot[((size_t)b * g.npbox + kk1 + N * (kk2 + N * kk3)) * nd + ind] += 2.0 * G[q][kk3];
+                    for (int s2 = 0; s2 < 4; ++s2) v += red[((w2 * 4 + s2) * N + k) * 16 + k1o];
+                    const int kk2 = w2 * NQ + q;
+                    pot[((size_t)b * g.npbox + k1o + N * (kk2 + N * k)) * nd + ind] += 2.0 * v;
                 }
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -232,11 +253,12 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
 
 template <int N>
 bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
-                 const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
-    const size_t lds = sizeof(double2) * ((size_t)t.npw * 64 + (size_t)N * t.npw);
+                 const int* sel, const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
+    const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * N);
+    const size_t lds = sff + sizeof(double2) * (size_t)N * t.npw;
     if (lds > lds_limit) return false;
-    hipLaunchKernelGGL(k_pw2pot3<N>, dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.npw, box, lev, slot, expn,
-                       pot);
+    hipLaunchKernelGGL(k_pw2pot3<N>, dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw, box, lev,
+                       slot, sel, expn, pot);
     return true;
 }
 
@@ -253,13 +275,22 @@ bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box
     }
 }
 
+bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
+    if (g.ndim != 3 || (t.n != 8 && t.n != 12 && t.n != 16)) return false;
+    const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * t.n);
+    const size_t lds = sff + sizeof(double2) * (size_t)t.n * t.npw;
+    const size_t lds1 = sizeof(double2) * (4 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
+    return lds <= t.lds_limit && lds1 <= t.lds_limit;
+}
+
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
-    if (g.ndim != 3 || ntask <= 0) return false;
+                      const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s,
+                      size_t lds_limit) {
+    if (g.ndim != 3 || ntask <= 0 || !sel) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
-        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
-        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, expn, pot, s, lds_limit);
+        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
+        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
+        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
         default: return false;
     }
 }

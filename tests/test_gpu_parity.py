@@ -113,3 +113,23 @@ def test_solver_repeat_and_linearity():
     assert st["nleaves"] == len(c.leaves)
     assert all(x >= 0 for x in st["stage_ms"][:6])
     s.close()
+
+
+def test_c2_workload_vs_reference():
+    """The benchmark workload (3-D periodic, n=16, eps=1e-9, delta=1e-4) against
+    the reference build where it is present, else the oracle."""
+    import bench
+    from oracle import refrun
+    print("[gpu] C2 workload vs reference", flush=True)
+    tree, cfg = bench.make_workload()
+    pot = np.zeros_like(tree.fvals)
+    fgt_amd.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, 1, 16, tree.npbox, tree.nboxes, tree.nlevels, tree.ltree,
+                   tree.itree, tree.iptr, tree.centers, tree.boxsize, tree.fvals, 1, pot)
+    if refrun.available():
+        ref, _ = refrun.boxfgt(tree, cfg["delta"], cfg["eps"])
+    else:
+        ref, _, _ = oracle.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, 1, 16, tree.nboxes, tree.nlevels, tree.itree,
+                                  tree.iptr, tree.centers, tree.boxsize, tree.fvals)
+    err = rel_l2(pot, ref)
+    print(f"[gpu]   nboxes={tree.nboxes} rel l2 = {err:.3e}", flush=True)
+    assert err < 1e-12
