@@ -19,13 +19,6 @@ namespace fgt {
 
 namespace {
 
-// Workgroups go to the 8 XCDs round-robin; map them so each XCD owns a
-// contiguous range of targets (neighbouring targets share sources in L2).
-__device__ __forceinline__ int xcd_task(int b, int nb) {
-    const int x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
-    return x * per + min(x, rem) + i;
-}
-
 // flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group
 template <int N>
 __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restrict__ loctab,
@@ -35,7 +28,7 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
                                                 const double* __restrict__ fvals, double* __restrict__ pot) {
     extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
     constexpr int NN = N * N;
-    const int task = xcd_task(blockIdx.x, gridDim.x);
+    const int task = blockIdx.x;
     const int tb = tgt[task], l = lev[task];
     const double* __restrict__ tabl = loctab + (size_t)l * 13 * NN;
     const int tid = threadIdx.x;

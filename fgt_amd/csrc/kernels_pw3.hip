@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // parent's local expansion with the tables pre-multiplied by the parent->child
 // phase (tabS[l][sign][k][i]) -- STEP 4 folded into STEP 5 for leaves.
 template <int N>
-__global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
+__global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
                                                 const double2* __restrict__ tabS, int npw,
                                                 const int* __restrict__ box, const int* __restrict__ lev,
                                                 const int* __restrict__ slot, const int* __restrict__ sel,
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
     const size_t sff_len = max((size_t)npw * 64, (size_t)(4 * 4 * 16 * N) / 2);
     double2* sU = sff + sff_len;
     constexpr int NQ = N / 4;
-    constexpr int NH = N / 2;
+    constexpr int PF = (N == 12) ? 6 : 8;
     const int task = xcd_task(blockIdx.x, gridDim.x);
     const int b = box[task], l = lev[task], sl = slot[task], se = sel[task];
     // UT[k][i] = U[i][k] for each axis
@@ -165,8 +165,9 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
         for (int r = 0; r < nh; r += 4) {
             __syncthreads();
             // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2.
-            // The N outputs are formed in two halves to keep register pressure
-            // low (the G accumulators stay live across this stage).
+            // Rows of E are fetched PF at a time (all loads issued before the
+            // FMAs that use them) so enough reads are in flight to cover HBM
+            // latency; lanes past npw read a valid duplicate and discard it.
             {
                 const int k3 = r + wave;
                 if (k3 < nh) {
@@ -174,23 +175,28 @@ __global__ void __launch_bounds__(256) k_pw2pot3(Geom g, const double2* __restri
                     for (int c = 0; c < npw; c += 64) {
                         const int k2 = c + lane;
                         const bool act = k2 < npw;
+                        const int k2c = act ? k2 : npw - 1;
+                        double2 acc[N];
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            double2 acc[NH];
+                        for (int k = 0; k < N; ++k) acc[k] = make_double2(0.0, 0.0);
+                        int k1 = 0;
+                        for (; k1 + PF <= npw; k1 += PF) {
+                            double2 e[PF];
 #pragma unroll
-                            for (int k = 0; k < NH; ++k) acc[k] = make_double2(0.0, 0.0);
-                            double2 e = act ? Ek[k2] : make_double2(0.0, 0.0);
-                            for (int k1 = 0; k1 < npw; ++k1) {
-                                const double2 en = (act && k1 + 1 < npw) ? Ek[(size_t)(k1 + 1) * npw + k2]
-                                                                         : make_double2(0.0, 0.0);
+                            for (int u = 0; u < PF; ++u) e[u] = Ek[(size_t)(k1 + u) * npw + k2c];
 #pragma unroll
-                                for (int k = 0; k < NH; ++k) acc[k] = zfma(UX[k1 * N + h * NH + k], e, acc[k]);
-                                e = en;
-                            }
-                            if (act)
+                            for (int u = 0; u < PF; ++u)
 #pragma unroll
-                                for (int k = 0; k < NH; ++k) sff[(k2 * 4 + wave) * 16 + h * NH + k] = acc[k];
+                                for (int k = 0; k < N; ++k) acc[k] = zfma(UX[(k1 + u) * N + k], e[u], acc[k]);
                         }
+                        for (; k1 < npw; ++k1) {
+                            const double2 e = Ek[(size_t)k1 * npw + k2c];
+#pragma unroll
+                            for (int k = 0; k < N; ++k) acc[k] = zfma(UX[k1 * N + k], e, acc[k]);
+                        }
+                        if (act)
+#pragma unroll
+                            for (int k = 0; k < N; ++k) sff[(k2 * 4 + wave) * 16 + k] = acc[k];
                     }
                 }
             }
