@@ -162,15 +162,18 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     if (S.fast3d) {
         const int nl = P.nlevels, np = P.npw;
         const size_t per = (size_t)n * np;
-        std::vector<fgt::cplx> ts(2 * per * (nl + 1), fgt::cplx(0.0));
-        for (int l = P.nlevstart + 1; l <= nl; ++l) {
+        std::vector<fgt::cplx> ts(3 * per * (nl + 1), fgt::cplx(0.0));
+        for (int l = P.nlevstart; l <= nl; ++l) {
             const int klev = nl - (l - 1);
-            const fgt::cplx* w = &P.wms[(size_t)(klev - 1) * np];
+            const fgt::cplx* w = (l > P.nlevstart) ? &P.wms[(size_t)(klev - 1) * np] : nullptr;
             for (int k = 0; k < np; ++k)
                 for (int i = 0; i < n; ++i) {
                     const fgt::cplx u = P.pw2pot[l * per + (size_t)i * np + k];
-                    ts[(2 * (size_t)l + 0) * per + (size_t)k * n + i] = u * w[k];
-                    ts[(2 * (size_t)l + 1) * per + (size_t)k * n + i] = u * std::conj(w[k]);
+                    ts[(3 * (size_t)l + 0) * per + (size_t)k * n + i] = u;
+                    if (w) {
+                        ts[(3 * (size_t)l + 1) * per + (size_t)k * n + i] = u * w[k];
+                        ts[(3 * (size_t)l + 2) * per + (size_t)k * n + i] = u * std::conj(w[k]);
+                    }
                 }
         }
         S.d_pw2potS.upload(to_d2(ts));

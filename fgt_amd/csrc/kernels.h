@@ -32,8 +32,8 @@ struct DevTables {
     // UT[l][k][i] (the register-blocked 3-D kernels read it wave-uniformly)
     const double2* pw2pot;
     const double2* pw2potT;
-    // pw2potT pre-multiplied by the parent->child split phase of each level:
-    // pw2potS[l][sign][k][i], sign 0 = +, 1 = conjugate (levels > nlevstart)
+    // pw2potT and its products with the parent->child split phase of each
+    // level: pw2potS[l][v][k][i], v = 0 plain, 1 = + phase, 2 = conjugate phase
     const double2* pw2potS;
     // merge/split 1-D phases wms[klev-1][k], M2L 1-D phases wm2l[o][k]
     const double2* wms;

@@ -140,11 +140,20 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     constexpr int PF = (N == 12) ? 6 : 8;
     const int task = xcd_task(blockIdx.x, gridDim.x);
     const int b = box[task], l = lev[task], sl = slot[task], se = sel[task];
-    // UT[k][i] = U[i][k] for each axis
-    const size_t per = (size_t)npw * N;
-    const double2* __restrict__ UX = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 1) & 1)) * per : tabT + l * per;
-    const double2* __restrict__ UY = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 2) & 1)) * per : tabT + l * per;
-    const double2* __restrict__ UZ = (se & 1) ? tabS + (2 * (size_t)l + ((se >> 3) & 1)) * per : tabT + l * per;
+    // UT[k][i] = U[i][k] for each axis: tabS[l][v] holds v = 0 the plain
+    // table, v = 1 / 2 the table times the + / conjugate split phase.  One base
+    // pointer plus wave-uniform offsets keeps the reads on the scalar path.
+    const int per = npw * N;
+    const int vx = (se & 1) ? 1 + ((se >> 1) & 1) : 0;
+    const int vy = (se & 1) ? 1 + ((se >> 2) & 1) : 0;
+    const int vz = (se & 1) ? 1 + ((se >> 3) & 1) : 0;
+    const int ox = __builtin_amdgcn_readfirstlane((3 * l + vx) * per);
+    const int oy = __builtin_amdgcn_readfirstlane((3 * l + vy) * per);
+    const int oz = __builtin_amdgcn_readfirstlane((3 * l + vz) * per);
+    const double2* __restrict__ UX = tabS + ox;
+    const double2* __restrict__ UY = tabS + oy;
+    const double2* __restrict__ UZ = tabS + oz;
+    (void)tabT;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
