@@ -154,7 +154,17 @@ def test_loctab_brute_force():
         np.testing.assert_allclose(got.T, ref, atol=1e-12 * np.abs(ref).max(), err_msg=f"k={k}")
 
 
-@pytest.mark.parametrize("name", ["f2_n8", "p2_cheb_n12", "f1_n8", "d1_free2"])
+def test_periodic_pwnodes_match_reference():
+    keys = [k for k in TABLES.files if k.startswith("ts_") and k.endswith("_1")]
+    assert keys
+    for k in keys:
+        ts_ref, ws_ref = TABLES[k], TABLES["ws_" + k[3:]]
+        ws, ts = oracle.periodic_pwnodes(1.0, 1e-3, len(ts_ref))
+        np.testing.assert_allclose(ts, ts_ref, rtol=1e-15, atol=1e-15)
+        np.testing.assert_allclose(ws, ws_ref, rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["f2_n8", "p2_cheb_n12", "f1_n8", "d1_free2", "d1_per2"])
 def test_oracle_closed_form(name):
     """Oracle vs the exact transform of the Gaussian source (leaf nodes).
 
