@@ -69,6 +69,11 @@ struct fgt_solver {
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
     DBuf d_s5_src, d_s5_sel;
+    DBuf d_p2pwS, d_fz_lev, d_fz_slot, d_fz_kids, d_fz_ksg;          // fused STEP 1+2 parents
+    DBuf d_s1_box, d_s1_lev, d_s1_slot;                              // standalone STEP 1 leaves
+    DBuf d_mp_slot, d_mp_first, d_mp_count, d_mp_child, d_mp_sign, d_mp_acc;  // merges of non-leaf children
+    std::vector<int> mp_level_off;
+    int n_fz = 0, n_s1 = 0;
     DBuf d_pcs_parent, d_pcs_child, d_pcs_sign;   // splits into non-leaf children only
     std::vector<int> pcs_level_off;
     double stage_ms[8] = {0};
@@ -219,7 +224,92 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         S.d_pcs_parent.upload(pp);
         S.d_pcs_child.upload(pcc);
         S.d_pcs_sign.upload(psg);
+
+        // fused STEP 1+2: value->PW tables times the child->parent merge phase
+        std::vector<fgt::cplx> t1(3 * per * (nl + 1), fgt::cplx(0.0));
+        for (int l = P.nlevstart; l <= nl; ++l) {
+            const int klev = nl - (l - 1);
+            const fgt::cplx* w = (l > P.nlevstart) ? &P.wms[(size_t)(klev - 1) * np] : nullptr;
+            for (int k = 0; k < np; ++k)
+                for (int m = 0; m < n; ++m) {
+                    const fgt::cplx v = P.p2pw[l * per + (size_t)k * n + m];
+                    t1[(3 * (size_t)l + 0) * per + (size_t)k * n + m] = v;
+                    if (w) {
+                        t1[(3 * (size_t)l + 1) * per + (size_t)k * n + m] = v * w[k];
+                        t1[(3 * (size_t)l + 2) * per + (size_t)k * n + m] = v * std::conj(w[k]);
+                    }
+                }
+        }
+        S.d_p2pwS.upload(to_d2(t1));
+        std::vector<int> fzl, fzs, fzk, fzg, s1b, s1l, s1s;
+        std::vector<int> mps, mpf, mpc, mch, msg, macc;
+        S.mp_level_off.assign(nl + 2, 0);
+        for (int l = 0; l <= nl; ++l) {
+            S.mp_level_off[l] = (int)mps.size();
+            if (l < P.nlevstart) continue;
+            for (int b = T.lfirst[l]; b <= T.llast[l]; ++b) {
+                if (T.nchild[b] == 0) {
+                    if (l == P.nlevstart && P.ifpwexp[b]) {
+                        s1b.push_back(b);
+                        s1l.push_back(l);
+                        s1s.push_back(P.slot_of_box[b]);
+                    }
+                    continue;
+                }
+                if (l >= nl) continue;
+                int kids[8], ksg[8], nk = 0, nnl = 0;
+                for (int c = 0; c < 8; ++c) kids[c] = -1, ksg[c] = 0;
+                for (int i = 0; i < T.nchild[b]; ++i) {
+                    const int c = T.child[(size_t)T.mc * b + i];
+                    int k = 0;
+                    for (int a = 0; a < 3; ++a)
+                        if (T.centers[3 * b + a] - T.centers[3 * c + a] < 0) k |= 1 << a;
+                    if (T.nchild[c] == 0) {
+                        kids[nk] = c;
+                        ksg[nk] = k;
+                        ++nk;
+                    } else {
+                        if (nnl == 0) {
+                            mps.push_back(P.slot_of_box[b]);
+                            mpf.push_back((int)mch.size());
+                        }
+                        mch.push_back(P.slot_of_box[c]);
+                        msg.push_back(k);
+                        ++nnl;
+                    }
+                }
+                if (nnl > 0) {
+                    mpc.push_back(nnl);
+                    macc.push_back(nk > 0 ? 1 : 0);
+                }
+                if (nk > 0) {
+                    fzl.push_back(l);
+                    fzs.push_back(P.slot_of_box[b]);
+                    for (int c = 0; c < 8; ++c) {
+                        fzk.push_back(kids[c]);
+                        fzg.push_back(ksg[c]);
+                    }
+                }
+            }
+        }
+        S.mp_level_off[nl + 1] = (int)mps.size();
+        S.n_fz = (int)fzl.size();
+        S.n_s1 = (int)s1b.size();
+        S.d_fz_lev.upload(fzl);
+        S.d_fz_slot.upload(fzs);
+        S.d_fz_kids.upload(fzk);
+        S.d_fz_ksg.upload(fzg);
+        S.d_s1_box.upload(s1b);
+        S.d_s1_lev.upload(s1l);
+        S.d_s1_slot.upload(s1s);
+        S.d_mp_slot.upload(mps);
+        S.d_mp_first.upload(mpf);
+        S.d_mp_count.upload(mpc);
+        S.d_mp_child.upload(mch);
+        S.d_mp_sign.upload(msg);
+        S.d_mp_acc.upload(macc);
     }
+    t.p2pwS = S.d_p2pwS.as<double2>();
     t.pw2potS = S.d_pw2potS.as<double2>();
     t.wms = S.d_wms.as<double2>();
     t.wm2l = S.d_wm2l.as<double2>();
@@ -291,18 +381,40 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
     hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
     mark(0);
     if (P.do_pw) {
-        // STEP 1: every PW leaf, all levels in one launch
-        fgt::launch_p2pw(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                         S.d_leaf_slot.as<int>(), S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
-        mark(1);
-        // STEP 2: fine to coarse
-        for (int l = P.nlevels - 1; l >= P.nlevstart; --l) {
-            const int a = P.par_level_off[l], b = P.par_level_off[l + 1];
-            fgt::launch_merge(g, t, b - a, S.d_par_slot.as<int>() + a, S.d_par_first.as<int>() + a,
-                              S.d_par_count.as<int>() + a, S.d_pc_child.as<int>(), S.d_pc_sign.as<int>(),
-                              P.nlevels - l, S.d_expn.as<double2>(), st);
+        if (S.fast3d) {
+            // STEP 1: leaves at the cutoff level alone; every other PW leaf is
+            // folded into its parent's expansion (STEP 1 + 2 fused), all levels
+            // in one launch (they read only fvals)
+            fgt::launch_p2pw(g, t, S.n_s1, S.d_s1_box.as<int>(), S.d_s1_lev.as<int>(), S.d_s1_slot.as<int>(),
+                             S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
+            if (!fgt::p2pw_fused_3d(g, t, S.n_fz, S.d_fz_lev.as<int>(), S.d_fz_slot.as<int>(),
+                                    S.d_fz_kids.as<int>(), S.d_fz_ksg.as<int>(), S.d_fvals.as<double>(),
+                                    S.d_expn.as<double2>(), st))
+                throw std::runtime_error("fused STEP 1+2 kernel not available for this shape");
+            hcheck(hipGetLastError(), "k_p2pwF3 launch");
+            mark(1);
+            // STEP 2: non-leaf children, fine to coarse
+            for (int l = P.nlevels - 1; l >= P.nlevstart; --l) {
+                const int a = S.mp_level_off[l], b = S.mp_level_off[l + 1];
+                fgt::launch_merge(g, t, b - a, S.d_mp_slot.as<int>() + a, S.d_mp_first.as<int>() + a,
+                                  S.d_mp_count.as<int>() + a, S.d_mp_child.as<int>(), S.d_mp_sign.as<int>(),
+                                  S.d_mp_acc.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+            }
+            mark(2);
+        } else {
+            // STEP 1: every PW leaf, all levels in one launch
+            fgt::launch_p2pw(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                             S.d_leaf_slot.as<int>(), S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
+            mark(1);
+            // STEP 2: fine to coarse
+            for (int l = P.nlevels - 1; l >= P.nlevstart; --l) {
+                const int a = P.par_level_off[l], b = P.par_level_off[l + 1];
+                fgt::launch_merge(g, t, b - a, S.d_par_slot.as<int>() + a, S.d_par_first.as<int>() + a,
+                                  S.d_par_count.as<int>() + a, S.d_pc_child.as<int>(), S.d_pc_sign.as<int>(),
+                                  nullptr, P.nlevels - l, S.d_expn.as<double2>(), st);
+            }
+            mark(2);
         }
-        mark(2);
         // STEP 3
         fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
                         S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);

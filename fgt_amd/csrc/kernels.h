@@ -35,6 +35,9 @@ struct DevTables {
     // pw2potT and its products with the parent->child split phase of each
     // level: pw2potS[l][v][k][i], v = 0 plain, 1 = + phase, 2 = conjugate phase
     const double2* pw2potS;
+    // p2pw and its products with the child->parent merge phase of each level:
+    // p2pwS[l][v][k][m], v = 0 plain, 1 = + phase, 2 = conjugate phase
+    const double2* p2pwS;
     // merge/split 1-D phases wms[klev-1][k], M2L 1-D phases wm2l[o][k]
     const double2* wms;
     const double2* wm2l;
@@ -57,14 +60,20 @@ bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* b
                       const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s,
                       size_t lds_limit);
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
+// Fused STEP 1 + 2 for parents with leaf children (3-D, N in {8,12,16}):
+// plev/pslot per parent, kids/ksg 8 per parent (leaf child box or -1, merge
+// sign bits).  Writes (overwrites) the parent's mp; returns false if not covered.
+bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev, const int* pslot,
+                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s);
 
 // Stage launchers (stream-ordered; all arguments device pointers).
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
                  const int* lev, const int* slot, const double* fvals,
                  double2* expn, hipStream_t s);
+// par_acc (nullable): per parent, nonzero = add to the parent's current mp
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child,
-                  const int* pc_sign, int klev, double2* expn, hipStream_t s);
+                  const int* pc_sign, const int* par_acc, int klev, double2* expn, hipStream_t s);
 void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                 const int* off, const int* src, const int* idx, double2* expn,
                 hipStream_t s);

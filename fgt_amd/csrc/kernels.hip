@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
                                                   const int* __restrict__ par_first,
                                                   const int* __restrict__ par_count,
                                                   const int* __restrict__ pc_child,
-                                                  const int* __restrict__ pc_sign, int klev,
+                                                  const int* __restrict__ pc_sign,
+                                                  const int* __restrict__ par_acc, int klev,
                                                   double2* __restrict__ expn) {
     const int task = blockIdx.y;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,7 +132,9 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
     const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
     const int first = par_first[task], cnt = par_count[task];
-    double2 acc = make_double2(0.0, 0.0);
+    // par_acc[task] != 0: the parent already holds its leaf children's part
+    double2 acc = (par_acc && par_acc[task]) ? expn[(size_t)par_slot[task] * 2 * tot + e]
+                                             : make_double2(0.0, 0.0);
     for (int i = 0; i < cnt; ++i) {
         const int cs = pc_child[first + i];
         const double2 v = expn[(size_t)cs * 2 * tot + e];
@@ -334,10 +337,10 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, c
 
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child, const int* pc_sign,
-                  int klev, double2* expn, hipStream_t s) {
+                  const int* par_acc, int klev, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
     hipLaunchKernelGGL(k_merge, dim3(blocks_for((long long)g.nd * g.nexp), ntask), dim3(kBlock), 0, s, g,
-                       t, par_slot, par_first, par_count, pc_child, pc_sign, klev, expn);
+                       t, par_slot, par_first, par_count, pc_child, pc_sign, par_acc, klev, expn);
     check(hipGetLastError(), "k_merge launch");
 }
 

@@ -118,6 +118,135 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
     }
 }
 
+// ------------------------------------------------------- STEP 1 + STEP 2
+// Parent expansion formed directly from its leaf children (STEP 1 of each
+// leaf child and its STEP 2 shift into the parent, fused):
+//   mp_P = sum_c (Tx o w_x^{c}) (Ty o w_y^{c}) (Tz o w_z^{c}) f_c
+// with w^{c} the child->parent merge phase of each axis.  Children are grouped
+// by their x and y signs: the z stage runs per child and is summed into four
+// (sx,sy) groups, the y stage runs per x sign over both y signs, the x stage
+// runs once over both x signs (K = 2N) -- four times fewer x-stage flops than
+// eight separate leaf transforms, and no leaf expansion reaches HBM.
+// tabS[l][v][k][m]: child-level table, v = 0 plain, 1 = + phase, 2 = conjugate.
+// Task lists: kids (up to 8 leaf children), ksg (merge sign bits, x = bit0).
+// Rounds of two slabs; wave w: slab w & 1 and half (w >> 1) of the k1 range.
+// LDS: sH [4 groups][2 slabs][m2][m1], sF [2 sx][2 slabs][m1][npw]
+template <int N>
+__global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS, int npw,
+                                               const int* __restrict__ plev, const int* __restrict__ pslot,
+                                               const int* __restrict__ kids, const int* __restrict__ ksg,
+                                               const double* __restrict__ fvals,
+                                               double2* __restrict__ expn) {
+    extern __shared__ double2 sm[];
+    double2* sH = sm;                     // 4*2*N*N
+    double2* sF = sH + 8 * N * N;         // 2*2*N*npw
+    const int task = blockIdx.x;
+    const int l = plev[task] + 1, sl = pslot[task];
+    const int per = npw * N;
+    const int base0 = __builtin_amdgcn_readfirstlane(3 * l * per);
+    const double2* __restrict__ T = tabS;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int nh = g.pz, nd = g.nd;
+    int nk = 0;
+    while (nk < 8 && kids[8 * task + nk] >= 0) ++nk;   // leaf children are packed first
+    for (int ind = 0; ind < nd; ++ind) {
+        double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
+        for (int r = 0; r < nh; r += 2) {
+            // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
+            if (tid < N * N) {
+                double2 H[4][2];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) H[q][0] = H[q][1] = make_double2(0.0, 0.0);
+                for (int c = 0; c < nk; ++c) {
+                    const int b = kids[8 * task + c], sg = ksg[8 * task + c];
+                    const int vz = 1 + ((sg >> 2) & 1);
+                    const double2* __restrict__ Tz = T + base0 + vz * per;
+                    double fr[N];
+#pragma unroll
+                    for (int m3 = 0; m3 < N; ++m3)
+                        fr[m3] = fvals[((size_t)b * g.npbox + tid + N * N * m3) * nd + ind];
+                    const int q = sg & 3;
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int k3 = min(r + s, nh - 1);
+                        double ar = 0.0, ai = 0.0;
+#pragma unroll
+                        for (int m3 = 0; m3 < N; ++m3) {
+                            const double2 t = Tz[k3 * N + m3];
+                            ar += t.x * fr[m3];
+                            ai += t.y * fr[m3];
+                        }
+                        // q is the same for all lanes (uniform per child)
+#pragma unroll
+                        for (int qq = 0; qq < 4; ++qq)
+                            if (qq == q) { H[qq][s].x += ar; H[qq][s].y += ai; }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) sH[((q * 2 + s) * N) * N + tid] = H[q][s];
+            }
+            __syncthreads();
+            // y: F[sx][s][m1][k2] = sum_sy sum_m2 Ty^{sy}[k2][m2] H[(sx,sy)][s][m2][m1]
+            // lane = (m1, s in 0..1, sx in 0..1), wave strides k2
+            {
+                const int m1 = lane & 15, s = (lane >> 4) & 1, sx = lane >> 5;
+                const bool act = (m1 < N) && (r + s < nh);
+                double2 hp[N], hm[N];
+#pragma unroll
+                for (int m2 = 0; m2 < N; ++m2) {
+                    hp[m2] = act ? sH[(((sx + 0) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
+                    hm[m2] = act ? sH[(((sx + 2) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
+                }
+                const double2* __restrict__ Typ = T + base0 + 1 * per;
+                const double2* __restrict__ Tym = T + base0 + 2 * per;
+                for (int k2 = wave; k2 < npw; k2 += 4) {
+                    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(Typ[k2 * N + m2], hp[m2], acc);
+#pragma unroll
+                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(Tym[k2 * N + m2], hm[m2], acc);
+                    if (act) sF[((sx * 2 + s) * N + m1) * npw + k2] = acc;
+                }
+            }
+            __syncthreads();
+            // x: E[k3][k1][k2] = sum_sx sum_m1 Tx^{sx}[k1][m1] F[sx][s][m1][k2]
+            {
+                const int s = wave & 1, hlf = wave >> 1;
+                const int k3 = r + s;
+                if (k3 < nh) {
+                    double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    const int k1a = hlf ? (npw + 1) / 2 : 0, k1b = hlf ? npw : (npw + 1) / 2;
+                    const double2* __restrict__ Txp = T + base0 + 1 * per;
+                    const double2* __restrict__ Txm = T + base0 + 2 * per;
+                    for (int c = 0; c < npw; c += 64) {
+                        const int k2 = c + lane;
+                        const bool act = k2 < npw;
+                        double2 fp[N], fm[N];
+#pragma unroll
+                        for (int m1 = 0; m1 < N; ++m1) {
+                            fp[m1] = act ? sF[((0 * 2 + s) * N + m1) * npw + k2] : make_double2(0.0, 0.0);
+                            fm[m1] = act ? sF[((1 * 2 + s) * N + m1) * npw + k2] : make_double2(0.0, 0.0);
+                        }
+                        for (int k1 = k1a; k1 < k1b; ++k1) {
+                            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txp[k1 * N + m1], fp[m1], acc);
+#pragma unroll
+                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txm[k1 * N + m1], fm[m1], acc);
+                            if (act) Ek[(size_t)k1 * npw + k2] = acc;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ---------------------------------------------------------------- STEP 5
 // tabT[l][k][i] = pw2pot table transposed (frequency k, node i).
 // LDS: sff [k2][s][16] (npw*64 complex), sU [i][k] (N*npw complex).
@@ -203,9 +332,11 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                             for (int k = 0; k < N; ++k) acc[k] = zfma(UX[k1 * N + k], e, acc[k]);
                         }
+                        // row (k2, slab) holds 16 entries; entry k is stored at k ^ (k2 & 15)
+                        // so the 64 lanes of a write hit different banks
                         if (act)
 #pragma unroll
-                            for (int k = 0; k < N; ++k) sff[(k2 * 4 + wave) * 16 + k] = acc[k];
+                            for (int k = 0; k < N; ++k) sff[(k2 * 4 + wave) * 16 + (k ^ (k2 & 15))] = acc[k];
                     }
                 }
             }
@@ -218,7 +349,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
                 for (int k2 = 0; k2 < npw; ++k2) {
-                    const double2 f = sff[(k2 * 4 + s) * 16 + kk1];
+                    const double2 f = sff[(k2 * 4 + s) * 16 + (kk1 ^ (k2 & 15))];
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
                 }
@@ -290,12 +421,37 @@ bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box
     }
 }
 
+bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev, const int* pslot,
+                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s) {
+    if (g.ndim != 3 || ntask <= 0) return ntask <= 0;
+    const int N = t.n;
+    const size_t lds = sizeof(double2) * (8 * (size_t)N * N + 4 * (size_t)N * t.npw);
+    if (lds > t.lds_limit) return false;
+    switch (N) {
+        case 8:
+            hipLaunchKernelGGL(k_p2pwF3<8>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot, kids,
+                               ksg, fvals, expn);
+            return true;
+        case 12:
+            hipLaunchKernelGGL(k_p2pwF3<12>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+                               kids, ksg, fvals, expn);
+            return true;
+        case 16:
+            hipLaunchKernelGGL(k_p2pwF3<16>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+                               kids, ksg, fvals, expn);
+            return true;
+        default:
+            return false;
+    }
+}
+
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     if (g.ndim != 3 || (t.n != 8 && t.n != 12 && t.n != 16)) return false;
     const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * t.n);
     const size_t lds = sff + sizeof(double2) * (size_t)t.n * t.npw;
     const size_t lds1 = sizeof(double2) * (4 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
-    return lds <= t.lds_limit && lds1 <= t.lds_limit;
+    const size_t lds2 = sizeof(double2) * (8 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
+    return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
 
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
