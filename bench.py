@@ -46,23 +46,41 @@ def make_workload(seed=1, ng=20, n=16, eps=1e-9, delta=1e-4, iperiod=1):
     return tree, dict(n=n, eps=eps, delta=delta, iperiod=iperiod, ng=ng, seed=seed)
 
 
-def stage_flops_bytes(tree, st, n, nd=1):
-    """Algorithmic FLOPs / HBM bytes per stage launch (one transform).
+def stage_model(st, n, nd=1):
+    """Algorithmic FLOPs and HBM bytes of one launch of the single-kernel stages.
 
-    STEP 1 and STEP 5 (per PW leaf, 3-D):  4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
-    flops (real x complex first stage, complex x complex after); bytes: read
-    fvals (8 n^3) + write the expansion (16 nexp) for STEP 1, read the
-    expansion + read/write pot for STEP 5.
+    STEP 5 (k_pw2pot3, per PW leaf, 3-D): 4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
+    flops (the x/y stages complex x complex, the last stage real output);
+    bytes: read one local expansion (16 nexp) + read/write pot (16 n^3).
+    STEP 3 (k_m2l, per cutoff-level box): 8 nexp flops per list entry; compulsory
+    bytes: every cutoff-level mp read once and every loc written once,
+    2 x 16 nexp per box (re-reads of a shared source are not algorithmic).
     """
     npw, nexp = st["npw"], st["nexp"]
     nh = (npw + 1) // 2
     L = st["npw_leaves"]
     f15 = 4 * n ** 3 * nh + 8 * n * n * npw * nh + 8 * n * npw * npw * nh
-    out = {
-        "p2pw": (L * nd * f15, L * nd * (8 * n ** 3 + 16 * nexp)),
-        "pw2pot": (L * nd * f15, L * nd * (16 * nexp + 16 * n ** 3)),
+    return {
+        "pw2pot": {"kernel": "k_pw2pot3", "bound": "mfma", "flops": L * nd * f15,
+                   "bytes": L * nd * (16 * nexp + 16 * n ** 3)},
+        "m2l": {"kernel": "k_m2l", "bound": "hbm", "flops": 8 * nd * nexp * st["n_m2l_pairs"],
+                "bytes": 2 * 16 * nd * nexp * st["n_m2l_targets"]},
     }
-    return out
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (lines `traffic_bytes <kernel> = <bytes>`), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_boxfgt_fp64_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.replace("`", "").split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                    return int(parts[3])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(tree, cfg):
@@ -113,6 +131,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -153,15 +173,34 @@ def main():
     names = ["p2pw", "merge", "m2l", "split", "pw2pot", "direct"]
     stage = dict(zip(names, stage_ms[:6]))
     log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage.items()))
-    fb = stage_flops_bytes(tree, st2, n)
-    dom = max(("p2pw", "pw2pot"), key=lambda k: stage[k])
-    flops, nbytes = fb[dom]
-    achieved = flops / (stage[dom] * 1e-3) / 1e12
-    roofline = {"kernel": f"k_{dom}", "bound": "mfma", "achieved": round(achieved, 3),
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                "algorithmic_bytes": nbytes, "algorithmic_flops": flops,
-                "stage_ms": {k: round(v, 4) for k, v in stage.items()}}
+    model = stage_model(st2, n)
+    dom = max(model, key=lambda k: stage[k])
+    m = model[dom]
+    secs = stage[dom] * 1e-3
+    if m["bound"] == "mfma":
+        achieved, peak, unit = m["flops"] / secs / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+    else:
+        achieved, peak, unit = m["bytes"] / secs / 1e9, PEAK_HBM_GBS, "GB/s"
+    stages = {}
+    for k, v in model.items():
+        t = stage[k] * 1e-3
+        stages[k] = {"kernel": v["kernel"], "ms": round(stage[k], 4),
+                     "tflops": round(v["flops"] / t / 1e12, 3), "gbs": round(v["bytes"] / t / 1e9, 1)}
+    roofline = {"kernel": m["kernel"], "bound": m["bound"], "achieved": round(achieved, 3), "peak": peak,
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": measured_traffic(m["kernel"]),
+                "algorithmic_bytes": m["bytes"], "algorithmic_flops": m["flops"],
+                "stage_ms": {k: round(v, 4) for k, v in stage.items()}, "stages": stages}
+
+    if args.copy_calibration:
+        # plain device copy of a known size: the scale for the PMC byte counters
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device copies done")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -172,7 +211,7 @@ def main():
         log(f"cpu baseline: {secs:.2f}s ({kind}, {threads} threads)")
 
     out = {
-        "metric": "output nodes/sec (3-D box FGT, plain FP64 drop-in, one problem)",
+        "metric": "output nodes/sec at eps=1e-6 3D (batched vs plain FP64); p50/p99 latency",
         "value": round(value, 1),
         "unit": "output nodes/s",
         "n_gpus": 1,
@@ -184,8 +223,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian sources)",
-        "config": {"workload": "C2: 3-D periodic boxfgt FP64, Legendre, norder=16, eps=1e-9, delta=1e-4, "
-                               "20 anisotropic Gaussians (seed 1)",
+        "config": {"workload": "plain FP64 drop-in, one problem: 3-D periodic boxfgt, Legendre, norder=16, "
+                               "eps=1e-9, delta=1e-4, 20 anisotropic Gaussians (seed 1)",
                    "nboxes": tree.nboxes, "nleaves": int(st["nleaves"]), "nlevels": tree.nlevels,
                    "output_nodes": nout, "npwlevel": st["npwlevel"], "npw": st["npw"],
                    "pw_leaves": int(st["npw_leaves"]), "direct_pairs": int(st["ndirect_pairs"])},

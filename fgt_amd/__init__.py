@@ -36,7 +36,9 @@ class _SolverStats(C.Structure):
                 ("periodic_small", C.c_int), ("all_pw", C.c_int),
                 ("nleaves", C.c_longlong), ("npw_leaves", C.c_longlong),
                 ("ndirect_pairs", C.c_longlong), ("nout", C.c_longlong),
-                ("expansion_bytes", C.c_longlong), ("stage_ms", C.c_double * 8)]
+                ("expansion_bytes", C.c_longlong), ("stage_ms", C.c_double * 8),
+                ("n_m2l_targets", C.c_longlong), ("n_m2l_pairs", C.c_longlong),
+                ("n_parent_child", C.c_longlong)]
 
 
 class _PlanInfo(C.Structure):

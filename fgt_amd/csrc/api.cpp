@@ -558,6 +558,9 @@ int fgt_solver_stats_get(const fgt_solver* s, fgt_solver_stats* o) {
     o->nout = (long long)P.nd * P.npbox * P.nleaves;
     o->expansion_bytes = (long long)s->d_expn.bytes;
     for (int i = 0; i < 8; ++i) o->stage_ms[i] = s->stage_ms[i];
+    o->n_m2l_targets = (long long)P.m2l_tgt.size();
+    o->n_m2l_pairs = (long long)P.m2l_src.size();
+    o->n_parent_child = (long long)P.pc_child.size();
     return 0;
 }
 

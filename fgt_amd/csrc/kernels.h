@@ -24,6 +24,16 @@ struct Geom {
     int ndim;
 };
 
+#ifdef __HIPCC__
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+// range of tasks so that tasks sharing data (children of one parent,
+// neighbouring targets, one element range of many targets) meet in the same L2.
+__device__ __forceinline__ int xcd_task(int b, int nb) {
+    const int x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + min(x, rem) + i;
+}
+#endif
+
 // Device-resident tables (all pointers are device pointers).
 struct DevTables {
     // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)

@@ -144,25 +144,80 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 }
 
 // ---------------------------------------------------------------- STEP 3
-__global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, const int* __restrict__ tgt,
+// One block per (element range, target); the linear block index runs over
+// targets fastest and each XCD takes a contiguous run of it, so the sources
+// shared by neighbouring targets (Morton order within a level) are read from
+// HBM about once per element range and then hit in that XCD's L2.
+// Each thread owns kM2lEpt elements (stride blockDim.x).  The 1-D phase
+// factors of its elements (3 offsets per axis) are loaded once into
+// registers and selected per list entry by the wave-uniform offset index;
+// source expansions are fetched four entries ahead of their use.
+constexpr int kM2lEpt = 2;
+
+__device__ __forceinline__ double2 sel3(double2 a, double2 b, double2 c, int o) {
+    return o == 0 ? a : (o == 1 ? b : c);
+}
+
+__global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
+                                                const int* __restrict__ tgt,
                                                 const int* __restrict__ off,
                                                 const int* __restrict__ src,
                                                 const int* __restrict__ idx,
                                                 double2* __restrict__ expn) {
-    const int task = blockIdx.y;
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lin = xcd_task(blockIdx.x, gridDim.x);
+    const int task = lin % ntask, chunk = lin / ntask;
     const long long tot = (long long)g.nd * g.nexp;
-    if (e >= tot) return;
-    const int r = (int)(e % g.nexp);
-    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    const int npw = t.npw;
+    long long e[kM2lEpt];
+    bool ok[kM2lEpt];
+    double2 fx[kM2lEpt][3], fy[kM2lEpt][3], fz[kM2lEpt][3], acc[kM2lEpt];
     const int ts = tgt[task];
-    double2 acc = expn[(size_t)ts * 2 * tot + e];
-    for (int j = off[task]; j < off[task + 1]; ++j) {
-        const double2 v = expn[(size_t)src[j] * 2 * tot + e];
-        acc = cfma(v, m2l_phase(t.wm2l, t.npw, idx[j], k1, k2, k3, g), acc);
+    const double2* __restrict__ base = expn;
+#pragma unroll
+    for (int q = 0; q < kM2lEpt; ++q) {
+        e[q] = ((long long)chunk * kM2lEpt + q) * blockDim.x + threadIdx.x;
+        ok[q] = e[q] < tot;
+        const long long ec = ok[q] ? e[q] : tot - 1;   // valid duplicate, never stored
+        e[q] = ec;
+        const int r = (int)(ec % g.nexp);
+        const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            fx[q][o] = g.px > 1 ? t.wm2l[o * npw + k1] : make_double2(1.0, 0.0);
+            fy[q][o] = g.py > 1 ? t.wm2l[o * npw + k2] : make_double2(1.0, 0.0);
+            fz[q][o] = t.wm2l[o * npw + k3];
+        }
+        acc[q] = base[(size_t)ts * 2 * tot + ec];
     }
-    expn[((size_t)ts * 2 + 1) * tot + e] = acc;
+    const int j0 = off[task], j1 = off[task + 1];
+#define M2L_PHASE(q, o)                                                                     \
+    cmul(cmul(sel3(fx[q][0], fx[q][1], fx[q][2], (o) % 3), sel3(fy[q][0], fy[q][1], fy[q][2], ((o) / 3) % 3)), \
+         sel3(fz[q][0], fz[q][1], fz[q][2], (o) / 9))
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+        double2 v[4][kM2lEpt];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < kM2lEpt; ++q) v[u][q] = base[(size_t)src[j + u] * 2 * tot + e[q]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int o = idx[j + u];
+#pragma unroll
+            for (int q = 0; q < kM2lEpt; ++q) acc[q] = cfma(v[u][q], M2L_PHASE(q, o), acc[q]);
+        }
+    }
+    for (; j < j1; ++j) {
+        const int o = idx[j];
+        const size_t sb = (size_t)src[j] * 2 * tot;
+#pragma unroll
+        for (int q = 0; q < kM2lEpt; ++q) acc[q] = cfma(base[sb + e[q]], M2L_PHASE(q, o), acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kM2lEpt; ++q)
+        if (ok[q]) expn[((size_t)ts * 2 + 1) * tot + e[q]] = acc[q];
 }
+#undef M2L_PHASE
 
 // ---------------------------------------------------------------- STEP 4
 __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
@@ -347,8 +402,9 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* off,
                 const int* src, const int* idx, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
-    hipLaunchKernelGGL(k_m2l, dim3(blocks_for((long long)g.nd * g.nexp), ntask), dim3(kBlock), 0, s, g, t,
-                       tgt, off, src, idx, expn);
+    const long long nblk = (long long)blocks_for(((long long)g.nd * g.nexp + kM2lEpt - 1) / kM2lEpt) * ntask;
+    if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l: grid too large");
+    hipLaunchKernelGGL(k_m2l, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, tgt, off, src, idx, expn);
     check(hipGetLastError(), "k_m2l launch");
 }
 

@@ -18,14 +18,6 @@ namespace fgt {
 
 namespace {
 
-// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-// range of tasks so that tasks sharing data (children of one parent,
-// neighbouring targets) meet in the same L2.
-__device__ __forceinline__ int xcd_task(int b, int nb) {
-    const int x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
-    return x * per + min(x, rem) + i;
-}
-
 __device__ __forceinline__ double2 zfma(double2 t, double2 f, double2 c) {
     return make_double2(c.x + t.x * f.x - t.y * f.y, c.y + t.x * f.y + t.y * f.x);
 }
@@ -130,7 +122,13 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // tabS[l][v][k][m]: child-level table, v = 0 plain, 1 = + phase, 2 = conjugate.
 // Task lists: kids (up to 8 leaf children), ksg (merge sign bits, x = bit0).
 // Rounds of two slabs; wave w: slab w & 1 and half (w >> 1) of the k1 range.
-// LDS: sH [4 groups][2 slabs][m2][m1], sF [2 sx][2 slabs][m1][npw]
+// Each parent is split over kFusedParts blocks, each taking a contiguous run
+// of rounds (the parts of one parent are adjacent in launch order and land on
+// one XCD, so the children's values they all read meet in its L2).
+// LDS: sH [4 groups][2 slabs][m2][m1] and sF [2 sx][2 slabs][m1][npw] share
+// one buffer: sH is fully read into registers before sF is written.
+constexpr int kFusedParts = 4;
+
 template <int N>
 __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS, int npw,
                                                const int* __restrict__ plev, const int* __restrict__ pslot,
@@ -139,8 +137,9 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                                                double2* __restrict__ expn) {
     extern __shared__ double2 sm[];
     double2* sH = sm;                     // 4*2*N*N
-    double2* sF = sH + 8 * N * N;         // 2*2*N*npw
-    const int task = blockIdx.x;
+    double2* sF = sm;                     // 2*2*N*npw (aliases sH)
+    const int lin = xcd_task(blockIdx.x, gridDim.x);
+    const int task = lin / kFusedParts, part = lin % kFusedParts;
     const int l = plev[task] + 1, sl = pslot[task];
     const int per = npw * N;
     const int base0 = __builtin_amdgcn_readfirstlane(3 * l * per);
@@ -151,9 +150,11 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     const int nh = g.pz, nd = g.nd;
     int nk = 0;
     while (nk < 8 && kids[8 * task + nk] >= 0) ++nk;   // leaf children are packed first
+    const int nround = (nh + 1) / 2;
+    const int rb = 2 * (part * nround / kFusedParts), re = 2 * ((part + 1) * nround / kFusedParts);
     for (int ind = 0; ind < nd; ++ind) {
         double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
-        for (int r = 0; r < nh; r += 2) {
+        for (int r = rb; r < re; r += 2) {
             // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
             if (tid < N * N) {
                 double2 H[4][2];
@@ -203,6 +204,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 }
                 const double2* __restrict__ Typ = T + base0 + 1 * per;
                 const double2* __restrict__ Tym = T + base0 + 2 * per;
+                __syncthreads();   // sH consumed; sF (same memory) may be written
                 for (int k2 = wave; k2 < npw; k2 += 4) {
                     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
@@ -425,19 +427,21 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
                    const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s) {
     if (g.ndim != 3 || ntask <= 0) return ntask <= 0;
     const int N = t.n;
-    const size_t lds = sizeof(double2) * (8 * (size_t)N * N + 4 * (size_t)N * t.npw);
+    const size_t lds = sizeof(double2) * std::max(8 * (size_t)N * N, 4 * (size_t)N * t.npw);
     if (lds > t.lds_limit) return false;
+    if ((long long)ntask * kFusedParts > 0x7fffffffLL) return false;
+    const unsigned nblk = (unsigned)(ntask * kFusedParts);
     switch (N) {
         case 8:
-            hipLaunchKernelGGL(k_p2pwF3<8>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot, kids,
+            hipLaunchKernelGGL(k_p2pwF3<8>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot, kids,
                                ksg, fvals, expn);
             return true;
         case 12:
-            hipLaunchKernelGGL(k_p2pwF3<12>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+            hipLaunchKernelGGL(k_p2pwF3<12>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
                                kids, ksg, fvals, expn);
             return true;
         case 16:
-            hipLaunchKernelGGL(k_p2pwF3<16>, dim3(ntask), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+            hipLaunchKernelGGL(k_p2pwF3<16>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
                                kids, ksg, fvals, expn);
             return true;
         default:
@@ -450,7 +454,7 @@ bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * t.n);
     const size_t lds = sff + sizeof(double2) * (size_t)t.n * t.npw;
     const size_t lds1 = sizeof(double2) * (4 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
-    const size_t lds2 = sizeof(double2) * (8 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
+    const size_t lds2 = sizeof(double2) * std::max(8 * (size_t)t.n * t.n, 4 * (size_t)t.n * t.npw);
     return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
 

@@ -68,6 +68,8 @@ typedef struct {
     long long nleaves, npw_leaves, ndirect_pairs, nout; /* nout = nd*npbox*nleaves */
     long long expansion_bytes;
     double stage_ms[8];   /* last execute: STEP 1..6 in [0..5] (when timed) */
+    long long n_m2l_targets, n_m2l_pairs;   /* cutoff-level boxes receiving a loc; list entries */
+    long long n_parent_child;                /* child->parent merge edges */
 } fgt_solver_stats;
 
 int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly,

@@ -342,6 +342,38 @@ void oracle_pw2pot(int ipoly, int n, int npw, const double *ts, double delta,
     free(x); free(w); free(u);
 }
 
+/* mk_pw2pgh_tables (bfgt_volrouts.f:339-401): the potential table and its
+ * first/second derivative versions, tab_x(j,i) = t_j tab(j,i) dfac and
+ * tab_xx(j,i) = t_j tab_x(j,i) dfac with dfac = i (D/(2 sqrt(delta))) (2/D);
+ * the upper half j >= npw/2 is formed, the lower half is its conjugate. */
+void oracle_pw2pgh(int ipoly, int n, int npw, const double *ts, double delta,
+                   double boxdim, double *tab_out, double *tabx_out, double *tabxx_out)
+{
+    double *x = (double *)malloc(sizeof(double) * n);
+    double *w = (double *)malloc(sizeof(double) * n);
+    double *u = (double *)malloc(sizeof(double) * n * n);
+    oracle_polyexps(ipoly, n, x, w, u);
+    double dsq = boxdim / 2 / sqrt(delta);
+    cplx dfac = I * dsq * 2 / boxdim;
+    int npw2 = npw / 2;
+    for (int i = 0; i < n; ++i) {
+        double xx = x[i] * dsq;
+        for (int j1 = npw2; j1 < npw; ++j1) {
+            cplx e = cexp(I * ts[j1] * xx);
+            cplx ex = ts[j1] * e * dfac;
+            cplx exx = ts[j1] * ex * dfac;
+            size_t a = j1 + (size_t)i * npw, b = (npw - 1 - j1) + (size_t)i * npw;
+            tab_out[2 * a] = creal(e);   tab_out[2 * a + 1] = cimag(e);
+            tabx_out[2 * a] = creal(ex); tabx_out[2 * a + 1] = cimag(ex);
+            tabxx_out[2 * a] = creal(exx); tabxx_out[2 * a + 1] = cimag(exx);
+            tab_out[2 * b] = creal(e);   tab_out[2 * b + 1] = -cimag(e);
+            tabx_out[2 * b] = creal(ex); tabx_out[2 * b + 1] = -cimag(ex);
+            tabxx_out[2 * b] = creal(exx); tabxx_out[2 * b + 1] = -cimag(exx);
+        }
+    }
+    free(x); free(w); free(u);
+}
+
 /* mk_legetab_recur (bfgt_volrouts.f:1095-1188): five-term recurrence for
  * I_n(t)/lambda = (1/lambda) int_{-1}^1 P_n(x) e^{-(t-x)^2/lambda^2} dx and the
  * matching first/second-derivative integrals, lambda <= 1/8. */
@@ -543,8 +575,19 @@ static void merge_patterns(int n, const int *ind, const int *indx,
 }
 
 /* mk_loctab_all with mrefinelev = 0, nloctab = 6 (bfgt_volrouts.f:407-660). */
+void oracle_loctab3(double eps, int ipoly, int n, double delta, double boxdim,
+                    double *tab_loc, double *tabx_loc, double *tabxx_loc, int *ind_loc);
+
 void oracle_loctab(double eps, int ipoly, int n, double delta, double boxdim,
                    double *tab_loc, int *ind_loc)
+{
+    oracle_loctab3(eps, ipoly, n, delta, boxdim, tab_loc, NULL, NULL, ind_loc);
+}
+
+/* mk_loctab_all (bfgt_volrouts.f:407-660) with the first/second derivative
+ * tables tabx_loc, tabxx_loc (nullable) in the layout of tab_loc. */
+void oracle_loctab3(double eps, int ipoly, int n, double delta, double boxdim,
+                    double *tab_loc, double *tabx_loc, double *tabxx_loc, int *ind_loc)
 {
     const int nq = 50;
     double tn[64], tw[64], tu[64 * 64];
@@ -610,6 +653,8 @@ void oracle_loctab(double eps, int ipoly, int n, double delta, double boxdim,
             merge_patterns(n, a, b, c, ind_loc + (k + 6) * 2 * (n + 1));
         }
     }
+    if (tabx_loc) memcpy(tabx_loc, lx, sizeof(double) * nn * 13);
+    if (tabxx_loc) memcpy(tabxx_loc, lxx, sizeof(double) * nn * 13);
     free(polyv); free(tab); free(lx);
 }
 
@@ -817,12 +862,21 @@ static void tens_prod_to_pw(int ndim, int n, int nd, int ind, const double *f,
 
 /* g{1,2,3}d_pw2pot (bfgt_pwrouts.f:305-790): pot += 2 Re(...), one density,
  * zero frequency halved when npw is odd. */
-static void pw2pot(int ndim, int n, int nd, int ind, int npw, const cplx *pw,
-                   const cplx *T, double *pot, cplx *work)
+/* One output component of gnd_pw2pgh (bfgt_pwrouts.f:233-299; 1-D 308-462,
+ * 2-D 473-708, 3-D 719-1065): out += 2 Re sum Tx Ty Tz pw, contracting x,
+ * then y, then the half axis (zero frequency halved when npw is odd).
+ * Tax[a] is the table of axis a: tab_pw2pot, tab_pw2der or tab_pw2dxx, so
+ * pot, each gradient and each Hessian component is one call with the
+ * matching tables (the reference forms them together from shared partial
+ * sums; each component's operation sequence is the same).
+ * out[ostride * node + ooff] receives node node = k1 + n k2 + n^2 k3. */
+static void pw2pot(int ndim, int n, int npw, const cplx *pw, const cplx *const *Tax,
+                   double *out, size_t ostride, size_t ooff, cplx *work)
 {
     int nh = (npw + 1) / 2, npw2 = npw / 2;
-#define PV(i) pot[(size_t)nd * (i) + ind]
+#define PV(i) out[ostride * (size_t)(i) + ooff]
     if (ndim == 1) {
+        const cplx *T = Tax[0];
         for (int k1 = 0; k1 < n; ++k1) {
             cplx cd = 0;
             for (int m1 = 0; m1 < npw2; ++m1) cd += T[m1 + npw * k1] * pw[m1];
@@ -830,47 +884,75 @@ static void pw2pot(int ndim, int n, int nd, int ind, int npw, const cplx *pw,
             PV(k1) += creal(cd) * 2;
         }
     } else if (ndim == 2) {
+        const cplx *Tx = Tax[0], *Ty = Tax[1];
         cplx *ff = work; /* ff(n,nh) */
         for (int m2 = 0; m2 < nh; ++m2)
             for (int k1 = 0; k1 < n; ++k1) {
                 cplx cd = 0;
-                for (int m1 = 0; m1 < npw; ++m1) cd += T[m1 + npw * k1] * pw[m1 + npw * m2];
+                for (int m1 = 0; m1 < npw; ++m1) cd += Tx[m1 + npw * k1] * pw[m1 + npw * m2];
                 ff[k1 + n * m2] = cd;
             }
         for (int k2 = 0; k2 < n; ++k2)
             for (int k1 = 0; k1 < n; ++k1) {
                 cplx cd = 0;
-                for (int m2 = 0; m2 < npw2; ++m2) cd += T[m2 + npw * k2] * ff[k1 + n * m2];
-                if (nh > npw2) cd += T[nh - 1 + npw * k2] * ff[k1 + n * (nh - 1)] / 2;
+                for (int m2 = 0; m2 < npw2; ++m2) cd += Ty[m2 + npw * k2] * ff[k1 + n * m2];
+                if (nh > npw2) cd += Ty[nh - 1 + npw * k2] * ff[k1 + n * (nh - 1)] / 2;
                 PV(k1 + n * k2) += creal(cd) * 2;
             }
     } else {
+        const cplx *Tx = Tax[0], *Ty = Tax[1], *Tz = Tax[2];
         cplx *ff = work;                          /* ff(n,npw,nh) */
         cplx *ff2 = work + (size_t)n * npw * nh;  /* ff2(n,n,nh)  */
         for (int m3 = 0; m3 < nh; ++m3)
             for (int m2 = 0; m2 < npw; ++m2)
                 for (int k1 = 0; k1 < n; ++k1) {
                     cplx cd = 0;
-                    for (int m1 = 0; m1 < npw; ++m1) cd += T[m1 + npw * k1] * pw[m1 + npw * (m2 + (size_t)npw * m3)];
+                    for (int m1 = 0; m1 < npw; ++m1) cd += Tx[m1 + npw * k1] * pw[m1 + npw * (m2 + (size_t)npw * m3)];
                     ff[k1 + n * (m2 + (size_t)npw * m3)] = cd;
                 }
         for (int m3 = 0; m3 < nh; ++m3)
             for (int k2 = 0; k2 < n; ++k2)
                 for (int k1 = 0; k1 < n; ++k1) {
                     cplx cd = 0;
-                    for (int m2 = 0; m2 < npw; ++m2) cd += T[m2 + npw * k2] * ff[k1 + n * (m2 + (size_t)npw * m3)];
+                    for (int m2 = 0; m2 < npw; ++m2) cd += Ty[m2 + npw * k2] * ff[k1 + n * (m2 + (size_t)npw * m3)];
                     ff2[k1 + n * (k2 + n * m3)] = cd;
                 }
         for (int k3 = 0; k3 < n; ++k3)
             for (int k2 = 0; k2 < n; ++k2)
                 for (int k1 = 0; k1 < n; ++k1) {
                     cplx cd = 0;
-                    for (int m3 = 0; m3 < npw2; ++m3) cd += T[m3 + npw * k3] * ff2[k1 + n * (k2 + n * m3)];
-                    if (nh > npw2) cd += T[nh - 1 + npw * k3] * ff2[k1 + n * (k2 + n * (nh - 1))] / 2;
+                    for (int m3 = 0; m3 < npw2; ++m3) cd += Tz[m3 + npw * k3] * ff2[k1 + n * (k2 + n * m3)];
+                    if (nh > npw2) cd += Tz[nh - 1 + npw * k3] * ff2[k1 + n * (k2 + n * (nh - 1))] / 2;
                     PV(k1 + n * (k2 + n * k3)) += creal(cd) * 2;
                 }
     }
 #undef PV
+}
+
+/* Output components of ifpgh (boxfgt.f:35-38 and the pg/pgh routines):
+ * per-axis table kind (0 value, 1 first derivative, 2 second derivative).
+ * Component 0 is pot; then grad(ndim); then the Hessian in the reference's
+ * order: 1-D xx; 2-D xx, xy, yy (bfgt_pwrouts.f:612-708); 3-D xx, yy, zz,
+ * xy, xz, yz (bfgt_pwrouts.f:906-1065). */
+static int pgh_components(int ndim, int ifpgh, int kinds[10][3])
+{
+    static const int h1[1][3] = {{2, 0, 0}};
+    static const int h2[3][3] = {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}};
+    static const int h3[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    int nc = 0;
+    kinds[nc][0] = kinds[nc][1] = kinds[nc][2] = 0; ++nc;
+    if (ifpgh >= 2)
+        for (int a = 0; a < ndim; ++a, ++nc) {
+            kinds[nc][0] = kinds[nc][1] = kinds[nc][2] = 0;
+            kinds[nc][a] = 1;
+        }
+    if (ifpgh >= 3) {
+        int nhess = ndim * (ndim + 1) / 2;
+        const int (*h)[3] = ndim == 1 ? h1 : (ndim == 2 ? h2 : h3);
+        for (int i = 0; i < nhess; ++i, ++nc)
+            for (int a = 0; a < 3; ++a) kinds[nc][a] = h[i][a];
+    }
+    return nc;
 }
 
 /* gnd_find_loctab_ind with mrefinelev = 0 (bfgt_local.f:1624-1660). */
@@ -889,20 +971,23 @@ static void find_loctab_ind(int ndim, int iperiod, const double *tc,
     }
 }
 
-/* gnd_tens_prod_to_potloc (bfgt_local.f:152-1618), one density: pot +=
- * (A_x (x) A_y (x) A_z) f with each A the table restricted to its band.  The
- * contraction order (x, then y, then z) differs from the reference's
- * band-width-dependent order only in rounding. */
+/* gnd_tens_prod_to_pghloc (bfgt_local.f:14-140; pot 152-1618, pg/pgh
+ * 644-920), one density, one output component: out += (A_x (x) A_y (x) A_z) f
+ * with each A the table of the component's kind on that axis (tab_loc,
+ * tabx_loc or tabxx_loc) restricted to the band of ind_loc.  The contraction
+ * order is x, then y, then z, as in the reference's pg/pgh routines; its
+ * potential-only routine picks the order by band width, which changes only
+ * rounding. */
 static void tens_prod_to_potloc(int ndim, int n, int nd, int ind,
-                                const double *f, double *pot,
-                                const double *tab, const int *indl,
+                                const double *f, double *out, size_t ostride, size_t ooff,
+                                const double *const *tabk, const int *kind, const int *indl,
                                 const int *ixyz, double *work)
 {
     const double *A[3];
     const int *B[3];
     for (int a = 0; a < ndim; ++a) {
         int slot = ixyz[a] + 6;
-        A[a] = tab + (size_t)slot * n * n;
+        A[a] = tabk[kind[a]] + (size_t)slot * n * n;
         B[a] = indl + (size_t)slot * 2 * (n + 1);
         int nx = B[a][2 * n + 1] - B[a][2 * n] + 1;
         if (nx == 0) return;
@@ -931,7 +1016,7 @@ static void tens_prod_to_potloc(int ndim, int n, int nd, int ind,
         double *tmp = cur; cur = nxt; nxt = tmp;
         stride *= n;
     }
-    for (int i = 0; i < npb; ++i) pot[(size_t)nd * i + ind] += cur[i];
+    for (int i = 0; i < npb; ++i) out[ostride * (size_t)i + ooff] += cur[i];
 }
 
 /* ------------------------------------------------------------------ */
@@ -1010,6 +1095,37 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
                   const double *boxsize, const double *fvals, double *pot,
                   int d1_mode, double *timeinfo, oracle_info *info)
 {
+    return oracle_boxfgt_pgh(nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels, itree, iptr,
+                             centers, boxsize, fvals, 1, pot, NULL, NULL, d1_mode, timeinfo, info);
+}
+
+/* Output of component c for box b (1-based) and density ind: base pointer
+ * and (stride, offset) within the box, Fortran pot(nd,npbox), grad(nd,ndim,
+ * npbox), hess(nd,nhess,npbox). */
+static double *comp_out(int c, int ndim, int nd, int npbox, int b, int ind, double *pot,
+                        double *grad, double *hess, size_t *ostride, size_t *ooff)
+{
+    int nhess = ndim * (ndim + 1) / 2;
+    if (c == 0) {
+        *ostride = nd; *ooff = ind;
+        return pot + (size_t)nd * npbox * (b - 1);
+    }
+    if (c <= ndim) {
+        *ostride = (size_t)nd * ndim; *ooff = (size_t)(c - 1) * nd + ind;
+        return grad + (size_t)nd * ndim * npbox * (b - 1);
+    }
+    *ostride = (size_t)nd * nhess; *ooff = (size_t)(c - 1 - ndim) * nd + ind;
+    return hess + (size_t)nd * nhess * npbox * (b - 1);
+}
+
+int oracle_boxfgt_pgh(int nd, int ndim, double delta, double eps, int ipoly,
+                      int iperiod, int norder, int nboxes, int nlevels,
+                      const int *itree, const int *iptr, const double *centers,
+                      const double *boxsize, const double *fvals, int ifpgh,
+                      double *pot, double *grad, double *hess,
+                      int d1_mode, double *timeinfo, oracle_info *info)
+{
+    if (ifpgh < 1 || ifpgh > 3 || (ifpgh >= 2 && !grad) || (ifpgh >= 3 && !hess)) return 1;
     if (ndim < 1 || ndim > 3 || norder < 1 || norder > 32 || nd < 1 ||
         (ipoly != 0 && ipoly != 1) || (iperiod != 0 && iperiod != 1) ||
         nboxes < 1 || nlevels < 0 || !(delta > 0) || !(eps > 0 && eps < 1))
@@ -1082,12 +1198,16 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
     int *list1 = (int *)calloc((size_t)mnl * nboxes, sizeof(int));
     modified_list1(&T, npwlevel, ifpwexp, iper, mnl, nlist1, list1);
 
+    int kinds[10][3];
+    const int ncomp = pgh_components(ndim, ifpgh, kinds);
     size_t nn = (size_t)n * n;
-    double *tab_loc = (double *)malloc(sizeof(double) * nn * 13 * (nlevels + 1));
+    size_t ltsz = nn * 13 * (nlevels + 1);
+    double *tab_loc = (double *)malloc(sizeof(double) * ltsz * 3);
     int *ind_loc = (int *)malloc(sizeof(int) * 2 * (n + 1) * 13 * (nlevels + 1));
     for (int l = 0; l <= nlevels; ++l)
-        oracle_loctab(eps, ipoly, n, delta, boxsize[l], tab_loc + nn * 13 * l,
-                      ind_loc + (size_t)2 * (n + 1) * 13 * l);
+        oracle_loctab3(eps, ipoly, n, delta, boxsize[l], tab_loc + nn * 13 * l,
+                       tab_loc + ltsz + nn * 13 * l, tab_loc + 2 * ltsz + nn * 13 * l,
+                       ind_loc + (size_t)2 * (n + 1) * 13 * l);
 
     int nlevstart = npwlevel > 0 ? npwlevel : 0;
     int nh = (npw + 1) / 2;
@@ -1103,12 +1223,16 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
     int do_pw = npwlevel < nlevels;
     if (do_pw) {
         tab_p2pw = (cplx *)malloc(sizeof(cplx) * (size_t)n * npw * (nlevels + 1));
-        tab_pw2pot = (cplx *)malloc(sizeof(cplx) * (size_t)n * npw * (nlevels + 1));
+        /* tab_pw2pot, tab_pw2der, tab_pw2dxx per level (boxfgt.f:639-647) */
+        tab_pw2pot = (cplx *)malloc(sizeof(cplx) * (size_t)n * npw * (nlevels + 1) * 3);
         for (int l = nlevstart; l <= nlevels; ++l) {
+            size_t pl = (size_t)n * npw * (nlevels + 1);
             oracle_poly2pw(ipoly, n, npw, ws, ts, delta0, boxsize[l],
                            (double *)(tab_p2pw + (size_t)n * npw * l));
-            oracle_pw2pot(ipoly, n, npw, ts, delta0, boxsize[l],
-                          (double *)(tab_pw2pot + (size_t)n * npw * l));
+            oracle_pw2pgh(ipoly, n, npw, ts, delta0, boxsize[l],
+                          (double *)(tab_pw2pot + (size_t)n * npw * l),
+                          (double *)(tab_pw2pot + pl + (size_t)n * npw * l),
+                          (double *)(tab_pw2pot + 2 * pl + (size_t)n * npw * l));
         }
         nmax = nlevels - nlevstart;
         wms = (cplx *)malloc(sizeof(cplx) * (size_t)nexp * mc * (nmax > 0 ? nmax : 1));
@@ -1235,7 +1359,9 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
         /* STEP 5 (boxfgt.f:865-881) */
         t0 = now_s();
         for (int ilev = nlevstart; ilev <= nlevels; ++ilev) {
-            const cplx *Tl = tab_pw2pot + (size_t)n * npw * ilev;
+            size_t pl = (size_t)n * npw * (nlevels + 1);
+            const cplx *Tk[3] = {tab_pw2pot + (size_t)n * npw * ilev, tab_pw2pot + pl + (size_t)n * npw * ilev,
+                                 tab_pw2pot + 2 * pl + (size_t)n * npw * ilev};
 #pragma omp parallel
             {
                 cplx *work = (cplx *)malloc(sizeof(cplx) * wsz);
@@ -1243,8 +1369,13 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
                 for (int ibox = LADDR(&T, 1, ilev); ibox <= LADDR(&T, 2, ilev); ++ibox) {
                     if (ifpwexp[ibox - 1] != 1 || NCHILD(&T, ibox) != 0) continue;
                     for (int ind = 0; ind < nd; ++ind)
-                        pw2pot(ndim, n, nd, ind, npw, rml + iaddr[2 * (ibox - 1) + 1] + (size_t)ind * nexp,
-                               Tl, pot + (size_t)nd * npbox * (ibox - 1), work);
+                        for (int c = 0; c < ncomp; ++c) {
+                            const cplx *Tax[3] = {Tk[kinds[c][0]], Tk[kinds[c][1]], Tk[kinds[c][2]]};
+                            size_t os, oo;
+                            double *out = comp_out(c, ndim, nd, npbox, ibox, ind, pot, grad, hess, &os, &oo);
+                            pw2pot(ndim, n, npw, rml + iaddr[2 * (ibox - 1) + 1] + (size_t)ind * nexp, Tax,
+                                   out, os, oo, work);
+                        }
                 }
                 free(work);
             }
@@ -1256,7 +1387,8 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
     t0 = now_s();
     if (!all_pw) {
         for (int ilev = 0; ilev <= nlevend; ++ilev) {
-            const double *tab = tab_loc + nn * 13 * ilev;
+            const double *tabk[3] = {tab_loc + nn * 13 * ilev, tab_loc + ltsz + nn * 13 * ilev,
+                                     tab_loc + 2 * ltsz + nn * 13 * ilev};
             const int *indl = ind_loc + (size_t)2 * (n + 1) * 13 * ilev;
 #pragma omp parallel
             {
@@ -1270,8 +1402,12 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
                         find_loctab_ind(ndim, iper, centers + (size_t)ndim * (ibox - 1),
                                         centers + (size_t)ndim * (jbox - 1), boxsize[jlev], bs0, ixyz);
                         for (int ind = 0; ind < nd; ++ind)
-                            tens_prod_to_potloc(ndim, n, nd, ind, fvals + (size_t)nd * npbox * (jbox - 1),
-                                                pot + (size_t)nd * npbox * (ibox - 1), tab, indl, ixyz, work);
+                            for (int c = 0; c < ncomp; ++c) {
+                                size_t os, oo;
+                                double *out = comp_out(c, ndim, nd, npbox, ibox, ind, pot, grad, hess, &os, &oo);
+                                tens_prod_to_potloc(ndim, n, nd, ind, fvals + (size_t)nd * npbox * (jbox - 1),
+                                                    out, os, oo, tabk, kinds[c], indl, ixyz, work);
+                            }
                     }
                 }
                 free(work);

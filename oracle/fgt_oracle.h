@@ -53,6 +53,19 @@ int oracle_boxfgt(int nd, int ndim, double delta, double eps, int ipoly,
                   const double *boxsize, const double *fvals, double *pot,
                   int d1_mode, double *timeinfo, oracle_info *info);
 
+/*
+ * Same with gradients and Hessians (ifpgh = 2: pot + grad, 3: + hess),
+ * grad(nd,ndim,npbox,nboxes) and hess(nd,ndim*(ndim+1)/2,npbox,nboxes)
+ * accumulated like pot (Hessian order 1-D xx; 2-D xx,xy,yy; 3-D
+ * xx,yy,zz,xy,xz,yz, as src/bfgt/bfgt_pwrouts.f:612-1065).
+ */
+int oracle_boxfgt_pgh(int nd, int ndim, double delta, double eps, int ipoly,
+                      int iperiod, int norder, int nboxes, int nlevels,
+                      const int *itree, const int *iptr, const double *centers,
+                      const double *boxsize, const double *fvals, int ifpgh,
+                      double *pot, double *grad, double *hess,
+                      int d1_mode, double *timeinfo, oracle_info *info);
+
 /* ---- exposed pieces, so tests can pin them one by one ---- */
 
 /* fgtpwterms (src/common/fgtterms.f:72-141). */
@@ -79,6 +92,13 @@ void oracle_pw2pot(int ipoly, int n, int npw, const double *ts, double delta,
  * ind[2,n+1,13] likewise (1-based row/col bounds, as the reference). */
 void oracle_loctab(double eps, int ipoly, int n, double delta, double boxdim,
                    double *tab, int *ind);
+/* Same plus the first/second derivative tables (nullable), same layout. */
+void oracle_loctab3(double eps, int ipoly, int n, double delta, double boxdim,
+                    double *tab, double *tabx, double *tabxx, int *ind);
+/* tab_pw2pot, tab_pw2der, tab_pw2dxx (npw,n) complex interleaved
+ * (mk_pw2pgh_tables, bfgt_volrouts.f:339-401). */
+void oracle_pw2pgh(int ipoly, int n, int npw, const double *ts, double delta,
+                   double boxdim, double *tab, double *tabx, double *tabxx);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,49 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels,
     return pot, info, ti
 
 
+def boxfgt_pgh(nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels,
+               itree, iptr, centers, boxsize, fvals, ifpgh, d1_mode=0):
+    """(pot, grad, hess, info): grad [nboxes, npbox, ndim(, nd)], hess likewise
+    with ndim*(ndim+1)/2 components (None for ifpgh < 3)."""
+    L = lib()
+    itree = np.ascontiguousarray(itree, np.int32)
+    iptr = np.ascontiguousarray(iptr, np.int32)
+    centers = np.ascontiguousarray(centers, np.float64)
+    boxsize = np.ascontiguousarray(boxsize, np.float64)
+    fvals = np.ascontiguousarray(fvals, np.float64)
+    npbox = norder ** ndim
+    tail = () if nd == 1 else (nd,)
+    pot = np.zeros_like(fvals)
+    grad = np.zeros((nboxes, npbox, ndim) + tail if ifpgh >= 2 else 1)
+    hess = np.zeros((nboxes, npbox, ndim * (ndim + 1) // 2) + tail if ifpgh >= 3 else 1)
+    info = OracleInfo()
+    rc = L.oracle_boxfgt_pgh(C.c_int(nd), C.c_int(ndim), C.c_double(delta), C.c_double(eps),
+                             C.c_int(ipoly), C.c_int(iperiod), C.c_int(norder),
+                             C.c_int(nboxes), C.c_int(nlevels), _p(itree), _p(iptr),
+                             _p(centers), _p(boxsize), _p(fvals), C.c_int(ifpgh), _p(pot), _p(grad),
+                             _p(hess), C.c_int(d1_mode), None, C.byref(info))
+    if rc != 0:
+        raise RuntimeError(f"oracle_boxfgt_pgh failed with code {rc}")
+    return pot, (grad if ifpgh >= 2 else None), (hess if ifpgh >= 3 else None), info
+
+
+def pw2pgh(ipoly, n, npw, ts, delta, boxdim):
+    """(tab_pw2pot, tab_pw2der, tab_pw2dxx), each [j, i] like the reference (npw, n)."""
+    t = [np.zeros(2 * n * npw) for _ in range(3)]
+    lib().oracle_pw2pgh(C.c_int(ipoly), C.c_int(n), C.c_int(npw), _p(np.ascontiguousarray(ts)),
+                        C.c_double(delta), C.c_double(boxdim), _p(t[0]), _p(t[1]), _p(t[2]))
+    return tuple(a.view(np.complex128).reshape(n, npw).T for a in t)
+
+
+def loctab3(eps, ipoly, n, delta, boxdim):
+    """(tab, tabx, tabxx, ind) as loctab, with the derivative tables."""
+    t = [np.zeros(n * n * 13) for _ in range(3)]
+    ind = np.zeros(2 * (n + 1) * 13, np.int32)
+    lib().oracle_loctab3(C.c_double(eps), C.c_int(ipoly), C.c_int(n), C.c_double(delta),
+                         C.c_double(boxdim), _p(t[0]), _p(t[1]), _p(t[2]), _p(ind))
+    return tuple(a.reshape(13, n, n) for a in t) + (ind.reshape(13, n + 1, 2),)
+
+
 def pwterms(bsize, delta, eps, iperiod):
     pmax = C.c_double()
     npw = C.c_int()

@@ -87,19 +87,26 @@ def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng) -> Tree:
                 iptr, centers, boxsize, fvals)
 
 
-def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None):
-    """Run the reference boxfgt (ifpgh=1, ifnewtree=0, no extra targets).
+def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh: int = 1):
+    """Run the reference boxfgt (ifnewtree=0, no extra targets).
 
-    Returns (pot[nboxes, npbox], timeinfo[20]).
+    Returns (pot, timeinfo[20]) for ifpgh=1, (pot, grad, hess, timeinfo) for
+    ifpgh=2/3 (hess None for ifpgh=2).  pot is shaped like fvals; grad is
+    Fortran grad(nd,ndim,npbox,nboxes) as [nboxes, npbox, ndim(, nd)], hess
+    likewise with ndim*(ndim+1)/2 components.
     """
     L = lib()
     npbox = tree.norder ** tree.ndim
     f = np.ascontiguousarray(tree.fvals if fvals is None else fvals, np.float64)
     pot = np.zeros_like(f)
-    grad = np.zeros(1, np.float64)
-    hess = np.zeros(1, np.float64)
+    ndim = tree.ndim
+    nhess = ndim * (ndim + 1) // 2
+    tail = () if nd == 1 else (nd,)
+    grad = np.zeros((tree.nboxes, npbox, ndim) + tail if ifpgh >= 2 else 1, np.float64)
+    hess = np.zeros((tree.nboxes, npbox, nhess) + tail if ifpgh >= 3 else 1, np.float64)
     targs = np.zeros(tree.ndim, np.float64)
     pote = np.zeros(nd, np.float64)
+    dum = np.zeros(1, np.float64)
     timeinfo = np.zeros(100, np.float64)
     itree = tree.itree.copy()   # the reference may write colleagues back (periodic, npwlevel<=1)
     ints = [C.c_int(v) for v in (nd, tree.ndim)]
@@ -109,7 +116,9 @@ def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None):
               C.byref(C.c_int(npbox)), C.byref(C.c_int(tree.nboxes)),
               C.byref(C.c_int(tree.nlevels)), C.byref(C.c_int(tree.ltree)),
               _p(itree), _p(tree.iptr), _p(tree.centers), _p(tree.boxsize),
-              _p(f), C.byref(C.c_int(1)), _p(pot), _p(grad), _p(hess),
+              _p(f), C.byref(C.c_int(ifpgh)), _p(pot), _p(grad), _p(hess),
               C.byref(C.c_int(0)), C.byref(C.c_int(0)), _p(targs),
-              C.byref(C.c_int(0)), _p(pote), _p(grad), _p(hess), _p(timeinfo))
-    return pot, timeinfo[:20].copy()
+              C.byref(C.c_int(0)), _p(pote), _p(dum), _p(dum), _p(timeinfo))
+    if ifpgh == 1:
+        return pot, timeinfo[:20].copy()
+    return pot, grad, (hess if ifpgh >= 3 else None), timeinfo[:20].copy()

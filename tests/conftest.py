@@ -44,6 +44,15 @@ class Case:
         self.pot_ref = np.zeros(shape)
         self.pot_ref[self.leaves] = z["pot_leaf"]
         self.gauss = (z["gauss_c"], z["gauss_s"], z["gauss_a"])
+        self.ifpgh = int(z["ifpgh"]) if "ifpgh" in z else 1
+        tail = () if self.nd == 1 else (self.nd,)
+        self.grad_ref = self.hess_ref = None
+        if self.ifpgh >= 2:
+            self.grad_ref = np.zeros((self.nboxes, self.npbox, self.ndim) + tail)
+            self.grad_ref[self.leaves] = z["grad_leaf"]
+        if self.ifpgh >= 3:
+            self.hess_ref = np.zeros((self.nboxes, self.npbox, self.ndim * (self.ndim + 1) // 2) + tail)
+            self.hess_ref[self.leaves] = z["hess_leaf"]
 
     def tree_args(self):
         return (self.nboxes, self.nlevels, self.ltree, self.itree, self.iptr, self.centers, self.boxsize)
@@ -100,6 +109,64 @@ def exact_gauss(points, gauss, delta, iperiod, images=2):
                 term = term * acc * np.sqrt(np.pi * sk * delta / (sk + delta))
         out += term
     return out
+
+
+def _factor_deriv(x, ck, sk, delta, iperiod, order, images=2):
+    """d^order/dx^order of the 1-D factor of exact_gauss (order 0, 1, 2).
+
+    Periodic: the lattice-image sum of Gaussians of variance S = s + delta.
+    Free space: the product of the two Gaussians is A(x) exp(-(y-mu)^2/tau)
+    with A = exp(-(x-c)^2/S), mu = (c delta + x s)/S, tau = s delta/S, so the
+    factor is A(x) B(mu(x)) with B the erf difference over [-1/2, 1/2];
+    B' and B'' are Gaussians at the interval ends.
+    """
+    from scipy.special import erf
+    S = sk + delta
+    if iperiod:
+        acc = np.zeros_like(x)
+        for m in range(-images, images + 1):
+            d = x - ck - m
+            e = np.exp(-d * d / S)
+            acc = acc + (e if order == 0 else (-2 * d / S) * e if order == 1 else (4 * d * d / S ** 2 - 2 / S) * e)
+        return acc * np.sqrt(np.pi * sk * delta / S)
+    tau = sk * delta / S
+    r = sk / S
+    mu = (ck * delta + x * sk) / S
+    A = np.exp(-(x - ck) ** 2 / S)
+    A1 = -2 * (x - ck) / S * A
+    A2 = (4 * (x - ck) ** 2 / S ** 2 - 2 / S) * A
+    B = np.sqrt(np.pi * tau) / 2 * (erf((0.5 - mu) / np.sqrt(tau)) - erf((-0.5 - mu) / np.sqrt(tau)))
+    e1, e2 = np.exp(-(0.5 - mu) ** 2 / tau), np.exp(-(0.5 + mu) ** 2 / tau)
+    B1 = -e1 + e2
+    B2 = -2 * (0.5 - mu) / tau * e1 - 2 * (0.5 + mu) / tau * e2
+    if order == 0:
+        return A * B
+    if order == 1:
+        return A1 * B + A * B1 * r
+    return A2 * B + 2 * A1 * B1 * r + A * B2 * r * r
+
+
+def exact_gauss_deriv(points, gauss, delta, iperiod, orders):
+    """Partial derivative of exact_gauss of the per-axis orders (tuple of 0/1/2)."""
+    c, s, a = gauss
+    out = np.zeros(points.shape[:-1])
+    ndim = points.shape[-1]
+    for g in range(len(a)):
+        term = np.full(points.shape[:-1], a[g])
+        for k in range(ndim):
+            term = term * _factor_deriv(points[..., k], c[g, k], s[g, k], delta, iperiod, orders[k])
+        out += term
+    return out
+
+
+def pgh_orders(ndim):
+    """Per-axis derivative orders of grad and Hessian components, in the
+    reference's component order (Hessian: 1-D xx; 2-D xx,xy,yy; 3-D
+    xx,yy,zz,xy,xz,yz)."""
+    grad = [tuple(1 if a == k else 0 for a in range(ndim)) for k in range(ndim)]
+    hess = {1: [(2,)], 2: [(2, 0), (1, 1), (0, 2)],
+            3: [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)]}[ndim]
+    return grad, hess
 
 
 def gpu_available():

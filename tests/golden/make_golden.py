@@ -9,6 +9,7 @@ Fixtures are data only: tree arrays, inputs and the reference's outputs.
   * boxfgt_<name>.npz : a tree from the reference's vol_tree_mem/vol_tree_build
     (src/common/tree_vol_coeffs_nd.f:83,444), its fvals, and the reference
     boxfgt potential (src/bfgt/boxfgt.f:5) for the stated (eps, delta).
+  * boxfgt_pgh*/pg*.npz : the same with ifpgh = 2/3 (gradients, Hessians).
   * tables.npz : the reference's 1-D tables and scalars (fgtterms.f,
     legeexps.f, chebexps.f, besseljs3d.f, bfgt_volrouts.f) for a grid of
     parameters.
@@ -46,6 +47,18 @@ CASES = {
 }
 
 
+# gradients and Hessians (ifpgh = 2 / 3), small trees:
+# name: (ndim, norder, ipoly, iperiod, tree_eps, eps, delta, seed, ng, width range, nd, ifpgh)
+PGH_CASES = {
+    "pgh3_n6":       (3, 6, 0, 0, 1e-1, 1e-6, 1e-2, 10, 2, (2e-3, 1e-2), 1, 3),
+    "pgh3_per_n8":   (3, 8, 0, 1, 1e-1, 1e-9, 1e-3, 11, 2, (2e-3, 5e-3), 1, 3),
+    "pg3_cheb_nd2":  (3, 6, 1, 0, 1e-1, 1e-6, 3e-3, 12, 2, (2e-3, 1e-2), 2, 2),
+    "pgh2_per_n8":   (2, 8, 0, 1, 1e-6, 1e-6, 1e-3, 13, 4, (1e-3, 1e-2), 1, 3),
+    "pgh2_n12":      (2, 12, 0, 0, 1e-6, 1e-9, 1e-4, 14, 3, (1e-3, 1e-2), 1, 3),
+    "pgh1_n8":       (1, 8, 0, 0, 1e-9, 1e-6, 1e-3, 15, 3, (1e-4, 1e-2), 1, 3),
+}
+
+
 def make_source(ndim, seed, ng, wr, iperiod):
     rng = np.random.default_rng(seed)
     lo, hi = (-0.5, 0.5) if iperiod else (-0.4, 0.4)
@@ -77,6 +90,29 @@ def boxfgt_case(name, spec):
                         delta=delta, nd=nd, tree_eps=teps, gauss_c=c, gauss_s=s, gauss_a=a)
     print(f"{name}: nboxes={tr.nboxes} nlevels={tr.nlevels} |pot|={np.abs(pot).max():.3e} "
           f"size={os.path.getsize(out)/1e6:.2f} MB")
+
+
+def pgh_case(name, spec):
+    ndim, n, ipoly, iper, teps, eps, delta, seed, ng, wr, nd, ifpgh = spec
+    c, s, a = make_source(ndim, seed, ng, wr, iper)
+    dp = refrun.gauss_params(c, s, a)
+    tr = refrun.build_tree(ndim, n, ipoly, iper, teps, dp, ng)
+    f = tr.fvals
+    if nd == 2:
+        f = np.stack([tr.fvals, np.cos(3.0 * tr.fvals)], axis=-1)
+    pot, grad, hess, _ = refrun.boxfgt(tr, delta, eps, nd=nd, fvals=f, ifpgh=ifpgh)
+    out = os.path.join(HERE, f"boxfgt_{name}.npz")
+    nch = tr.itree[tr.iptr[3] - 1: tr.iptr[3] - 1 + tr.nboxes]
+    leaves = np.nonzero(nch == 0)[0].astype(np.int32)
+    extra = {} if hess is None else {"hess_leaf": hess[leaves]}
+    np.savez_compressed(out, ndim=ndim, norder=n, ipoly=ipoly, iperiod=iper, nboxes=tr.nboxes,
+                        nlevels=tr.nlevels, ltree=tr.ltree, itree=tr.itree, iptr=tr.iptr,
+                        centers=tr.centers, boxsize=tr.boxsize, leaves=leaves,
+                        fvals_leaf=f[leaves], pot_leaf=pot[leaves], grad_leaf=grad[leaves],
+                        ifpgh=ifpgh, eps=eps, delta=delta, nd=nd, tree_eps=teps, gauss_c=c, gauss_s=s,
+                        gauss_a=a, **extra)
+    print(f"{name}: nboxes={tr.nboxes} nlevels={tr.nlevels} leaves={len(leaves)} "
+          f"|grad|={np.abs(grad).max():.3e} size={os.path.getsize(out)/1e6:.2f} MB")
 
 
 def _dp(x):
@@ -148,6 +184,8 @@ def tables():
             out[f"ts_{key}"] = ts
             out[f"p2pw_{key}"] = t1
             out[f"pw2pot_{key}"] = t2
+            out[f"pw2der_{key}"] = t2x
+            out[f"pw2dxx_{key}"] = t2xx
             if iper == 0:
                 nt = 6
                 tl = np.zeros(n * n * 13); tlx = np.zeros(n * n * 13); tlxx = np.zeros(n * n * 13)
@@ -155,6 +193,8 @@ def tables():
                 L.mk_loctab_all_(_dp(eps), _ip(ipoly), _ip(n), _ip(50), _dp(delta), _dp(boxdim), _ip(0),
                                  _ip(nt), P(tl), P(tlx), P(tlxx), P(ind))
                 out[f"loctab_{key}"] = tl
+                out[f"loctabx_{key}"] = tlx
+                out[f"loctabxx_{key}"] = tlxx
                 out[f"locind_{key}"] = ind
     path = os.path.join(HERE, "tables.npz")
     np.savez_compressed(path, **out)
@@ -169,3 +209,6 @@ if __name__ == "__main__":
     for name, spec in CASES.items():
         if not only or name in only:
             boxfgt_case(name, spec)
+    for name, spec in PGH_CASES.items():
+        if not only or name in only:
+            pgh_case(name, spec)
