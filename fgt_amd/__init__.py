@@ -60,6 +60,8 @@ def lib():
         L.fgt_solver_create.restype = C.c_int
         L.fgt_solver_fvals_device.restype = C.c_void_p
         L.fgt_solver_pot_device.restype = C.c_void_p
+        L.fgt_solver_grad_device.restype = C.c_void_p
+        L.fgt_solver_hess_device.restype = C.c_void_p
         L.fgt_plan_direct_pairs.restype = C.c_longlong
         _lib = L
     return _lib
@@ -97,9 +99,12 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
 
     Arrays use the reference's Fortran memory order; with C-ordered numpy
     arrays that means ``centers[nboxes, ndim]``, ``fvals[nboxes, npbox(, nd)]``
-    and ``pot`` of the same shape as ``fvals`` (accumulated, ``+=``).
-    Raises :class:`FgtError` for unsupported options (ifpgh > 1,
-    ifpghtarg > 0, ifnewtree = 1) instead of silently returning.
+    and ``pot`` of the same shape as ``fvals`` (accumulated, ``+=``);
+    ``grad[nboxes, npbox, ndim(, nd)]`` for ifpgh >= 2 and
+    ``hess[nboxes, npbox, ndim*(ndim+1)/2(, nd)]`` for ifpgh = 3, also
+    accumulated (Hessian order 1-D xx; 2-D xx, xy, yy; 3-D xx, yy, zz, xy,
+    xz, yz).  Raises :class:`FgtError` for unsupported options (ifpghtarg > 0,
+    ifnewtree = 1) instead of silently returning.
     """
     itree = np.ascontiguousarray(itree, np.int32)
     iptr = np.ascontiguousarray(iptr, np.int32)
@@ -110,6 +115,12 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
         raise TypeError("pot must be a C-contiguous float64 array (it is accumulated in place)")
     if pot.size != nd * npbox * nboxes or fvals.size != nd * npbox * nboxes:
         raise ValueError("fvals/pot must hold nd*npbox*nboxes values")
+    for arr, need, nm in ((grad, 2, "grad"), (hess, 3, "hess")):
+        if ifpgh >= need:
+            ncomp = ndim if nm == "grad" else ndim * (ndim + 1) // 2
+            if not (isinstance(arr, np.ndarray) and arr.dtype == np.float64 and arr.flags.c_contiguous
+                    and arr.size == nd * ncomp * npbox * nboxes):
+                raise TypeError(f"{nm} must be a C-contiguous float64 array of nd*{ncomp}*npbox*nboxes values")
     dummy = np.zeros(1, np.float64)
     ti = np.zeros(20, np.float64) if timeinfo is None else timeinfo
     rc = lib().fgt_boxfgt(_i(nd), _i(ndim), _d(delta), _d(eps), _i(ipoly), _i(iperiod),
@@ -154,6 +165,27 @@ class Solver:
 
     def execute(self, stream=None, timed=False):
         _check(lib().fgt_solver_execute(self._h, C.c_void_p(stream or 0), C.c_int(1 if timed else 0)))
+
+    def set_ifpgh(self, ifpgh):
+        """1 potential, 2 + gradient, 3 + Hessian for later executes."""
+        _check(lib().fgt_solver_set_ifpgh(self._h, C.c_int(ifpgh)))
+        self.ifpgh = ifpgh
+
+    def _shape(self, ncomp):
+        tail = () if self.nd == 1 else (self.nd,)
+        return (self.nboxes, self.npbox, ncomp) + tail
+
+    def get_grad(self, out=None, accumulate=False):
+        if out is None:
+            out = np.zeros(self._shape(self.ndim))
+        _check(lib().fgt_solver_get_grad(self._h, _p(out), C.c_int(1 if accumulate else 0)))
+        return out
+
+    def get_hess(self, out=None, accumulate=False):
+        if out is None:
+            out = np.zeros(self._shape(self.ndim * (self.ndim + 1) // 2))
+        _check(lib().fgt_solver_get_hess(self._h, _p(out), C.c_int(1 if accumulate else 0)))
+        return out
 
     def get_pot(self, out=None, accumulate=False):
         if out is None:

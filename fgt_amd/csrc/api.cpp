@@ -61,10 +61,14 @@ struct fgt_solver {
     DBuf d_p2pw, d_pw2pot, d_pw2potT, d_pw2potS, d_wms, d_wm2l, d_one, d_loctab, d_locband;
     // data
     DBuf d_fvals, d_pot, d_expn;
+    int ifpgh = 1;          // 1 pot, 2 + grad, 3 + hess
+    DBuf d_grad, d_hess;    // grad(nd,ndim,npbox,nboxes), hess(nd,nhess,npbox,nboxes)
     // lists
     DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
+    DBuf d_m2l_nbr;            // [target][27] source slot per offset, or -1
+    bool m2l27 = false;        // every offset holds at most one source
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
@@ -160,6 +164,8 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     t.p2pw = S.d_p2pw.as<double2>();
     t.pw2pot = S.d_pw2pot.as<double2>();
     t.pw2potT = S.d_pw2potT.as<double2>();
+    t.pw2pot_ks = P.pw2pot_ks;
+    t.loctab_ks = P.loctab_ks;
     t.lds_limit = prop.sharedMemPerBlock;
     t.n = n;
     t.npw = P.npw;
@@ -167,21 +173,27 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     if (S.fast3d) {
         const int nl = P.nlevels, np = P.npw;
         const size_t per = (size_t)n * np;
-        std::vector<fgt::cplx> ts(3 * per * (nl + 1), fgt::cplx(0.0));
-        for (int l = P.nlevstart; l <= nl; ++l) {
-            const int klev = nl - (l - 1);
-            const fgt::cplx* w = (l > P.nlevstart) ? &P.wms[(size_t)(klev - 1) * np] : nullptr;
-            for (int k = 0; k < np; ++k)
-                for (int i = 0; i < n; ++i) {
-                    const fgt::cplx u = P.pw2pot[l * per + (size_t)i * np + k];
-                    ts[(3 * (size_t)l + 0) * per + (size_t)k * n + i] = u;
-                    if (w) {
-                        ts[(3 * (size_t)l + 1) * per + (size_t)k * n + i] = u * w[k];
-                        ts[(3 * (size_t)l + 2) * per + (size_t)k * n + i] = u * std::conj(w[k]);
+        // [kind][l][v][k][i], kind = value / first / second derivative table
+        const size_t sks = 3 * per * (nl + 1);
+        std::vector<fgt::cplx> ts(3 * sks, fgt::cplx(0.0));
+        for (int kind = 0; kind < 3; ++kind)
+            for (int l = P.nlevstart; l <= nl; ++l) {
+                const int klev = nl - (l - 1);
+                const fgt::cplx* w = (l > P.nlevstart) ? &P.wms[(size_t)(klev - 1) * np] : nullptr;
+                const fgt::cplx* U = &P.pw2pot[kind * P.pw2pot_ks + l * per];
+                fgt::cplx* o = &ts[kind * sks + 3 * (size_t)l * per];
+                for (int k = 0; k < np; ++k)
+                    for (int i = 0; i < n; ++i) {
+                        const fgt::cplx u = U[(size_t)i * np + k];
+                        o[(size_t)k * n + i] = u;
+                        if (w) {
+                            o[per + (size_t)k * n + i] = u * w[k];
+                            o[2 * per + (size_t)k * n + i] = u * std::conj(w[k]);
+                        }
                     }
-                }
-        }
+            }
         S.d_pw2potS.upload(to_d2(ts));
+        t.pw2potS_ks = sks;
         const fgt::TreeView& T = P.tree;
         std::vector<int> src(P.leaf_box.size()), sel(P.leaf_box.size());
         for (size_t i = 0; i < P.leaf_box.size(); ++i) {
@@ -341,6 +353,18 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     std::vector<int> mi(P.m2l_idx.size());
     for (size_t i = 0; i < mi.size(); ++i) mi[i] = map_m2l(ndim, P.m2l_idx[i]);
     S.d_m2l_idx.upload(mi);
+    {
+        std::vector<int> nbr(P.m2l_tgt.size() * 27, -1);
+        bool uniq = true;
+        for (size_t tk = 0; tk < P.m2l_tgt.size() && uniq; ++tk)
+            for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
+                int& slot = nbr[tk * 27 + mi[j]];
+                if (slot >= 0) { uniq = false; break; }
+                slot = P.m2l_src[j];
+            }
+        S.m2l27 = uniq;
+        if (uniq) S.d_m2l_nbr.upload(nbr);
+    }
     S.d_dir_tgt.upload(P.dir_tgt);
     S.d_dir_lev.upload(P.dir_lev);
     S.d_dir_off.upload(P.dir_off);
@@ -370,6 +394,41 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     S.d_dir_flags.upload(df);
 }
 
+// Output components of ifpgh and the table kind each uses per axis
+// (gnd_pw2pgh, bfgt_pwrouts.f:233-1065; gnd_tens_prod_to_pghloc,
+// bfgt_local.f:14-920): pot; grad x, y, z; Hessian in the reference's order
+// (1-D xx; 2-D xx, xy, yy; 3-D xx, yy, zz, xy, xz, yz).  Reference axes map
+// onto the kernels' (x, y, z) like the geometry: 2-D (x, y) -> (x, z),
+// 1-D x -> z.
+struct Comp {
+    fgt::Kinds kd;
+    fgt::OutDesc od;
+};
+
+std::vector<Comp> components(fgt_solver& S) {
+    const fgt::Plan& P = S.plan;
+    const int ndim = P.ndim, nd = P.nd, nhess = ndim * (ndim + 1) / 2;
+    auto axes = [&](int r0, int r1, int r2) {
+        if (ndim == 3) return fgt::Kinds{r0, r1, r2};
+        if (ndim == 2) return fgt::Kinds{r0, 0, r1};
+        return fgt::Kinds{0, 0, r0};
+    };
+    std::vector<Comp> c;
+    c.push_back({axes(0, 0, 0), {S.d_pot.as<double>(), nd, 0}});
+    if (S.ifpgh >= 2)
+        for (int a = 0; a < ndim; ++a)
+            c.push_back({axes(a == 0, a == 1, a == 2), {S.d_grad.as<double>(), nd * ndim, a * nd}});
+    if (S.ifpgh >= 3) {
+        static const int h1[1][3] = {{2, 0, 0}};
+        static const int h2[3][3] = {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}};
+        static const int h3[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+        const int(*h)[3] = ndim == 1 ? h1 : (ndim == 2 ? h2 : h3);
+        for (int k = 0; k < nhess; ++k)
+            c.push_back({axes(h[k][0], h[k][1], h[k][2]), {S.d_hess.as<double>(), nd * nhess, k * nd}});
+    }
+    return c;
+}
+
 void execute(fgt_solver& S, hipStream_t st, bool timed) {
     const fgt::Plan& P = S.plan;
     const fgt::Geom& g = S.geom;
@@ -379,6 +438,9 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
     auto mark = [&](int i) { if (timed) hcheck(hipEventRecord(ev[i], st), "hipEventRecord"); };
     hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
+    if (S.ifpgh >= 2) hcheck(hipMemsetAsync(S.d_grad.p, 0, S.d_grad.bytes, st), "hipMemsetAsync grad");
+    if (S.ifpgh >= 3) hcheck(hipMemsetAsync(S.d_hess.p, 0, S.d_hess.bytes, st), "hipMemsetAsync hess");
+    const std::vector<Comp> comps = components(S);
     mark(0);
     if (P.do_pw) {
         if (S.fast3d) {
@@ -416,8 +478,12 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
             mark(2);
         }
         // STEP 3
-        fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
-                        S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);
+        if (S.m2l27)
+            fgt::launch_m2l27(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_nbr.as<int>(),
+                              S.d_expn.as<double2>(), st);
+        else
+            fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
+                            S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)
@@ -433,23 +499,25 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
             }
         }
         mark(4);
-        // STEP 5
-        if (S.fast3d)
-            fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                               S.d_s5_src.as<int>(), S.d_s5_sel.as<int>(), S.d_expn.as<double2>(),
-                               S.d_pot.as<double>(), st);
-        else
-            fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                               S.d_leaf_slot.as<int>(), nullptr, S.d_expn.as<double2>(), S.d_pot.as<double>(),
-                               st);
+        // STEP 5, once per output component (pot, grad, hess entries)
+        for (const Comp& c : comps) {
+            if (S.fast3d)
+                fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                                   S.d_s5_src.as<int>(), S.d_s5_sel.as<int>(), S.d_expn.as<double2>(), c.kd,
+                                   c.od, st);
+            else
+                fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
+                                   S.d_leaf_slot.as<int>(), nullptr, S.d_expn.as<double2>(), c.kd, c.od, st);
+        }
         mark(5);
     } else {
         for (int i = 1; i <= 5; ++i) mark(i);
     }
     // STEP 6
-    fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
-                       S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
-                       S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), S.d_pot.as<double>(), st);
+    for (const Comp& c : comps)
+        fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
+                           S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
+                           S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, st);
     mark(6);
     if (timed) {
         hcheck(hipEventSynchronize(ev[6]), "hipEventSynchronize");
@@ -564,6 +632,50 @@ int fgt_solver_stats_get(const fgt_solver* s, fgt_solver_stats* o) {
     return 0;
 }
 
+int fgt_solver_set_ifpgh(fgt_solver* s, int ifpgh) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null solver");
+        if (ifpgh < 1 || ifpgh > 3) throw std::invalid_argument("ifpgh must be 1, 2 or 3");
+        const fgt::Plan& P = s->plan;
+        const size_t nodes = (size_t)P.nd * P.npbox * P.nboxes;
+        if (ifpgh >= 2 && !s->d_grad.p) s->d_grad.alloc(sizeof(double) * nodes * P.ndim);
+        if (ifpgh >= 3 && !s->d_hess.p) s->d_hess.alloc(sizeof(double) * nodes * (P.ndim * (P.ndim + 1) / 2));
+        s->ifpgh = ifpgh;
+    });
+}
+
+namespace {
+void download(const DBuf& b, double* out, int accumulate, const char* what) {
+    if (!accumulate) {
+        hcheck(hipMemcpy(out, b.p, b.bytes, hipMemcpyDeviceToHost), what);
+        return;
+    }
+    const size_t cnt = b.bytes / sizeof(double);
+    std::vector<double> tmp(cnt);
+    hcheck(hipMemcpy(tmp.data(), b.p, b.bytes, hipMemcpyDeviceToHost), what);
+    for (size_t i = 0; i < cnt; ++i) out[i] += tmp[i];
+}
+}  // namespace
+
+int fgt_solver_get_grad(fgt_solver* s, double* grad, int accumulate) {
+    return guarded([&] {
+        if (!s || !grad) throw std::invalid_argument("null argument");
+        if (s->ifpgh < 2) throw std::invalid_argument("gradients not requested (fgt_solver_set_ifpgh)");
+        download(s->d_grad, grad, accumulate, "download grad");
+    });
+}
+
+int fgt_solver_get_hess(fgt_solver* s, double* hess, int accumulate) {
+    return guarded([&] {
+        if (!s || !hess) throw std::invalid_argument("null argument");
+        if (s->ifpgh < 3) throw std::invalid_argument("Hessians not requested (fgt_solver_set_ifpgh)");
+        download(s->d_hess, hess, accumulate, "download hess");
+    });
+}
+
+void* fgt_solver_grad_device(fgt_solver* s) { return s ? s->d_grad.p : nullptr; }
+void* fgt_solver_hess_device(fgt_solver* s) { return s ? s->d_hess.p : nullptr; }
+
 void* fgt_solver_fvals_device(fgt_solver* s) { return s ? s->d_fvals.p : nullptr; }
 void* fgt_solver_pot_device(fgt_solver* s) { return s ? s->d_pot.p : nullptr; }
 
@@ -574,7 +686,7 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
                double* boxsize, double* fvals, int* ifpgh, double* pot, double* grad, double* hess,
                int* ifnewtree, int* ntarg, double* targs, int* ifpghtarg, double* pote, double* grade,
                double* hesse, double* timeinfo) {
-    (void)grad; (void)hess; (void)ntarg; (void)targs; (void)pote; (void)grade; (void)hesse;
+    (void)ntarg; (void)targs; (void)pote; (void)grade; (void)hesse;
     if (timeinfo)
         for (int i = 0; i < 20; ++i) timeinfo[i] = 0.0;
     return guarded([&] {
@@ -582,7 +694,9 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
             !nlevels || !ltree || !itree || !iptr || !centers || !boxsize || !fvals || !ifpgh || !pot ||
             !ifnewtree || !ifpghtarg)
             throw std::invalid_argument("null argument");
-        if (*ifpgh != 1) throw std::invalid_argument("only ifpgh = 1 (potential) is supported");
+        if (*ifpgh < 1 || *ifpgh > 3) throw std::invalid_argument("ifpgh must be 1, 2 or 3");
+        if (*ifpgh >= 2 && !grad) throw std::invalid_argument("null grad");
+        if (*ifpgh >= 3 && !hess) throw std::invalid_argument("null hess");
         if (*ifpghtarg != 0) throw std::invalid_argument("extra targets (ifpghtarg >= 1) are not supported");
         if (*ifnewtree != 0) throw std::invalid_argument("ifnewtree = 1 is not supported");
         const double t0 = now_s();
@@ -590,15 +704,19 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
                           *ltree, itree, iptr, centers, boxsize};
         fgt_solver S;
         build_solver(S, in);
+        if (*ifpgh > 1) {
+            const int rc = fgt_solver_set_ifpgh(&S, *ifpgh);
+            if (rc) throw std::runtime_error(g_msg);
+        }
         hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
         const double t1 = now_s();
         execute(S, nullptr, timeinfo != nullptr);
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const double t2 = now_s();
-        const size_t cnt = S.d_pot.bytes / sizeof(double);
-        std::vector<double> tmp(cnt);
-        hcheck(hipMemcpy(tmp.data(), S.d_pot.p, S.d_pot.bytes, hipMemcpyDeviceToHost), "download pot");
-        for (size_t i = 0; i < cnt; ++i) pot[i] += tmp[i];
+        // accumulate into the caller's arrays (+=), as the reference does
+        download(S.d_pot, pot, 1, "download pot");
+        if (*ifpgh >= 2) download(S.d_grad, grad, 1, "download grad");
+        if (*ifpgh >= 3) download(S.d_hess, hess, 1, "download hess");
         const double t3 = now_s();
         if (timeinfo) {
             timeinfo[0] = t1 - t0;

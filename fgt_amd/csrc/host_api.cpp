@@ -52,6 +52,33 @@ void fgt_host_loctab(double eps, int ipoly, int n, double delta, double boxdim, 
     }
 }
 
+void fgt_host_pw2pgh(int ipoly, int n, int npw, const double* ts, double delta, double boxdim, double* tab,
+                     double* tabx, double* tabxx) {
+    std::vector<double> x(n), w(n), u(static_cast<size_t>(n) * n);
+    fgt::poly_exps(ipoly, n, x.data(), w.data(), u.data());
+    const size_t m = static_cast<size_t>(n) * npw;
+    std::vector<fgt::cplx> t(3 * m);
+    fgt::make_pw2pgh(n, npw, ts, x.data(), delta, boxdim, t.data(), t.data() + m, t.data() + 2 * m);
+    std::memcpy(tab, t.data(), sizeof(double) * 2 * m);
+    std::memcpy(tabx, t.data() + m, sizeof(double) * 2 * m);
+    std::memcpy(tabxx, t.data() + 2 * m, sizeof(double) * 2 * m);
+}
+
+void fgt_host_loctab_der(double eps, int ipoly, int n, double delta, double boxdim, double* tabx,
+                         double* tabxx) {
+    const size_t nn = static_cast<size_t>(n) * n;
+    std::vector<double> t(nn * fgt::kLocSlots), tx(nn * fgt::kLocSlots), txx(nn * fgt::kLocSlots);
+    std::vector<int> band(static_cast<size_t>(2) * n * fgt::kLocSlots), cols(2 * fgt::kLocSlots);
+    fgt::make_loctab_level(eps, ipoly, n, delta, boxdim, t.data(), band.data(), cols.data(), tx.data(),
+                           txx.data());
+    for (int s = 0; s < fgt::kLocSlots; ++s)
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j) {
+                tabx[s * nn + j + static_cast<size_t>(k) * n] = tx[(s * n + k) * n + j];
+                tabxx[s * nn + j + static_cast<size_t>(k) * n] = txx[(s * n + k) * n + j];
+            }
+}
+
 static int plan_for(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int nboxes,
                     int nlevels, int ltree, const int* itree, const int* iptr, const double* centers,
                     const double* boxsize, fgt::Plan& P) {

@@ -34,16 +34,31 @@ __device__ __forceinline__ int xcd_task(int b, int nb) {
 }
 #endif
 
+// One output component of STEP 5 / STEP 6 (potential, one gradient or one
+// Hessian entry): value of node `node` of box b, density ind, goes to
+// p[(b*npbox + node)*ns + coff + ind] (ns = nd * components of the array).
+struct OutDesc {
+    double* p;
+    int ns;
+    int coff;
+};
+
+// Per kernel axis (x, y, z): which 1-D table kind a component uses
+// (0 value, 1 first derivative, 2 second derivative).
+struct Kinds {
+    int x, y, z;
+};
+
 // Device-resident tables (all pointers are device pointers).
 struct DevTables {
     // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)
     const double2* p2pw;
-    // per level l: PW->value  U[l][i][k] (i < n, k < npw), and transposed
-    // UT[l][k][i] (the register-blocked 3-D kernels read it wave-uniformly)
+    // per kind and level l: PW->value  U[kind][l][i][k] (i < n, k < npw),
+    // and transposed UT[kind][l][k][i]
     const double2* pw2pot;
     const double2* pw2potT;
     // pw2potT and its products with the parent->child split phase of each
-    // level: pw2potS[l][v][k][i], v = 0 plain, 1 = + phase, 2 = conjugate phase
+    // level: pw2potS[kind][l][v][k][i], v = 0 plain, 1 = + phase, 2 = conjugate
     const double2* pw2potS;
     // p2pw and its products with the child->parent merge phase of each level:
     // p2pwS[l][v][k][m], v = 0 plain, 1 = + phase, 2 = conjugate phase
@@ -53,9 +68,11 @@ struct DevTables {
     const double2* wm2l;
     // identity table for unit axes (one complex 1.0)
     const double2* one;
-    // near-field masked tables loc[l][slot][k][j], bands [l][slot][k][2]
+    // near-field masked tables loc[kind][l][slot][k][j], bands [l][slot][k][2]
     const double* loctab;
     const int* locband;
+    // kind strides (elements): pw2pot / pw2potT, pw2potS, loctab
+    size_t pw2pot_ks, pw2potS_ks, loctab_ks;
     int n;     // norder
     int npw;
     size_t lds_limit;  // device LDS bytes per block
@@ -67,8 +84,8 @@ bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box
                     const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
 // sel: per task 0 (own loc) or 1 | split-sign bits << 1 (evaluate the parent's loc)
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s,
-                      size_t lds_limit);
+                      const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od,
+                      hipStream_t s, size_t lds_limit);
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 // Fused STEP 1 + 2 for parents with leaf children (3-D, N in {8,12,16}):
 // plev/pslot per parent, kids/ksg 8 per parent (leaf child box or -1, merge
@@ -87,19 +104,23 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                 const int* off, const int* src, const int* idx, double2* expn,
                 hipStream_t s);
+// STEP 3 over a per-target 27-offset neighbour table nbr[task][o] (slot or -1;
+// at most one source per offset)
+void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* nbr, double2* expn,
+                  hipStream_t s);
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
                    const int* lev, const int* slot, const int* sel, const double2* expn,
-                   double* pot, hipStream_t s);
+                   Kinds kd, OutDesc od, hipStream_t s);
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                    const int* lev, const int* off, const int* src, const int* code,
-                   const int* flags, const double* fvals, double* pot, hipStream_t s);
+                   const int* flags, const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
-                      const double* fvals, double* pot, hipStream_t s);
+                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
 
 }  // namespace fgt

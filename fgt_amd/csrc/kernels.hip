@@ -219,6 +219,65 @@ __global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
 }
 #undef M2L_PHASE
 
+// STEP 3 with a per-target table of the 3^3 stencil: nbr[task][o] is the
+// source slot at offset o = ox + 3 oy + 9 oz (or -1).  A block owns one
+// element range (one element per thread) and kM2lGroup consecutive targets
+// (Morton neighbours, so their sources repeat and hit in L2).  Each thread
+// forms the nine products w_x w_y and three w_z of its element once and
+// reuses them for every target of the group; per list entry only the source
+// load and two complex products remain.  The offsets are unrolled, so the
+// phase operands are fixed registers and the validity tests are wave-uniform
+// branches; the nine loads of one z plane are issued before their use.
+constexpr int kM2lGroup = 8;
+
+__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup,
+                                                  const int* __restrict__ tgt,
+                                                  const int* __restrict__ nbr,
+                                                  double2* __restrict__ expn) {
+    const int lin = xcd_task(blockIdx.x, gridDim.x);
+    const int grp = lin % ngroup, chunk = lin / ngroup;
+    const long long tot = (long long)g.nd * g.nexp;
+    const int npw = t.npw;
+    long long e = (long long)chunk * blockDim.x + threadIdx.x;
+    const bool ok = e < tot;
+    if (!ok) e = tot - 1;   // valid duplicate, never stored
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    double2 fxy[9], fz[3];
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy) {
+        const double2 wy = g.py > 1 ? t.wm2l[oy * npw + k2] : make_double2(1.0, 0.0);
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox) {
+            const double2 wx = g.px > 1 ? t.wm2l[ox * npw + k1] : make_double2(1.0, 0.0);
+            fxy[ox + 3 * oy] = cmul(wx, wy);
+        }
+    }
+#pragma unroll
+    for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
+    for (int q = 0; q < kM2lGroup; ++q) {
+        const int task = grp * kM2lGroup + q;
+        if (task >= ntask) break;
+        const int ts = tgt[task];
+        const int* __restrict__ nb = nbr + (size_t)task * 27;
+        double2 acc = expn[(size_t)ts * 2 * tot + e];
+#pragma unroll
+        for (int oz = 0; oz < 3; ++oz) {
+            int sv[9];
+            double2 v[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) sv[i] = nb[oz * 9 + i];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                if (sv[i] >= 0) v[i] = expn[(size_t)sv[i] * 2 * tot + e];
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                if (sv[i] >= 0) acc = cfma(v[i], cmul(fxy[i], fz[oz]), acc);
+        }
+        if (ok) expn[((size_t)ts * 2 + 1) * tot + e] = acc;
+    }
+}
+
 // ---------------------------------------------------------------- STEP 4
 __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
                                                   const int* __restrict__ pc_child,
@@ -236,19 +295,24 @@ __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int
 }
 
 // ---------------------------------------------------------------- STEP 5
+// Tables per axis by kind (kd); the z table is staged in LDS, the x/y
+// tables come from LDS when they are the same kind, else from global memory.
 __global__ void __launch_bounds__(kBlock) k_pw2pot(Geom g, DevTables t, const int* __restrict__ box,
                                                    const int* __restrict__ lev,
                                                    const int* __restrict__ slot,
                                                    const double2* __restrict__ expn,
-                                                   double* __restrict__ pot) {
+                                                   Kinds kd, OutDesc od) {
     extern __shared__ double2 sm[];
     const int task = blockIdx.x;
     const int b = box[task], l = lev[task], s = slot[task];
     const int n = t.n, npw = t.npw;
     double2* sU = sm;                 // [n][npw]
     double2* ff = sU + n * npw;       // [kk1][k2]  nx*py
-    const double2* U = t.pw2pot + (size_t)l * n * npw;
+    const size_t lo = (size_t)l * n * npw;
+    const double2* U = t.pw2pot + kd.z * t.pw2pot_ks + lo;
     for (int i = threadIdx.x; i < n * npw; i += blockDim.x) sU[i] = U[i];
+    const double2* Ux = kd.x == kd.z ? sU : t.pw2pot + kd.x * t.pw2pot_ks + lo;
+    const double2* Uy = kd.y == kd.z ? sU : t.pw2pot + kd.y * t.pw2pot_ks + lo;
     const int nxy = g.nx * g.ny;
     const int kk1 = threadIdx.x % g.nx, kk2 = threadIdx.x / g.nx;
     const bool halve_last = (npw % 2) == 1;
@@ -265,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) k_pw2pot(Geom g, DevTables t, const in
                 if (g.nx == 1) {
                     v = Ek[k2];
                 } else {
-                    for (int k1 = 0; k1 < g.px; ++k1) v = cfma(sU[a1 * npw + k1], Ek[(size_t)k1 * g.py + k2], v);
+                    for (int k1 = 0; k1 < g.px; ++k1) v = cfma(Ux[a1 * npw + k1], Ek[(size_t)k1 * g.py + k2], v);
                 }
                 ff[a1 * g.py + k2] = v;
             }
@@ -275,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) k_pw2pot(Geom g, DevTables t, const in
                 if (g.ny == 1) {
                     v = ff[kk1 * g.py];
                 } else {
-                    for (int k2 = 0; k2 < g.py; ++k2) v = cfma(sU[kk2 * npw + k2], ff[kk1 * g.py + k2], v);
+                    for (int k2 = 0; k2 < g.py; ++k2) v = cfma(Uy[kk2 * npw + k2], ff[kk1 * g.py + k2], v);
                 }
                 const double wgt = (halve_last && k3 == g.pz - 1) ? 0.5 : 1.0;
                 for (int kk3 = 0; kk3 < g.nz; ++kk3) {
@@ -286,7 +350,8 @@ __global__ void __launch_bounds__(kBlock) k_pw2pot(Geom g, DevTables t, const in
         }
         if ((int)threadIdx.x < nxy)
             for (int kk3 = 0; kk3 < g.nz; ++kk3)
-                pot[((size_t)b * g.npbox + kk1 + g.nx * (kk2 + g.ny * kk3)) * g.nd + ind] += 2.0 * acc[kk3];
+                od.p[((size_t)b * g.npbox + kk1 + g.nx * (kk2 + g.ny * kk3)) * od.ns + od.coff + ind] +=
+                    2.0 * acc[kk3];
     }
 }
 
@@ -300,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
                                                    const int* __restrict__ src,
                                                    const int* __restrict__ code,
                                                    const double* __restrict__ fvals,
-                                                   double* __restrict__ pot) {
+                                                   Kinds kd, OutDesc od) {
     extern __shared__ double dsm[];
     const int task = blockIdx.x;
     const int tb = tgt[task], l = lev[task];
@@ -311,7 +376,7 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
     double* sA = h + nxy;              // 3 tables [k][j]  n*n each
     int* sB = (int*)(sA + 3 * n * n);  // 3 bands [k][2]
     const size_t nn = (size_t)n * n;
-    const double* tabl = t.loctab + (size_t)l * 13 * nn;
+    const int kax[3] = {kd.x, kd.y, kd.z};
     const int* bandl = t.locband + (size_t)l * 13 * 2 * n;
     const int k1 = threadIdx.x % g.nx, k2 = threadIdx.x / g.nx;
     for (int ind = 0; ind < g.nd; ++ind) {
@@ -324,7 +389,7 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
                 sf[i] = fvals[((size_t)sb * g.npbox + i) * g.nd + ind];
             for (int a = 0; a < 3; ++a) {
                 const int slot = (c >> (4 * a)) & 15;
-                const double* A = tabl + (size_t)slot * nn;
+                const double* A = t.loctab + kax[a] * t.loctab_ks + ((size_t)l * 13 + slot) * nn;
                 const int* B = bandl + (size_t)slot * 2 * n;
                 for (int i = threadIdx.x; i < n * n; i += blockDim.x) sA[a * nn + i] = A[i];
                 for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) sB[a * 2 * n + i] = B[i];
@@ -362,7 +427,7 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
         }
         if ((int)threadIdx.x < nxy)
             for (int k3 = 0; k3 < g.nz; ++k3)
-                pot[((size_t)tb * g.npbox + k1 + g.nx * (k2 + g.ny * k3)) * g.nd + ind] += acc[k3];
+                od.p[((size_t)tb * g.npbox + k1 + g.nx * (k2 + g.ny * k3)) * od.ns + od.coff + ind] += acc[k3];
     }
 }
 
@@ -408,6 +473,16 @@ void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt, co
     check(hipGetLastError(), "k_m2l launch");
 }
 
+void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* nbr, double2* expn,
+                  hipStream_t s) {
+    if (ntask <= 0) return;
+    const int ngroup = (ntask + kM2lGroup - 1) / kM2lGroup;
+    const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
+    if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
+    hipLaunchKernelGGL(k_m2l27, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, tgt, nbr, expn);
+    check(hipGetLastError(), "k_m2l27 launch");
+}
+
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s) {
     if (nentry <= 0) return;
@@ -417,29 +492,29 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
 }
 
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                   const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s) {
+                   const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;
-    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, sel, expn, pot, s, t.lds_limit)) {
+    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, t.lds_limit)) {
         check(hipGetLastError(), "k_pw2pot3 launch");
         return;
     }
     const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
-    hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, pot);
+    hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, kd, od);
     check(hipGetLastError(), "k_pw2pot launch");
 }
 
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                    const int* off, const int* src, const int* code, const int* flags, const double* fvals,
-                   double* pot, hipStream_t s) {
+                   Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;
-    if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s)) {
+    if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s)) {
         check(hipGetLastError(), "k_direct3 launch");
         return;
     }
     const size_t lds = sizeof(double) * ((size_t)g.npbox + (size_t)g.nx * g.ny + 3 * (size_t)t.n * t.n) +
                        sizeof(int) * 6 * (size_t)t.n;
     hipLaunchKernelGGL(k_direct, dim3(ntask), dim3(kBlock), lds, s, g, t, tgt, lev, off, src, code, fvals,
-                       pot);
+                       kd, od);
     check(hipGetLastError(), "k_direct launch");
 }
 

@@ -25,12 +25,16 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
                                                 const int* __restrict__ tgt, const int* __restrict__ lev,
                                                 const int* __restrict__ off, const int* __restrict__ src,
                                                 const int* __restrict__ code, const int* __restrict__ flags,
-                                                const double* __restrict__ fvals, double* __restrict__ pot) {
+                                                const double* __restrict__ fvals, size_t ks, Kinds kd,
+                                                OutDesc od) {
     extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
     constexpr int NN = N * N;
     const int task = blockIdx.x;
     const int tb = tgt[task], l = lev[task];
-    const double* __restrict__ tabl = loctab + (size_t)l * 13 * NN;
+    // table kind per axis (value / first / second derivative): stride ks
+    const double* __restrict__ tabx = loctab + kd.x * ks + (size_t)l * 13 * NN;
+    const double* __restrict__ taby = loctab + kd.y * ks + (size_t)l * 13 * NN;
+    const double* __restrict__ tabz = loctab + kd.z * ks + (size_t)l * 13 * NN;
     const int tid = threadIdx.x;
     const bool act = tid < NN;
     // x stage: thread = (j2, j3); y stage: thread = (k1, j3); z stage: thread = (k1, k2)
@@ -43,7 +47,7 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
         for (int i = 0; i < N; ++i) X[i] = Y[i] = acc[i] = 0.0;
         for (int j = j0; j < j1e; ++j) {
             const int sb = src[j], c = code[j], fl = flags[j];
-            const double* __restrict__ Ax = tabl + (size_t)(c & 15) * NN;
+            const double* __restrict__ Ax = tabx + (size_t)(c & 15) * NN;
             // ---- x: X[k1] += sum_j1 Ax[k1][j1] f[j1, j2=a, j3=bq]
             if (act) {
                 double f[N];
@@ -60,7 +64,7 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
             }
             if (fl & 1) {
                 // ---- y: Y[k2] += sum_j2 Ay[k2][j2] X[k1, j2, j3]
-                const double* __restrict__ Ay = tabl + (size_t)((c >> 4) & 15) * NN;
+                const double* __restrict__ Ay = taby + (size_t)((c >> 4) & 15) * NN;
                 __syncthreads();
                 if (act) {
 #pragma unroll
@@ -83,7 +87,7 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
                 for (int i = 0; i < N; ++i) X[i] = 0.0;
                 if (fl & 2) {
                     // ---- z: acc[k3] += sum_j3 Az[k3][j3] Y[k1, k2, j3]
-                    const double* __restrict__ Az = tabl + (size_t)((c >> 8) & 15) * NN;
+                    const double* __restrict__ Az = tabz + (size_t)((c >> 8) & 15) * NN;
                     __syncthreads();
                     if (act) {
 #pragma unroll
@@ -110,7 +114,7 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
         if (act) {
 #pragma unroll
             for (int k3 = 0; k3 < N; ++k3)
-                pot[((size_t)tb * g.npbox + a + N * (bq + N * k3)) * nd + ind] += acc[k3];
+                od.p[((size_t)tb * g.npbox + a + N * (bq + N * k3)) * od.ns + od.coff + ind] += acc[k3];
         }
         __syncthreads();
     }
@@ -118,12 +122,12 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
 
 template <int N>
 bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev, const int* off,
-                 const int* src, const int* code, const int* flags, const double* fvals, double* pot,
+                 const int* src, const int* code, const int* flags, const double* fvals, Kinds kd, OutDesc od,
                  hipStream_t s) {
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
     hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, tgt, lev, off, src, code,
-                       flags, fvals, pot);
+                       flags, fvals, t.loctab_ks, kd, od);
     return true;
 }
 
@@ -131,12 +135,12 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
 
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
-                      const double* fvals, double* pot, hipStream_t s) {
+                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s) {
     if (g.ndim != 3 || ntask <= 0 || !flags) return false;
     switch (t.n) {
-        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
-        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
-        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, pot, s);
+        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
+        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
+        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
         default: return false;
     }
 }

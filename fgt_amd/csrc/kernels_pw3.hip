@@ -261,8 +261,8 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                                                 const double2* __restrict__ tabS, int npw,
                                                 const int* __restrict__ box, const int* __restrict__ lev,
                                                 const int* __restrict__ slot, const int* __restrict__ sel,
-                                                const double2* __restrict__ expn,
-                                                double* __restrict__ pot) {
+                                                const double2* __restrict__ expn, size_t ks, Kinds kd,
+                                                OutDesc od) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
     const size_t sff_len = max((size_t)npw * 64, (size_t)(4 * 4 * 16 * N) / 2);
@@ -278,9 +278,10 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const int vx = (se & 1) ? 1 + ((se >> 1) & 1) : 0;
     const int vy = (se & 1) ? 1 + ((se >> 2) & 1) : 0;
     const int vz = (se & 1) ? 1 + ((se >> 3) & 1) : 0;
-    const int ox = __builtin_amdgcn_readfirstlane((3 * l + vx) * per);
-    const int oy = __builtin_amdgcn_readfirstlane((3 * l + vy) * per);
-    const int oz = __builtin_amdgcn_readfirstlane((3 * l + vz) * per);
+    // table kind per axis (value / first / second derivative): stride ks
+    const int ox = __builtin_amdgcn_readfirstlane((int)(kd.x * ks) + (3 * l + vx) * per);
+    const int oy = __builtin_amdgcn_readfirstlane((int)(kd.y * ks) + (3 * l + vy) * per);
+    const int oz = __builtin_amdgcn_readfirstlane((int)(kd.z * ks) + (3 * l + vz) * per);
     const double2* __restrict__ UX = tabS + ox;
     const double2* __restrict__ UY = tabS + oy;
     const double2* __restrict__ UZ = tabS + oz;
@@ -381,7 +382,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                     for (int s2 = 0; s2 < 4; ++s2) v += red[((w2 * 4 + s2) * N + k) * 16 + k1o];
                     const int kk2 = w2 * NQ + q;
-                    pot[((size_t)b * g.npbox + k1o + N * (kk2 + N * k)) * nd + ind] += 2.0 * v;
+                    od.p[((size_t)b * g.npbox + k1o + N * (kk2 + N * k)) * od.ns + od.coff + ind] += 2.0 * v;
                 }
             }
         }
@@ -401,12 +402,13 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
 
 template <int N>
 bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
-                 const int* sel, const double2* expn, double* pot, hipStream_t s, size_t lds_limit) {
+                 const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
     const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * N);
     const size_t lds = sff + sizeof(double2) * (size_t)N * t.npw;
     if (lds > lds_limit) return false;
+    if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
     hipLaunchKernelGGL(k_pw2pot3<N>, dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw, box, lev,
-                       slot, sel, expn, pot);
+                       slot, sel, expn, t.pw2potS_ks, kd, od);
     return true;
 }
 
@@ -459,13 +461,13 @@ bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
 }
 
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const int* sel, const double2* expn, double* pot, hipStream_t s,
+                      const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s,
                       size_t lds_limit) {
     if (g.ndim != 3 || ntask <= 0 || !sel) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
-        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
-        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, sel, expn, pot, s, lds_limit);
+        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
+        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
+        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
         default: return false;
     }
 }

@@ -248,6 +248,29 @@ void make_pw2pot(int n, int npw, const double* ts, const double* xs, double delt
     }
 }
 
+// Plane wave -> node values and their derivatives (mk_pw2pgh_tables,
+// src/bfgt/bfgt_volrouts.f:339-401): for the upper half k >= npw/2,
+// e = exp(i t_k x_i D/(2 sqrt delta)), der = t_k e dfac, dxx = t_k der dfac
+// with dfac = i (D/(2 sqrt delta)) (2/D); the lower half is the conjugate.
+void make_pw2pgh(int n, int npw, const double* ts, const double* xs, double delta,
+                 double boxdim, cplx* out0, cplx* out1, cplx* out2) {
+    const double dsq = boxdim / 2 / std::sqrt(delta);
+    const cplx dfac = cplx(0.0, 1.0) * dsq * 2.0 / boxdim;
+    for (int i = 0; i < n; ++i) {
+        const double x = xs[i] * dsq;
+        for (int k = npw / 2; k < npw; ++k) {
+            const cplx e = std::exp(cplx(0.0, ts[k] * x));
+            const cplx ex = ts[k] * e * dfac;
+            const cplx exx = ts[k] * ex * dfac;
+            const size_t a = static_cast<size_t>(i) * npw + k, b = static_cast<size_t>(i) * npw + (npw - 1 - k);
+            out0[a] = e;
+            out0[b] = std::conj(e);
+            if (out1) { out1[a] = ex; out1[b] = std::conj(ex); }
+            if (out2) { out2[a] = exx; out2[b] = std::conj(exx); }
+        }
+    }
+}
+
 namespace {
 
 // I_m(t)/lambda integrals of P_m e^{-(t-x)^2/lambda^2} over [-1,1] by the
@@ -365,7 +388,7 @@ void band_pattern(double eps, int n, const double* a, int* ind) {
 //   tab[slot][k][j] masked to the band, band[slot][k] = (lo, hi) 0-based,
 //   cols[slot] = (first, last) 0-based target columns (first > last: empty).
 void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim,
-                       double* tab, int* band, int* cols) {
+                       double* tab, int* band, int* cols, double* tabx, double* tabxx) {
     constexpr int nq = 50;
     std::vector<double> tn(n), tw(n), tu(n * n), xq(nq), wq(nq), pv(nq * n), p(n + 1);
     poly_exps(ipoly, n, tn.data(), tw.data(), tu.data());
@@ -485,15 +508,20 @@ void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim
         // column range and masked table in [k][j] layout
         cols[2 * slot] = u[2 * n] - 1;
         cols[2 * slot + 1] = u[2 * n + 1] - 1;
-        const double* src = F(0, k);
+        double* outs[3] = {tab, tabx, tabxx};
         for (int kk = 0; kk < n; ++kk) {
             int lo = u[2 * kk] - 1, hi = u[2 * kk + 1] - 1;
             const bool colon = kk >= cols[2 * slot] && kk <= cols[2 * slot + 1];
             if (!colon) { lo = 0; hi = -1; }
             band[2 * (slot * n + kk)] = lo;
             band[2 * (slot * n + kk) + 1] = hi;
-            for (int j = 0; j < n; ++j)
-                tab[(static_cast<size_t>(slot) * n + kk) * n + j] = (j >= lo && j <= hi) ? src[j + kk * n] : 0.0;
+            for (int kind = 0; kind < 3; ++kind) {
+                if (!outs[kind]) continue;
+                const double* src = F(kind, k);
+                for (int j = 0; j < n; ++j)
+                    outs[kind][(static_cast<size_t>(slot) * n + kk) * n + j] =
+                        (j >= lo && j <= hi) ? src[j + kk * n] : 0.0;
+            }
         }
     }
 }
@@ -683,13 +711,16 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
 
     // ---- near-field tables per level ----
     const size_t nn = static_cast<size_t>(n) * n;
-    P.loctab.assign(nn * kLocSlots * (nl + 1), 0.0);
+    P.loctab_ks = nn * kLocSlots * (nl + 1);
+    P.loctab.assign(3 * P.loctab_ks, 0.0);
     P.locband.assign(static_cast<size_t>(2) * n * kLocSlots * (nl + 1), 0);
     P.loccols.assign(static_cast<size_t>(2) * kLocSlots * (nl + 1), 0);
     const int nlevend = std::min(npwlevel, nl);
     for (int l = 0; l <= nlevend; ++l)
         make_loctab_level(in.eps, in.ipoly, n, in.delta, T.boxsize[l], &P.loctab[nn * kLocSlots * l],
-                          &P.locband[static_cast<size_t>(2) * n * kLocSlots * l], &P.loccols[2 * kLocSlots * l]);
+                          &P.locband[static_cast<size_t>(2) * n * kLocSlots * l], &P.loccols[2 * kLocSlots * l],
+                          &P.loctab[P.loctab_ks + nn * kLocSlots * l],
+                          &P.loctab[2 * P.loctab_ks + nn * kLocSlots * l]);
 
     // ---- direct interactions: modified list1 (gnd_compute_modified_list1,
     //      tree_routs_nd.f:864-1055) flattened per target, with table slots ----
@@ -779,12 +810,14 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
     // ---- PW tables (boxfgt.f:629-664) ----
     const int nls = P.nlevstart;
     P.p2pw.assign(static_cast<size_t>(n) * npw * (nl + 1), cplx(0));
-    P.pw2pot.assign(static_cast<size_t>(n) * npw * (nl + 1), cplx(0));
+    P.pw2pot_ks = static_cast<size_t>(n) * npw * (nl + 1);
+    P.pw2pot.assign(3 * P.pw2pot_ks, cplx(0));
     for (int l = nls; l <= nl; ++l) {
         make_p2pw(in.ipoly, n, npw, P.ws.data(), P.ts.data(), P.delta0, T.boxsize[l],
                   &P.p2pw[static_cast<size_t>(n) * npw * l]);
-        make_pw2pot(n, npw, P.ts.data(), P.xnodes.data(), P.delta0, T.boxsize[l],
-                    &P.pw2pot[static_cast<size_t>(n) * npw * l]);
+        const size_t o = static_cast<size_t>(n) * npw * l;
+        make_pw2pgh(n, npw, P.ts.data(), P.xnodes.data(), P.delta0, T.boxsize[l], &P.pw2pot[o],
+                    &P.pw2pot[P.pw2pot_ks + o], &P.pw2pot[2 * P.pw2pot_ks + o]);
     }
     // merge/split phases by repeated squaring (g?d_mk_merge_split_pw_matrices,
     // bfgt_pwrouts.f:1344-1543); kept as 1-D factors, the kernels form the

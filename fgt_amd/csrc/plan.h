@@ -65,8 +65,10 @@ struct Plan {
     std::vector<double> xnodes;             // n, on [-1,1]
     // p2pw[l][k][m]: value m -> frequency k (tab_poly2pw(m,k) at level l)
     std::vector<cplx> p2pw;
-    // pw2pot[l][i][k]: frequency k -> node i (tab_pw2pot(k,i) at level l)
+    // pw2pot[kind][l][i][k]: frequency k -> node i at level l; kind 0 =
+    // tab_pw2pot, 1 = tab_pw2der, 2 = tab_pw2dxx (first/second derivative)
     std::vector<cplx> pw2pot;
+    size_t pw2pot_ks = 0;   // kind stride (elements)
     // merge/split 1-D phases wms[klev-1][j] = exp(i ts_j xmin2)^(2^(klev-1))
     std::vector<cplx> wms;
     // M2L 1-D phases wm2l[o][j], o = offset+1 in {-1,0,1}
@@ -74,7 +76,10 @@ struct Plan {
     // near-field tables per target level: loc[l][slot][k][j] (source value j
     // -> target node k), zero outside the band; band[l][slot][k] = (lo, hi)
     // 0-based inclusive row range (lo > hi: empty); colr[l][slot] = column range
+    // loctab[kind][l][slot][k][j], kind as for pw2pot (tab_loc, tabx_loc,
+    // tabxx_loc), all three masked to the same band
     std::vector<double> loctab;
+    size_t loctab_ks = 0;   // kind stride (elements)
     std::vector<int> locband;   // 2 per (l, slot, k)
     std::vector<int> loccols;   // 2 per (l, slot)
 
@@ -116,9 +121,15 @@ void make_p2pw(int ipoly, int n, int npw, const double* ws, const double* ts,
                double delta, double boxdim, cplx* out);
 void make_pw2pot(int n, int npw, const double* ts, const double* xs, double delta,
                  double boxdim, cplx* out);
+// The same table with its first/second derivative versions (out1/out2
+// nullable), each out[i*npw + k].
+void make_pw2pgh(int n, int npw, const double* ts, const double* xs, double delta,
+                 double boxdim, cplx* out0, cplx* out1, cplx* out2);
 // One target level's near-field tables: tab[slot][k][j] (masked to the band),
-// band[slot][k] = (lo, hi) 0-based rows, cols[slot] = (first, last) columns.
+// band[slot][k] = (lo, hi) 0-based rows, cols[slot] = (first, last) columns;
+// tabx/tabxx (nullable): the first/second derivative tables, same mask.
 void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim,
-                       double* tab, int* band, int* cols);
+                       double* tab, int* band, int* cols, double* tabx = nullptr,
+                       double* tabxx = nullptr);
 
 }  // namespace fgt

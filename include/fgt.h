@@ -22,7 +22,10 @@ extern "C" {
  * arguments, gfortran/flang default mangling, so a Fortran caller links it
  * in place of the reference.  pot(nd,npbox,nboxes) is accumulated (+=) at
  * the tensor nodes of every leaf, as the reference does; the caller zeroes it.
- * Supported: ifpgh = 1, ifpghtarg = 0, ifnewtree = 0.  Anything else leaves
+ * ifpgh = 2 / 3 also accumulate grad(nd,ndim,npbox,nboxes) and
+ * hess(nd,ndim*(ndim+1)/2,npbox,nboxes) (Hessian order 1-D xx; 2-D xx,xy,yy;
+ * 3-D xx,yy,zz,xy,xz,yz).
+ * Supported: ifpgh = 1..3, ifpghtarg = 0, ifnewtree = 0.  Anything else leaves
  * the outputs untouched and records a nonzero status (fgt_last_status); the
  * reference has no error channel (it neither checks nor reports).
  * timeinfo(1..20) is zeroed; (1) plan+upload, (2..7) STEP 1..6 device
@@ -85,11 +88,23 @@ int fgt_solver_set_fvals(fgt_solver *s, const double *fvals);
 int fgt_solver_execute(fgt_solver *s, void *stream, int timed);
 /* device potential -> host; accumulate != 0 adds into pot (reference +=). */
 int fgt_solver_get_pot(fgt_solver *s, double *pot, int accumulate);
+/* Output set of later executes, as boxfgt's ifpgh (src/bfgt/boxfgt.f:35-38):
+ * 1 potential, 2 + gradient, 3 + Hessian.  Allocates the device gradient
+ * grad(nd,ndim,npbox,nboxes) / Hessian hess(nd,ndim*(ndim+1)/2,npbox,nboxes)
+ * arrays (Hessian order 1-D xx; 2-D xx,xy,yy; 3-D xx,yy,zz,xy,xz,yz, as
+ * src/bfgt/bfgt_pwrouts.f:612-1065).  Default 1. */
+int fgt_solver_set_ifpgh(fgt_solver *s, int ifpgh);
+/* device gradient / Hessian -> host (accumulate != 0 adds, reference +=);
+ * status 2 when not requested by fgt_solver_set_ifpgh. */
+int fgt_solver_get_grad(fgt_solver *s, double *grad, int accumulate);
+int fgt_solver_get_hess(fgt_solver *s, double *hess, int accumulate);
 int fgt_solver_stats_get(const fgt_solver *s, fgt_solver_stats *out);
 /* Device pointers of the resident fvals / pot buffers (for callers that
  * fill or read HBM directly). */
 void *fgt_solver_fvals_device(fgt_solver *s);
 void *fgt_solver_pot_device(fgt_solver *s);
+void *fgt_solver_grad_device(fgt_solver *s);   /* NULL until ifpgh >= 2 */
+void *fgt_solver_hess_device(fgt_solver *s);   /* NULL until ifpgh >= 3 */
 void fgt_solver_destroy(fgt_solver *s);
 
 #ifdef __cplusplus

@@ -38,6 +38,17 @@ void fgt_host_pw2pot(int ipoly, int n, int npw, const double *ts,
 void fgt_host_loctab(double eps, int ipoly, int n, double delta,
                      double boxdim, double *tab, int *ind);
 
+/* mk_pw2pgh_tables (src/bfgt/bfgt_volrouts.f:339-401): tab_pw2pot,
+ * tab_pw2der, tab_pw2dxx, each (npw,n) complex interleaved. */
+void fgt_host_pw2pgh(int ipoly, int n, int npw, const double *ts,
+                     double delta, double boxdim, double *tab, double *tabx,
+                     double *tabxx);
+/* mk_loctab_all's derivative tables tabx_loc / tabxx_loc
+ * (src/bfgt/bfgt_volrouts.f:407-660), layout of fgt_host_loctab, zeroed
+ * outside the same band as tab. */
+void fgt_host_loctab_der(double eps, int ipoly, int n, double delta,
+                         double boxdim, double *tabx, double *tabxx);
+
 /* Plan summary for a tree (boxfgt.f:132-162 cutoff rules, list sizes),
  * computed on the host only. */
 typedef struct {

@@ -122,6 +122,46 @@ def test_loctab_match_reference(key):
         assert np.all(tab[s][~mask] == 0)
 
 
+@pytest.mark.parametrize("key", _keys("pw2der_"))
+def test_pw2pgh_tables_match_reference(key):
+    ipoly, n, delta, eps, boxdim, iper = key.split("_")
+    ipoly, n, delta, boxdim = int(ipoly), int(n), float(delta), float(boxdim)
+    ts = TABLES[f"ts_{key}"]
+    npw = len(ts)
+    t = [np.zeros(2 * n * npw) for _ in range(3)]
+    fgt_amd.lib().fgt_host_pw2pgh(C.c_int(ipoly), C.c_int(n), C.c_int(npw), _p(ts), C.c_double(delta),
+                                  C.c_double(boxdim), _p(t[0]), _p(t[1]), _p(t[2]))
+    for got, nm in zip(t, ("pw2pot", "pw2der", "pw2dxx")):
+        ref = TABLES[f"{nm}_{key}"]
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * max(1.0, np.abs(ref).max()), err_msg=nm)
+
+
+@pytest.mark.parametrize("key", _keys("loctabx_"))
+def test_loctab_derivatives_match_reference(key):
+    ipoly, n, delta, eps, boxdim, iper = key.split("_")
+    ipoly, n, delta, eps, boxdim = int(ipoly), int(n), float(delta), float(eps), float(boxdim)
+    tx, txx = np.zeros(n * n * 13), np.zeros(n * n * 13)
+    fgt_amd.lib().fgt_host_loctab_der(C.c_double(eps), C.c_int(ipoly), C.c_int(n), C.c_double(delta),
+                                      C.c_double(boxdim), _p(tx), _p(txx))
+    rind = TABLES[f"locind_{key}"].reshape(13, n + 1, 2)
+    for got, nm in ((tx.reshape(13, n, n), "loctabx"), (txx.reshape(13, n, n), "loctabxx")):
+        rtab = TABLES[f"{nm}_{key}"].reshape(13, n, n)
+        for k in range(-6, 7):
+            if abs(k) == 5:
+                continue
+            s = k + 6
+            mask = np.zeros((n, n), bool)
+            lo_c, hi_c = rind[s, n]
+            for col in range(n):
+                lo, hi = rind[s, col]
+                if lo_c <= col + 1 <= hi_c and lo >= 1:
+                    mask[col, lo - 1:hi] = True
+            scale = max(np.abs(rtab[s]).max(), 1e-300)
+            np.testing.assert_allclose(got[s][mask], rtab[s][mask], rtol=0, atol=1e-12 * scale,
+                                       err_msg=f"{nm} k={k}")
+            assert np.all(got[s][~mask] == 0)
+
+
 @pytest.mark.parametrize("name", golden_names())
 def test_plan_summary_matches_oracle(name):
     c = load_case(name)
@@ -158,8 +198,8 @@ def _call_boxfgt(c, ifpgh=1, ifpghtarg=0, ifnewtree=0, ipoly=None, norder=None):
     return rc, pot
 
 
-@pytest.mark.parametrize("kw", [dict(ifpgh=2), dict(ifpghtarg=1), dict(ifnewtree=1), dict(ipoly=3),
-                                dict(norder=40)])
+@pytest.mark.parametrize("kw", [dict(ifpgh=4), dict(ifpgh=0), dict(ifpghtarg=1), dict(ifnewtree=1),
+                                dict(ipoly=3), dict(norder=40)])
 def test_unsupported_options_report_status_and_leave_outputs(kw):
     c = load_case("c1_small")
     rc, pot = _call_boxfgt(c, **kw)
