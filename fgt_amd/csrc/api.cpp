@@ -67,8 +67,7 @@ struct fgt_solver {
     DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
-    DBuf d_m2l_nbr;            // [target][27] source slot per offset, or -1
-    bool m2l27 = false;        // every offset holds at most one source
+    DBuf d_m2l_toff, d_m2l_poff, d_m2l_soff, d_m2l_so;   // grouped STEP 3 lists (padded)
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
@@ -158,7 +157,12 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         S.d_pw2potT.upload(to_d2(tr));
         S.d_wms.upload(to_d2(P.wms));
         S.d_wm2l.upload(to_d2(P.wm2l));
-        S.d_expn.alloc(sizeof(double2) * 2 * (size_t)P.nd * P.nexp * P.nslots);
+        // one extra slot past the last stays zero: padding entries of the
+        // grouped STEP 3 lists read it
+        const size_t slot_elems = 2 * (size_t)P.nd * P.nexp;
+        S.d_expn.alloc(sizeof(double2) * slot_elems * (P.nslots + 1));
+        hcheck(hipMemset(S.d_expn.as<double2>() + slot_elems * P.nslots, 0, sizeof(double2) * slot_elems),
+               "zero slot");
     }
     fgt::DevTables& t = S.tabs;
     t.p2pw = S.d_p2pw.as<double2>();
@@ -353,18 +357,28 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     std::vector<int> mi(P.m2l_idx.size());
     for (size_t i = 0; i < mi.size(); ++i) mi[i] = map_m2l(ndim, P.m2l_idx[i]);
     S.d_m2l_idx.upload(mi);
-    {
-        std::vector<int> nbr(P.m2l_tgt.size() * 27, -1);
-        bool uniq = true;
-        for (size_t tk = 0; tk < P.m2l_tgt.size() && uniq; ++tk)
+    if (P.do_pw) {
+        const long long se = 2LL * P.nd * P.nexp;
+        std::vector<long long> toff(P.m2l_tgt.size()), soff;
+        std::vector<int> poff(P.m2l_tgt.size() + 1, 0), so;
+        for (size_t tk = 0; tk < P.m2l_tgt.size(); ++tk) {
+            toff[tk] = se * P.m2l_tgt[tk];
             for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
-                int& slot = nbr[tk * 27 + mi[j]];
-                if (slot >= 0) { uniq = false; break; }
-                slot = P.m2l_src[j];
+                soff.push_back(se * P.m2l_src[j]);
+                so.push_back(mi[j]);
             }
-        S.m2l27 = uniq;
-        if (uniq) S.d_m2l_nbr.upload(nbr);
+            while (soff.size() % 4) {
+                soff.push_back(se * P.nslots);   // the zero slot
+                so.push_back(13);                // centre offset (unit phase)
+            }
+            poff[tk + 1] = (int)soff.size();
+        }
+        S.d_m2l_toff.upload(toff);
+        S.d_m2l_poff.upload(poff);
+        S.d_m2l_soff.upload(soff);
+        S.d_m2l_so.upload(so);
     }
+
     S.d_dir_tgt.upload(P.dir_tgt);
     S.d_dir_lev.upload(P.dir_lev);
     S.d_dir_off.upload(P.dir_off);
@@ -478,12 +492,8 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
             mark(2);
         }
         // STEP 3
-        if (S.m2l27)
-            fgt::launch_m2l27(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_nbr.as<int>(),
-                              S.d_expn.as<double2>(), st);
-        else
-            fgt::launch_m2l(g, t, (int)P.m2l_tgt.size(), S.d_m2l_tgt.as<int>(), S.d_m2l_off.as<int>(),
-                            S.d_m2l_src.as<int>(), S.d_m2l_idx.as<int>(), S.d_expn.as<double2>(), st);
+        fgt::launch_m2l27(g, t, (int)P.m2l_tgt.size(), S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(),
+                          S.d_m2l_soff.as<long long>(), S.d_m2l_so.as<int>(), S.d_expn.as<double2>(), st);
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)

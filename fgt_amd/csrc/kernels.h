@@ -104,10 +104,13 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                 const int* off, const int* src, const int* idx, double2* expn,
                 hipStream_t s);
-// STEP 3 over a per-target 27-offset neighbour table nbr[task][o] (slot or -1;
-// at most one source per offset)
-void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* nbr, double2* expn,
-                  hipStream_t s);
+// STEP 3 with targets grouped per block.  toff[task]: element offset of the
+// target slot (slot * 2 * nd * nexp); poff: CSR over list entries, each
+// target's list padded to a multiple of 4; soff[j]: element offset of the
+// source slot (padding points at an all-zero slot); so[j]: offset index
+// o = ox + 3 oy + 9 oz in 0..26.
+void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
+                  const long long* soff, const int* so, double2* expn, hipStream_t s);
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);

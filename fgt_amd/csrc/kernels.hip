@@ -219,20 +219,39 @@ __global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
 }
 #undef M2L_PHASE
 
-// STEP 3 with a per-target table of the 3^3 stencil: nbr[task][o] is the
-// source slot at offset o = ox + 3 oy + 9 oz (or -1).  A block owns one
-// element range (one element per thread) and kM2lGroup consecutive targets
-// (Morton neighbours, so their sources repeat and hit in L2).  Each thread
-// forms the nine products w_x w_y and three w_z of its element once and
-// reuses them for every target of the group; per list entry only the source
-// load and two complex products remain.  The offsets are unrolled, so the
-// phase operands are fixed registers and the validity tests are wave-uniform
-// branches; the nine loads of one z plane are issued before their use.
+// STEP 3, grouped: a block owns one element range (one element per thread)
+// and kM2lGroup consecutive targets (Morton neighbours, so their sources
+// repeat and hit in L2).  Each thread forms the nine products w_x w_y and
+// three w_z of its element once and reuses them for every list entry of the
+// group.  The entry's offset o = ox + 3 oy + 9 oz is wave-uniform; a switch
+// on it picks the phase factors from fixed registers (scalar branches, no
+// per-lane selects), and four source loads are issued ahead of their use.
 constexpr int kM2lGroup = 8;
+constexpr int kM2lBatch = 4;   // source loads issued together (list padding unit)
+
+#define M2L_CASE(o)                                                     \
+    case o:                                                             \
+        acc = cfma(v, cmul(fxy[(o) % 9], fz[(o) / 9]), acc);             \
+        break;
+
+__device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, const double2 (&fxy)[9],
+                                             const double2 (&fz)[3]) {
+    switch (o) {
+        M2L_CASE(0) M2L_CASE(1) M2L_CASE(2) M2L_CASE(3) M2L_CASE(4) M2L_CASE(5) M2L_CASE(6) M2L_CASE(7)
+        M2L_CASE(8) M2L_CASE(9) M2L_CASE(10) M2L_CASE(11) M2L_CASE(12) M2L_CASE(13) M2L_CASE(14)
+        M2L_CASE(15) M2L_CASE(16) M2L_CASE(17) M2L_CASE(18) M2L_CASE(19) M2L_CASE(20) M2L_CASE(21)
+        M2L_CASE(22) M2L_CASE(23) M2L_CASE(24) M2L_CASE(25) M2L_CASE(26)
+        default: break;
+    }
+    return acc;
+}
+#undef M2L_CASE
 
 __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup,
-                                                  const int* __restrict__ tgt,
-                                                  const int* __restrict__ nbr,
+                                                  const long long* __restrict__ toff,
+                                                  const int* __restrict__ poff,
+                                                  const long long* __restrict__ soff,
+                                                  const int* __restrict__ so,
                                                   double2* __restrict__ expn) {
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int grp = lin % ngroup, chunk = lin / ngroup;
@@ -255,26 +274,20 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask
     }
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
-    for (int q = 0; q < kM2lGroup; ++q) {
-        const int task = grp * kM2lGroup + q;
-        if (task >= ntask) break;
-        const int ts = tgt[task];
-        const int* __restrict__ nb = nbr + (size_t)task * 27;
-        double2 acc = expn[(size_t)ts * 2 * tot + e];
+    const double2* __restrict__ eb = expn + e;
+    const int tb = grp * kM2lGroup, te = min(tb + kM2lGroup, ntask);
+    for (int task = tb; task < te; ++task) {
+        const long long to = toff[task];
+        double2 acc = eb[to];
+        // lists are padded to a multiple of kM2lBatch with a zero expansion
+        for (int j = poff[task]; j < poff[task + 1]; j += kM2lBatch) {
+            double2 v[kM2lBatch];
 #pragma unroll
-        for (int oz = 0; oz < 3; ++oz) {
-            int sv[9];
-            double2 v[9];
+            for (int u = 0; u < kM2lBatch; ++u) v[u] = eb[soff[j + u]];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) sv[i] = nb[oz * 9 + i];
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-                if (sv[i] >= 0) v[i] = expn[(size_t)sv[i] * 2 * tot + e];
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-                if (sv[i] >= 0) acc = cfma(v[i], cmul(fxy[i], fz[oz]), acc);
+            for (int u = 0; u < kM2lBatch; ++u) acc = m2l_apply(so[j + u], v[u], acc, fxy, fz);
         }
-        if (ok) expn[((size_t)ts * 2 + 1) * tot + e] = acc;
+        if (ok) expn[to + tot + e] = acc;
     }
 }
 
@@ -473,13 +486,14 @@ void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt, co
     check(hipGetLastError(), "k_m2l launch");
 }
 
-void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* nbr, double2* expn,
-                  hipStream_t s) {
+void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
+                  const long long* soff, const int* so, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
     const int ngroup = (ntask + kM2lGroup - 1) / kM2lGroup;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
-    hipLaunchKernelGGL(k_m2l27, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, tgt, nbr, expn);
+    hipLaunchKernelGGL(k_m2l27, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff, poff, soff,
+                       so, expn);
     check(hipGetLastError(), "k_m2l27 launch");
 }
 
