@@ -52,7 +52,7 @@ def stage_model(st, n, nd=1):
     STEP 5 (k_pw2pot3, per PW leaf, 3-D): 4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
     flops (the x/y stages complex x complex, the last stage real output);
     bytes: read one local expansion (16 nexp) + read/write pot (16 n^3).
-    STEP 3 (k_m2l, per cutoff-level box): 8 nexp flops per list entry; compulsory
+    STEP 3 (k_m2l27, per cutoff-level box): 8 nexp flops per list entry; compulsory
     bytes: every cutoff-level mp read once and every loc written once,
     2 x 16 nexp per box (re-reads of a shared source are not algorithmic).
     """
@@ -63,7 +63,7 @@ def stage_model(st, n, nd=1):
     return {
         "pw2pot": {"kernel": "k_pw2pot3", "bound": "mfma", "flops": L * nd * f15,
                    "bytes": L * nd * (16 * nexp + 16 * n ** 3)},
-        "m2l": {"kernel": "k_m2l", "bound": "hbm", "flops": 8 * nd * nexp * st["n_m2l_pairs"],
+        "m2l": {"kernel": "k_m2l27", "bound": "hbm", "flops": 8 * nd * nexp * st["n_m2l_pairs"],
                 "bytes": 2 * 16 * nd * nexp * st["n_m2l_targets"]},
     }
 
