@@ -103,8 +103,11 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
     ``grad[nboxes, npbox, ndim(, nd)]`` for ifpgh >= 2 and
     ``hess[nboxes, npbox, ndim*(ndim+1)/2(, nd)]`` for ifpgh = 3, also
     accumulated (Hessian order 1-D xx; 2-D xx, xy, yy; 3-D xx, yy, zz, xy,
-    xz, yz).  Raises :class:`FgtError` for unsupported options (ifpghtarg > 0,
-    ifnewtree = 1) instead of silently returning.
+    xz, yz).  With ifpghtarg >= 1, ``targs[ntarg, ndim]`` and
+    ``pote[ntarg(, nd)]``, ``grade[ntarg, ndim(, nd)]``,
+    ``hesse[ntarg, nhess(, nd)]`` receive the values at the targets
+    (overwritten).  Raises :class:`FgtError` for unsupported options
+    (ifnewtree = 1) instead of silently returning.
     """
     itree = np.ascontiguousarray(itree, np.int32)
     iptr = np.ascontiguousarray(iptr, np.int32)
@@ -121,6 +124,15 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
             if not (isinstance(arr, np.ndarray) and arr.dtype == np.float64 and arr.flags.c_contiguous
                     and arr.size == nd * ncomp * npbox * nboxes):
                 raise TypeError(f"{nm} must be a C-contiguous float64 array of nd*{ncomp}*npbox*nboxes values")
+    if ifpghtarg >= 1 and ntarg > 0:
+        targs = np.ascontiguousarray(targs, np.float64)
+        if targs.size != ndim * ntarg:
+            raise ValueError("targs must hold ndim*ntarg values")
+        nhess = ndim * (ndim + 1) // 2
+        for arr, need, ncomp, nm in ((pote, 1, 1, "pote"), (grade, 2, ndim, "grade"), (hesse, 3, nhess, "hesse")):
+            if ifpghtarg >= need and not (isinstance(arr, np.ndarray) and arr.dtype == np.float64
+                                          and arr.flags.c_contiguous and arr.size == nd * ncomp * ntarg):
+                raise TypeError(f"{nm} must be a C-contiguous float64 array of nd*{ncomp}*ntarg values")
     dummy = np.zeros(1, np.float64)
     ti = np.zeros(20, np.float64) if timeinfo is None else timeinfo
     rc = lib().fgt_boxfgt(_i(nd), _i(ndim), _d(delta), _d(eps), _i(ipoly), _i(iperiod),
@@ -186,6 +198,20 @@ class Solver:
             out = np.zeros(self._shape(self.ndim * (self.ndim + 1) // 2))
         _check(lib().fgt_solver_get_hess(self._h, _p(out), C.c_int(1 if accumulate else 0)))
         return out
+
+    def eval_targets(self, targs, ifpghtarg=1):
+        """Values of the last execute at targets [nt, ndim]: (pote, grade, hesse)."""
+        t = np.ascontiguousarray(targs, np.float64).reshape(-1, self.ndim)
+        nt = t.shape[0]
+        tail = () if self.nd == 1 else (self.nd,)
+        nh = self.ndim * (self.ndim + 1) // 2
+        pote = np.zeros((nt,) + tail)
+        grade = np.zeros((nt, self.ndim) + tail) if ifpghtarg >= 2 else None
+        hesse = np.zeros((nt, nh) + tail) if ifpghtarg >= 3 else None
+        d = np.zeros(1)
+        _check(lib().fgt_solver_eval_targets(self._h, C.c_int(nt), _p(t), C.c_int(ifpghtarg), _p(pote),
+                                             _p(d if grade is None else grade), _p(d if hesse is None else hesse)))
+        return pote, grade, hesse
 
     def get_pot(self, out=None, accumulate=False):
         if out is None:

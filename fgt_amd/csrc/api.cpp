@@ -63,6 +63,10 @@ struct fgt_solver {
     DBuf d_fvals, d_pot, d_expn;
     int ifpgh = 1;          // 1 pot, 2 + grad, 3 + hess
     DBuf d_grad, d_hess;    // grad(nd,ndim,npbox,nboxes), hess(nd,nhess,npbox,nboxes)
+    // extra targets (STEP 11): set up on first use
+    bool t_ready = false;
+    DBuf d_t_u, d_t_leaves, d_t_centers, d_t_bsz, d_t_coefs;
+    int t_nleaf = 0;
     // lists
     DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
@@ -540,6 +544,86 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
     }
 }
 
+// STEP 11 (extra targets, src/bfgt/boxfgt.f:1343-1378): values of the last
+// execute (device pot / grad / hess) -> coefficients on every leaf -> values
+// at the targets.  Targets are assigned to leaves on the host by the
+// reference's root-to-leaf descent (child bit a set when t_a - c_a >= 0,
+// find_childbox_ind, src/common/pts_tree_nd.f:713-740) and grouped by leaf.
+void eval_targets(fgt_solver& S, int nt, const double* targs, int ifpghtarg, double* pote, double* grade,
+                  double* hesse, hipStream_t st) {
+    const fgt::Plan& P = S.plan;
+    const fgt::TreeView& T = P.tree;
+    const int ndim = P.ndim, n = P.n, nd = P.nd, mc = 1 << ndim;
+    if (ifpghtarg > S.ifpgh) throw std::invalid_argument("targets need ifpgh >= ifpghtarg on the solver");
+    if (nt <= 0) return;
+    if (!S.t_ready) {
+        std::vector<double> x(n), w(n), u((size_t)n * n);
+        fgt::poly_exps(P.ipoly, n, x.data(), w.data(), u.data());
+        S.d_t_u.upload(u);
+        std::vector<int> leaves;
+        std::vector<double> bsz(P.nboxes);
+        for (int b = 0; b < P.nboxes; ++b) {
+            if (T.nchild[b] == 0) leaves.push_back(b);
+            bsz[b] = T.boxsize[T.level[b]];
+        }
+        S.t_nleaf = (int)leaves.size();
+        S.d_t_leaves.upload(leaves);
+        S.d_t_centers.upload(T.centers);
+        S.d_t_bsz.upload(bsz);
+        S.t_ready = true;
+    }
+    std::vector<int> leaf(nt), perm(nt);
+    for (int it = 0; it < nt; ++it) {
+        int b = 0;
+        while (T.nchild[b] > 0) {
+            int k = 0;
+            for (int a = 0; a < ndim; ++a)
+                if (targs[(size_t)ndim * it + a] - T.centers[(size_t)ndim * b + a] >= 0) k |= 1 << a;
+            const int c = T.child[(size_t)mc * b + k];
+            if (c < 0) throw std::invalid_argument("tree has a box with a missing child");
+            b = c;
+        }
+        leaf[it] = b;
+        perm[it] = it;
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return leaf[a] < leaf[b]; });
+    std::vector<double> ts((size_t)nt * ndim);
+    std::vector<int> tbox, tfirst, tcount;
+    for (int i = 0; i < nt; ++i) {
+        for (int a = 0; a < ndim; ++a) ts[(size_t)ndim * i + a] = targs[(size_t)ndim * perm[i] + a];
+        if (i == 0 || leaf[perm[i]] != leaf[perm[i - 1]] || tcount.back() == 256) {
+            tbox.push_back(leaf[perm[i]]);
+            tfirst.push_back(i);
+            tcount.push_back(0);
+        }
+        ++tcount.back();
+    }
+    DBuf d_ts, d_perm, d_tbox, d_tfirst, d_tcount, d_out;
+    d_ts.upload(ts);
+    d_perm.upload(perm);
+    d_tbox.upload(tbox);
+    d_tfirst.upload(tfirst);
+    d_tcount.upload(tcount);
+    const int nhess = ndim * (ndim + 1) / 2;
+    struct Out { const DBuf* vals; int ncomp; double* host; };
+    const Out outs[3] = {{&S.d_pot, 1, pote}, {&S.d_grad, ndim, grade}, {&S.d_hess, nhess, hesse}};
+    for (int L = 0; L < ifpghtarg; ++L) {
+        const int ncd = nd * outs[L].ncomp;
+        const size_t need = sizeof(double) * (size_t)ncd * P.npbox * P.nboxes;
+        if (S.d_t_coefs.bytes < need) S.d_t_coefs.alloc(need);
+        d_out.alloc(sizeof(double) * (size_t)ncd * nt);
+        fgt::launch_vals2coefs(ndim, n, ncd, S.d_t_u.as<double>(), S.t_nleaf, S.d_t_leaves.as<int>(),
+                               outs[L].vals->as<double>(), S.d_t_coefs.as<double>(), st);
+        hcheck(hipGetLastError(), "k_vals2coefs launch");
+        fgt::launch_evalt(ndim, n, P.ipoly, ncd, (int)tbox.size(), d_tbox.as<int>(), d_tfirst.as<int>(),
+                          d_tcount.as<int>(), S.d_t_centers.as<double>(), S.d_t_bsz.as<double>(), d_ts.as<double>(),
+                          d_perm.as<int>(), S.d_t_coefs.as<double>(), d_out.as<double>(), st);
+        hcheck(hipGetLastError(), "k_evalt launch");
+        hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+        hcheck(hipMemcpy(outs[L].host, d_out.p, d_out.bytes, hipMemcpyDeviceToHost), "download target values");
+    }
+}
+
 template <class F>
 int guarded(F&& f) {
     try {
@@ -683,6 +767,18 @@ int fgt_solver_get_hess(fgt_solver* s, double* hess, int accumulate) {
     });
 }
 
+int fgt_solver_eval_targets(fgt_solver* s, int ntarg, const double* targs, int ifpghtarg, double* pote,
+                            double* grade, double* hesse) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null solver");
+        if (ifpghtarg < 1 || ifpghtarg > 3) throw std::invalid_argument("ifpghtarg must be 1, 2 or 3");
+        if (ntarg < 0) throw std::invalid_argument("ntarg < 0");
+        if (ntarg > 0 && (!targs || !pote || (ifpghtarg >= 2 && !grade) || (ifpghtarg >= 3 && !hesse)))
+            throw std::invalid_argument("null target array");
+        eval_targets(*s, ntarg, targs, ifpghtarg, pote, grade, hesse, nullptr);
+    });
+}
+
 void* fgt_solver_grad_device(fgt_solver* s) { return s ? s->d_grad.p : nullptr; }
 void* fgt_solver_hess_device(fgt_solver* s) { return s ? s->d_hess.p : nullptr; }
 
@@ -707,15 +803,22 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         if (*ifpgh < 1 || *ifpgh > 3) throw std::invalid_argument("ifpgh must be 1, 2 or 3");
         if (*ifpgh >= 2 && !grad) throw std::invalid_argument("null grad");
         if (*ifpgh >= 3 && !hess) throw std::invalid_argument("null hess");
-        if (*ifpghtarg != 0) throw std::invalid_argument("extra targets (ifpghtarg >= 1) are not supported");
+        if (*ifpghtarg < 0 || *ifpghtarg > 3) throw std::invalid_argument("ifpghtarg must be 0..3");
+        const int nt = (*ifpghtarg >= 1 && ntarg) ? *ntarg : 0;
+        if (*ifpghtarg >= 1 && (!ntarg || *ntarg < 0)) throw std::invalid_argument("bad ntarg");
+        if (nt > 0 && (!targs || !pote || (*ifpghtarg >= 2 && !grade) || (*ifpghtarg >= 3 && !hesse)))
+            throw std::invalid_argument("null target array");
         if (*ifnewtree != 0) throw std::invalid_argument("ifnewtree = 1 is not supported");
         const double t0 = now_s();
         fgt::PlanInput in{*nd, *ndim, *delta, *eps, *ipoly, *iperiod, *norder, *npbox, *nboxes, *nlevels,
                           *ltree, itree, iptr, centers, boxsize};
         fgt_solver S;
         build_solver(S, in);
-        if (*ifpgh > 1) {
-            const int rc = fgt_solver_set_ifpgh(&S, *ifpgh);
+        // gradients / Hessians are formed when the caller or the targets need
+        // them (boxfgt.f:202-272 routes ifpgh < ifpghtarg the same way)
+        const int ifpgh_dev = std::max(*ifpgh, nt > 0 ? *ifpghtarg : 0);
+        if (ifpgh_dev > 1) {
+            const int rc = fgt_solver_set_ifpgh(&S, ifpgh_dev);
             if (rc) throw std::runtime_error(g_msg);
         }
         hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
@@ -728,6 +831,18 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         if (*ifpgh >= 2) download(S.d_grad, grad, 1, "download grad");
         if (*ifpgh >= 3) download(S.d_hess, hess, 1, "download hess");
         const double t3 = now_s();
+        if (nt > 0) {
+            // STEP 11 evaluates what the reference holds at that point: the
+            // caller's accumulated arrays for the outputs it requested, fresh
+            // values for the ones formed only for the targets
+            hcheck(hipMemcpy(S.d_pot.p, pot, S.d_pot.bytes, hipMemcpyHostToDevice), "upload pot");
+            if (*ifpgh >= 2 && *ifpghtarg >= 2)
+                hcheck(hipMemcpy(S.d_grad.p, grad, S.d_grad.bytes, hipMemcpyHostToDevice), "upload grad");
+            if (*ifpgh >= 3 && *ifpghtarg >= 3)
+                hcheck(hipMemcpy(S.d_hess.p, hess, S.d_hess.bytes, hipMemcpyHostToDevice), "upload hess");
+            eval_targets(S, nt, targs, *ifpghtarg, pote, grade, hesse, nullptr);
+            if (timeinfo) timeinfo[11] = now_s() - t3;
+        }
         if (timeinfo) {
             timeinfo[0] = t1 - t0;
             for (int i = 0; i < 6; ++i) timeinfo[1 + i] = S.stage_ms[i] * 1e-3;

@@ -126,4 +126,14 @@ bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* t
                       const int* off, const int* src, const int* code, const int* flags,
                       const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
 
+// Extra targets (STEP 11): values (ncd, npbox, nboxes) on the listed leaves
+// -> coefficients coefs[b][q][i] (u = values->coefficients, u(k,j) at
+// u[k + j*n]); then per task (leaf tbox, sorted targets tfirst..+tcount,
+// tcount <= 256) the expansion at each target, out[perm[t]*ncd + q].
+void launch_vals2coefs(int ndim, int n, int ncd, const double* u, int nleaf, const int* leaves, const double* vals,
+                       double* coefs, hipStream_t s);
+void launch_evalt(int ndim, int n, int ipoly, int ncd, int ntask, const int* tbox, const int* tfirst,
+                  const int* tcount, const double* centers, const double* bsize_of_box, const double* targs_sorted,
+                  const int* perm, const double* coefs, double* out, hipStream_t s);
+
 }  // namespace fgt

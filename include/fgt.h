@@ -25,11 +25,15 @@ extern "C" {
  * ifpgh = 2 / 3 also accumulate grad(nd,ndim,npbox,nboxes) and
  * hess(nd,ndim*(ndim+1)/2,npbox,nboxes) (Hessian order 1-D xx; 2-D xx,xy,yy;
  * 3-D xx,yy,zz,xy,xz,yz).
- * Supported: ifpgh = 1..3, ifpghtarg = 0, ifnewtree = 0.  Anything else leaves
+ * ifpghtarg = 1..3 with ntarg >= 1 evaluates pot (+ grad, + hess) at the
+ * targets targs(ndim,ntarg) into pote(nd,ntarg), grade(nd,ndim,ntarg),
+ * hesse(nd,nhess,ntarg) (overwritten), from the polynomial expansions of
+ * the leaf values, as the reference's STEP 11 (boxfgt.f:1343-1378).
+ * Supported: ifpgh = 1..3, ifpghtarg = 0..3, ifnewtree = 0.  Anything else leaves
  * the outputs untouched and records a nonzero status (fgt_last_status); the
  * reference has no error channel (it neither checks nor reports).
  * timeinfo(1..20) is zeroed; (1) plan+upload, (2..7) STEP 1..6 device
- * seconds, (8) download.
+ * seconds, (8) download, (12) extra targets.
  */
 void boxfgt_(int *nd, int *ndim, double *delta, double *eps, int *ipoly,
              int *iperiod, int *norder, int *npbox, int *nboxes, int *nlevels,
@@ -98,6 +102,14 @@ int fgt_solver_set_ifpgh(fgt_solver *s, int ifpgh);
  * status 2 when not requested by fgt_solver_set_ifpgh. */
 int fgt_solver_get_grad(fgt_solver *s, double *grad, int accumulate);
 int fgt_solver_get_hess(fgt_solver *s, double *hess, int accumulate);
+/* STEP 11 on the last execute's device pot / grad / hess: values at ntarg
+ * targets targs(ndim,ntarg) (treedata_trans_nd + treedata_evalt_nd,
+ * src/common/tree_data_routs_nd.f:16-95,582-686) into pote(nd,ntarg),
+ * grade(nd,ndim,ntarg) (ifpghtarg >= 2), hesse(nd,nhess,ntarg) (= 3),
+ * overwritten.  Needs ifpghtarg <= the solver's ifpgh. */
+int fgt_solver_eval_targets(fgt_solver *s, int ntarg, const double *targs,
+                            int ifpghtarg, double *pote, double *grade,
+                            double *hesse);
 int fgt_solver_stats_get(const fgt_solver *s, fgt_solver_stats *out);
 /* Device pointers of the resident fvals / pot buffers (for callers that
  * fill or read HBM directly). */

@@ -1423,3 +1423,82 @@ done:
     free(nlistpw); free(listpw); free(rml); free(iaddr);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* STEP 11: extra targets                                              */
+/* ------------------------------------------------------------------ */
+
+/* treedata_trans_nd with itype = 0 (tree_data_routs_nd.f:16-95, via
+ * ortho_trans_nd, tensor_prod_routs_nd.f:21-300): values -> orthogonal
+ * polynomial coefficients on one leaf, x then y then z:
+ * c(k,j,i) = sum u(k,k1) u(j,j1) u(i,i1) f(k1,j1,i1); then
+ * treedata_evalt_nd (tree_data_routs_nd.f:582-686): each target descends
+ * from the root to its leaf (find_childbox_ind, pts_tree_nd.f:713-740: child
+ * bit a set when t_a - c_a >= 0; pts_tree_sort, pts_tree_nd.f:1226-1256) and
+ * the expansion is evaluated at (t - c) 2/boxsize (ortho_evalt_nd,
+ * tensor_prod_routs_nd.f:1726-1907: x, then y, then z).
+ * vals(nd,npbox,nboxes) -> out(nd,nt), overwritten. */
+int oracle_evalt(int ndim, int nd, int ipoly, int norder, int nboxes, int nlevels,
+                 const int *itree, const int *iptr, const double *centers,
+                 const double *boxsize, const double *vals, int nt,
+                 const double *targ, double *out)
+{
+    if (ndim < 1 || ndim > 3 || nd < 1 || norder < 1 || norder > 32 || nt < 0) return 1;
+    const int n = norder, mc = 1 << ndim, npbox = ipow(n, ndim);
+    const int *ilevel = itree + (iptr[1] - 1), *nchild = itree + (iptr[3] - 1);
+    const int *ichild = itree + (iptr[4] - 1);
+    double xq[64], wq[64], u[64 * 64];
+    oracle_polyexps(ipoly, n, xq, wq, u);
+    (void)nboxes; (void)nlevels;
+#pragma omp parallel
+    {
+        double *c = (double *)malloc(sizeof(double) * 2 * npbox);
+        double *w = c + npbox;
+#pragma omp for schedule(dynamic, 64)
+        for (int it = 0; it < nt; ++it) {
+            const double *t = targ + (size_t)ndim * it;
+            int b = 1;
+            while (nchild[b - 1] > 0) {
+                int k = 0;
+                for (int a = 0; a < ndim; ++a)
+                    if (t[a] - centers[(size_t)ndim * (b - 1) + a] >= 0) k |= 1 << a;
+                b = ichild[(size_t)mc * (b - 1) + k];
+            }
+            const double sc = 2.0 / boxsize[ilevel[b - 1]];
+            double p[3][64];
+            for (int a = 0; a < ndim; ++a) {
+                double x = (t[a] - centers[(size_t)ndim * (b - 1) + a]) * sc;
+                if (ipoly == 0) legepols(x, n - 1, p[a]); else chebpols(x, n - 1, p[a]);
+            }
+            for (int ind = 0; ind < nd; ++ind) {
+                /* coefficients of this leaf, density ind */
+                for (int i = 0; i < npbox; ++i) c[i] = vals[((size_t)(b - 1) * npbox + i) * nd + ind];
+                int stride = 1;
+                for (int a = 0; a < ndim; ++a) {
+                    for (int i = 0; i < npbox; ++i) {
+                        int ka = (i / stride) % n, base = i - ka * stride;
+                        double dd = 0;
+                        for (int k1 = 0; k1 < n; ++k1) dd += u[ka + k1 * n] * c[base + k1 * stride];
+                        w[i] = dd;
+                    }
+                    memcpy(c, w, sizeof(double) * npbox);
+                    stride *= n;
+                }
+                /* evaluate: contract x, then y, then z */
+                int len = npbox;
+                for (int a = 0; a < ndim; ++a) {
+                    len /= n;
+                    for (int r = 0; r < len; ++r) {
+                        double pp = 0;
+                        for (int k = 0; k < n; ++k) pp += c[r * n + k] * p[a][k];
+                        w[r] = pp;
+                    }
+                    memcpy(c, w, sizeof(double) * len);
+                }
+                out[(size_t)it * nd + ind] = c[0];
+            }
+        }
+        free(c);
+    }
+    return 0;
+}

@@ -66,6 +66,17 @@ int oracle_boxfgt_pgh(int nd, int ndim, double delta, double eps, int ipoly,
                       double *pot, double *grad, double *hess,
                       int d1_mode, double *timeinfo, oracle_info *info);
 
+/*
+ * STEP 11 for ifpghtarg >= 1 (src/bfgt/boxfgt.f:1343-1378): values on the
+ * leaves (nd,npbox,nboxes) -> polynomial coefficients -> values at nt
+ * targets targ(ndim,nt), out(nd,nt) overwritten.  For grad/hess call with
+ * nd*ndim / nd*nhess densities, as the reference does.
+ */
+int oracle_evalt(int ndim, int nd, int ipoly, int norder, int nboxes, int nlevels,
+                 const int *itree, const int *iptr, const double *centers,
+                 const double *boxsize, const double *vals, int nt,
+                 const double *targ, double *out);
+
 /* ---- exposed pieces, so tests can pin them one by one ---- */
 
 /* fgtpwterms (src/common/fgtterms.f:72-141). */

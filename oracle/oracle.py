@@ -87,6 +87,21 @@ def boxfgt_pgh(nd, ndim, delta, eps, ipoly, iperiod, norder, nboxes, nlevels,
     return pot, (grad if ifpgh >= 2 else None), (hess if ifpgh >= 3 else None), info
 
 
+def evalt(ndim, nd, ipoly, norder, nboxes, nlevels, itree, iptr, centers, boxsize, vals, targs):
+    """Values (nboxes, npbox[, nd]) on the leaves -> values at targs (nt, ndim): (nt[, nd])."""
+    targs = np.ascontiguousarray(targs, np.float64)
+    nt = targs.shape[0]
+    out = np.zeros((nt,) + ((nd,) if nd > 1 else ()))
+    rc = lib().oracle_evalt(C.c_int(ndim), C.c_int(nd), C.c_int(ipoly), C.c_int(norder), C.c_int(nboxes),
+                            C.c_int(nlevels), _p(np.ascontiguousarray(itree, np.int32)),
+                            _p(np.ascontiguousarray(iptr, np.int32)), _p(np.ascontiguousarray(centers)),
+                            _p(np.ascontiguousarray(boxsize)), _p(np.ascontiguousarray(vals, np.float64)),
+                            C.c_int(nt), _p(targs), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_evalt failed with code {rc}")
+    return out
+
+
 def pw2pgh(ipoly, n, npw, ts, delta, boxdim):
     """(tab_pw2pot, tab_pw2der, tab_pw2dxx), each [j, i] like the reference (npw, n)."""
     t = [np.zeros(2 * n * npw) for _ in range(3)]

@@ -87,13 +87,16 @@ def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng) -> Tree:
                 iptr, centers, boxsize, fvals)
 
 
-def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh: int = 1):
+def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh: int = 1, targs=None,
+           ifpghtarg: int = 0):
     """Run the reference boxfgt (ifnewtree=0, no extra targets).
 
     Returns (pot, timeinfo[20]) for ifpgh=1, (pot, grad, hess, timeinfo) for
     ifpgh=2/3 (hess None for ifpgh=2).  pot is shaped like fvals; grad is
     Fortran grad(nd,ndim,npbox,nboxes) as [nboxes, npbox, ndim(, nd)], hess
-    likewise with ndim*(ndim+1)/2 components.
+    likewise with ndim*(ndim+1)/2 components.  With targs (nt, ndim) and
+    ifpghtarg >= 1 the result is (pot, grad, hess, pote, grade, hesse, timeinfo)
+    with pote [nt(, nd)], grade [nt, ndim(, nd)], hesse [nt, nhess(, nd)].
     """
     L = lib()
     npbox = tree.norder ** tree.ndim
@@ -104,8 +107,12 @@ def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh:
     tail = () if nd == 1 else (nd,)
     grad = np.zeros((tree.nboxes, npbox, ndim) + tail if ifpgh >= 2 else 1, np.float64)
     hess = np.zeros((tree.nboxes, npbox, nhess) + tail if ifpgh >= 3 else 1, np.float64)
-    targs = np.zeros(tree.ndim, np.float64)
-    pote = np.zeros(nd, np.float64)
+    nt = 0 if targs is None else len(targs)
+    tail2 = () if nd == 1 else (nd,)
+    targs = np.zeros((1, tree.ndim)) if targs is None else np.ascontiguousarray(targs, np.float64)
+    pote = np.zeros((max(nt, 1),) + tail2, np.float64)
+    grade = np.zeros((max(nt, 1), ndim) + tail2, np.float64)
+    hesse = np.zeros((max(nt, 1), nhess) + tail2, np.float64)
     dum = np.zeros(1, np.float64)
     timeinfo = np.zeros(100, np.float64)
     itree = tree.itree.copy()   # the reference may write colleagues back (periodic, npwlevel<=1)
@@ -117,8 +124,11 @@ def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh:
               C.byref(C.c_int(tree.nlevels)), C.byref(C.c_int(tree.ltree)),
               _p(itree), _p(tree.iptr), _p(tree.centers), _p(tree.boxsize),
               _p(f), C.byref(C.c_int(ifpgh)), _p(pot), _p(grad), _p(hess),
-              C.byref(C.c_int(0)), C.byref(C.c_int(0)), _p(targs),
-              C.byref(C.c_int(0)), _p(pote), _p(dum), _p(dum), _p(timeinfo))
+              C.byref(C.c_int(0)), C.byref(C.c_int(nt)), _p(targs),
+              C.byref(C.c_int(ifpghtarg if nt else 0)), _p(pote), _p(grade), _p(hesse), _p(timeinfo))
+    if nt and ifpghtarg >= 1:
+        return (pot, grad if ifpgh >= 2 else None, hess if ifpgh >= 3 else None, pote,
+                grade if ifpghtarg >= 2 else None, hesse if ifpghtarg >= 3 else None, timeinfo[:20].copy())
     if ifpgh == 1:
         return pot, timeinfo[:20].copy()
     return pot, grad, (hess if ifpgh >= 3 else None), timeinfo[:20].copy()

@@ -47,12 +47,18 @@ class Case:
         self.ifpgh = int(z["ifpgh"]) if "ifpgh" in z else 1
         tail = () if self.nd == 1 else (self.nd,)
         self.grad_ref = self.hess_ref = None
-        if self.ifpgh >= 2:
+        if self.ifpgh >= 2 and "grad_leaf" in z:
             self.grad_ref = np.zeros((self.nboxes, self.npbox, self.ndim) + tail)
             self.grad_ref[self.leaves] = z["grad_leaf"]
-        if self.ifpgh >= 3:
+        if self.ifpgh >= 3 and "hess_leaf" in z:
             self.hess_ref = np.zeros((self.nboxes, self.npbox, self.ndim * (self.ndim + 1) // 2) + tail)
             self.hess_ref[self.leaves] = z["hess_leaf"]
+        # extra targets (ifpghtarg >= 1): targets and the reference's values there
+        self.ifpghtarg = int(z["ifpghtarg"]) if "ifpghtarg" in z else 0
+        self.targs = z["targs"] if "targs" in z else None
+        self.pote_ref = z["pote"] if "pote" in z else None
+        self.grade_ref = z["grade"] if "grade" in z else None
+        self.hesse_ref = z["hesse"] if "hesse" in z else None
 
     def tree_args(self):
         return (self.nboxes, self.nlevels, self.ltree, self.itree, self.iptr, self.centers, self.boxsize)

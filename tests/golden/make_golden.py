@@ -10,6 +10,8 @@ Fixtures are data only: tree arrays, inputs and the reference's outputs.
     (src/common/tree_vol_coeffs_nd.f:83,444), its fvals, and the reference
     boxfgt potential (src/bfgt/boxfgt.f:5) for the stated (eps, delta).
   * boxfgt_pgh*/pg*.npz : the same with ifpgh = 2/3 (gradients, Hessians).
+  * boxfgt_tgt*.npz : with extra targets (ifpghtarg = 1..3): targets and the
+    reference's pote / grade / hesse.
   * tables.npz : the reference's 1-D tables and scalars (fgtterms.f,
     legeexps.f, chebexps.f, besseljs3d.f, bfgt_volrouts.f) for a grid of
     parameters.
@@ -57,6 +59,28 @@ PGH_CASES = {
     "pgh2_n12":      (2, 12, 0, 0, 1e-6, 1e-9, 1e-4, 14, 3, (1e-3, 1e-2), 1, 3),
     "pgh1_n8":       (1, 8, 0, 0, 1e-9, 1e-6, 1e-3, 15, 3, (1e-4, 1e-2), 1, 3),
 }
+
+
+# extra targets (ifpghtarg >= 1):
+# name: (ndim, norder, ipoly, iperiod, tree_eps, eps, delta, seed, ng, width range, nd, ifpgh, ifpghtarg, ntarg)
+TARG_CASES = {
+    "tgt3_n8":       (3, 8, 0, 0, 1e-2, 1e-6, 1e-3, 16, 3, (1e-3, 1e-2), 1, 1, 1, 400),
+    "tgt3_pgh_n6":   (3, 6, 0, 1, 1e-1, 1e-6, 1e-3, 17, 2, (2e-3, 1e-2), 1, 3, 3, 300),
+    "tgt2_up_n8":    (2, 8, 1, 0, 1e-6, 1e-6, 1e-3, 18, 3, (1e-3, 1e-2), 2, 1, 3, 500),
+    "tgt1_n8":       (1, 8, 0, 1, 1e-9, 1e-6, 1e-3, 19, 3, (1e-4, 1e-2), 1, 2, 2, 200),
+}
+
+
+def make_targets(ndim, nt, seed, tr):
+    """Uniform points in the root box, plus leaf centres and points on box faces."""
+    rng = np.random.default_rng(seed + 1000)
+    t = rng.uniform(-0.5, 0.5, (nt, ndim))
+    nch = tr.itree[tr.iptr[3] - 1: tr.iptr[3] - 1 + tr.nboxes]
+    leaves = np.nonzero(nch == 0)[0]
+    k = min(len(leaves), nt // 8)
+    t[:k] = tr.centers[leaves[:k]]                       # leaf centres
+    t[k:2 * k] = np.round(t[k:2 * k] * 8) / 8            # on box faces / corners
+    return t
 
 
 def make_source(ndim, seed, ng, wr, iperiod):
@@ -113,6 +137,35 @@ def pgh_case(name, spec):
                         gauss_a=a, **extra)
     print(f"{name}: nboxes={tr.nboxes} nlevels={tr.nlevels} leaves={len(leaves)} "
           f"|grad|={np.abs(grad).max():.3e} size={os.path.getsize(out)/1e6:.2f} MB")
+
+
+def targ_case(name, spec):
+    ndim, n, ipoly, iper, teps, eps, delta, seed, ng, wr, nd, ifpgh, ifpghtarg, nt = spec
+    c, s, a = make_source(ndim, seed, ng, wr, iper)
+    dp = refrun.gauss_params(c, s, a)
+    tr = refrun.build_tree(ndim, n, ipoly, iper, teps, dp, ng)
+    f = tr.fvals
+    if nd == 2:
+        f = np.stack([tr.fvals, np.cos(3.0 * tr.fvals)], axis=-1)
+    targs = make_targets(ndim, nt, seed, tr)
+    pot, grad, hess, pote, grade, hesse, _ = refrun.boxfgt(tr, delta, eps, nd=nd, fvals=f, ifpgh=ifpgh,
+                                                           targs=targs, ifpghtarg=ifpghtarg)
+    out = os.path.join(HERE, f"boxfgt_{name}.npz")
+    nch = tr.itree[tr.iptr[3] - 1: tr.iptr[3] - 1 + tr.nboxes]
+    leaves = np.nonzero(nch == 0)[0].astype(np.int32)
+    # grad/hess on the leaves are covered by the pgh fixtures; keep the file small
+    extra = {"pote": pote, "targs": targs, "ifpghtarg": ifpghtarg}
+    if grade is not None:
+        extra["grade"] = grade
+    if hesse is not None:
+        extra["hesse"] = hesse
+    np.savez_compressed(out, ndim=ndim, norder=n, ipoly=ipoly, iperiod=iper, nboxes=tr.nboxes,
+                        nlevels=tr.nlevels, ltree=tr.ltree, itree=tr.itree, iptr=tr.iptr,
+                        centers=tr.centers, boxsize=tr.boxsize, leaves=leaves,
+                        fvals_leaf=f[leaves], pot_leaf=pot[leaves], ifpgh=ifpgh, eps=eps, delta=delta, nd=nd,
+                        tree_eps=teps, gauss_c=c, gauss_s=s, gauss_a=a, **extra)
+    print(f"{name}: nboxes={tr.nboxes} nlevels={tr.nlevels} nt={nt} |pote|={np.abs(pote).max():.3e} "
+          f"size={os.path.getsize(out)/1e6:.2f} MB")
 
 
 def _dp(x):
@@ -212,3 +265,6 @@ if __name__ == "__main__":
     for name, spec in PGH_CASES.items():
         if not only or name in only:
             pgh_case(name, spec)
+    for name, spec in TARG_CASES.items():
+        if not only or name in only:
+            targ_case(name, spec)
