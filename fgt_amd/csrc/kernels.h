@@ -87,6 +87,8 @@ bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* b
                       const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od,
                       hipStream_t s, size_t lds_limit);
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
+// true when FGT_PW2POT_VALU=1: STEP 5 keeps its x stage on the VALU
+bool pw2pot_valu_only();
 // Fused STEP 1 + 2 for parents with leaf children (3-D, N in {8,12,16}):
 // plev/pslot per parent, kids/ksg 8 per parent (leaf child box or -1, merge
 // sign bits).  Writes (overwrites) the parent's mp; returns false if not covered.

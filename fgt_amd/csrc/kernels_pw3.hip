@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace fgt {
 
@@ -256,7 +257,16 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // tables; otherwise bit0 set and bits 1..3 = split signs (x,y,z): evaluate the
 // parent's local expansion with the tables pre-multiplied by the parent->child
 // phase (tabS[l][sign][k][i]) -- STEP 4 folded into STEP 5 for leaves.
-template <int N>
+// MF: x stage on the FP64 matrix cores (N = 16, npw <= 64): per slab the
+// complex product ff[kk1][k2] = sum_k1 U[kk1][k1] E[k1][k2] runs as
+// v_mfma_f64_16x16x4f64 tiles (rows kk1, columns 16 k2, K = 4 k1), four real
+// products per tile (Ur Er - Ui Ei, Ur Ei + Ui Er).  Operand layout of that
+// instruction: A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], result lane
+// l, element v = D[l/16 + 4v][l%16].  The table is read per lane from
+// global memory (cached); E rows are streamed straight into the B operand.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int N, bool MF>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
                                                 const double2* __restrict__ tabS, int npw,
                                                 const int* __restrict__ box, const int* __restrict__ lev,
@@ -309,7 +319,47 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             // Rows of E are fetched PF at a time (all loads issued before the
             // FMAs that use them) so enough reads are in flight to cover HBM
             // latency; lanes past npw read a valid duplicate and discard it.
-            {
+            if constexpr (MF) {
+                const int k3 = r + wave;
+                if (k3 < nh) {
+                    const double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    const int li = lane & 15, lk = lane >> 4;
+                    dbl4 are[4], aim[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) are[t] = aim[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    const int ksteps = (npw + 3) >> 2;
+                    for (int st = 0; st < ksteps; ++st) {
+                        const int k1 = 4 * st + lk;
+                        const bool kv = k1 < npw;
+                        const int k1c = kv ? k1 : npw - 1;
+                        double2 u = UX[k1c * N + li];
+                        if (!kv) u = make_double2(0.0, 0.0);
+                        double2 e[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int k2 = 16 * t + li;
+                            e[t] = Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)];
+                        }
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, e[t].x, are[t], 0, 0, 0);
+                            are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, e[t].y, are[t], 0, 0, 0);
+                            aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, e[t].y, aim[t], 0, 0, 0);
+                            aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, e[t].x, aim[t], 0, 0, 0);
+                        }
+                    }
+                    // lane holds kk1 = lk + 4v for column k2 = 16t + li
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int k2 = 16 * t + li;
+                        if (k2 < npw)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v)
+                                sff[(k2 * 4 + wave) * 16 + ((lk + 4 * v) ^ (k2 & 15))] =
+                                    make_double2(are[t][v], aim[t][v]);
+                    }
+                }
+            } else {
                 const int k3 = r + wave;
                 if (k3 < nh) {
                     const double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
@@ -407,12 +457,26 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, c
     const size_t lds = sff + sizeof(double2) * (size_t)N * t.npw;
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
-    hipLaunchKernelGGL(k_pw2pot3<N>, dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw, box, lev,
-                       slot, sel, expn, t.pw2potS_ks, kd, od);
+    if (N == 16 && t.npw <= 64 && !pw2pot_valu_only())
+        hipLaunchKernelGGL((k_pw2pot3<N, true>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw,
+                           box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
+    else
+        hipLaunchKernelGGL((k_pw2pot3<N, false>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw,
+                           box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
     return true;
 }
 
 }  // namespace
+
+// FGT_PW2POT_VALU=1 selects the FP64 VALU x stage of STEP 5 instead of the
+// matrix-core one (for comparison runs).
+bool pw2pot_valu_only() {
+    static const bool v = [] {
+        const char* e = std::getenv("FGT_PW2POT_VALU");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
 
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                     const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
