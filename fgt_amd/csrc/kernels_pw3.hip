@@ -23,6 +23,19 @@ __device__ __forceinline__ double2 zfma(double2 t, double2 f, double2 c) {
     return make_double2(c.x + t.x * f.x - t.y * f.y, c.y + t.x * f.y + t.y * f.x);
 }
 
+// Complex 16x16x4 tile product on the FP64 matrix cores: (re, im) += A B with
+// A, B complex, as four v_mfma_f64_16x16x4f64 (Ar Br - Ai Bi, Ar Bi + Ai Br).
+// Operand layout of that instruction: A lane l = A[l%16][l/16], B lane l =
+// B[l/16][l%16]; result lane l, element v = D[l/16 + 4v][l%16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void cmfma(double2 a, double2 b, dbl4& re, dbl4& im) {
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, im, 0, 0, 0);
+}
+
 // ---------------------------------------------------------------- STEP 1
 // LDS: sff [m2][s][m1] (N*4*N complex), sff2 [s][m1][k2] (4*N*npw complex)
 template <int N>
@@ -259,12 +272,10 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // phase (tabS[l][sign][k][i]) -- STEP 4 folded into STEP 5 for leaves.
 // MF: x stage on the FP64 matrix cores (N = 16, npw <= 64): per slab the
 // complex product ff[kk1][k2] = sum_k1 U[kk1][k1] E[k1][k2] runs as
-// v_mfma_f64_16x16x4f64 tiles (rows kk1, columns 16 k2, K = 4 k1), four real
-// products per tile (Ur Er - Ui Ei, Ur Ei + Ui Er).  Operand layout of that
-// instruction: A lane l = A[l%16][l/16], B lane l = B[l/16][l%16], result lane
-// l, element v = D[l/16 + 4v][l%16].  The table is read per lane from
-// global memory (cached); E rows are streamed straight into the B operand.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
+// 16x16x4 tiles (rows kk1, columns 16 k2, K = 4 k1).  The table is read per
+// lane from global memory (cached); E rows stream straight into the B
+// operand.  (For STEP 1 the matrix-core form is slower: its 50 x 50 output
+// pads to 64 x 64 tiles, where the VALU form wastes only the lane padding.)
 
 template <int N, bool MF>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
@@ -341,12 +352,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                             e[t] = Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)];
                         }
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, e[t].x, are[t], 0, 0, 0);
-                            are[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, e[t].y, are[t], 0, 0, 0);
-                            aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, e[t].y, aim[t], 0, 0, 0);
-                            aim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, e[t].x, aim[t], 0, 0, 0);
-                        }
+                        for (int t = 0; t < 4; ++t) cmfma(u, e[t], are[t], aim[t]);
                     }
                     // lane holds kk1 = lk + 4v for column k2 = 16t + li
 #pragma unroll
