@@ -229,9 +229,11 @@ __global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
 constexpr int kM2lGroup = 8;
 constexpr int kM2lBatch = 4;   // source loads issued together (list padding unit)
 
+// (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
+// so the kernel stays small in registers and more waves are resident
 #define M2L_CASE(o)                                                     \
     case o:                                                             \
-        acc = cfma(v, cmul(fxy[(o) % 9], fz[(o) / 9]), acc);             \
+        acc = cfma(cmul(v, fxy[(o) % 9]), fz[(o) / 9], acc);             \
         break;
 
 __device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, const double2 (&fxy)[9],
