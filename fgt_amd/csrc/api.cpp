@@ -371,7 +371,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
                 soff.push_back(se * P.m2l_src[j]);
                 so.push_back(mi[j]);
             }
-            while (soff.size() % 4) {
+            while (soff.size() % 2) {
                 soff.push_back(se * P.nslots);   // the zero slot
                 so.push_back(13);                // centre offset (unit phase)
             }

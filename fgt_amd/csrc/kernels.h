@@ -108,7 +108,7 @@ void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                 hipStream_t s);
 // STEP 3 with targets grouped per block.  toff[task]: element offset of the
 // target slot (slot * 2 * nd * nexp); poff: CSR over list entries, each
-// target's list padded to a multiple of 4; soff[j]: element offset of the
+// target's list padded to a multiple of 2; soff[j]: element offset of the
 // source slot (padding points at an all-zero slot); so[j]: offset index
 // o = ox + 3 oy + 9 oz in 0..26.
 void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,

@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
 // on it picks the phase factors from fixed registers (scalar branches, no
 // per-lane selects), and four source loads are issued ahead of their use.
 constexpr int kM2lGroup = 8;
-constexpr int kM2lBatch = 4;   // source loads issued together (list padding unit)
+constexpr int kM2lBatch = 2;   // source loads issued together (list padding unit)
 
 // (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
 // so the kernel stays small in registers and more waves are resident
