@@ -383,16 +383,27 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         S.d_m2l_so.upload(so);
     }
 
-    S.d_dir_tgt.upload(P.dir_tgt);
-    S.d_dir_lev.upload(P.dir_lev);
-    S.d_dir_off.upload(P.dir_off);
-    // near-field entries: per target, sorted by (z, y, x) table slot so the
-    // 3-D kernel can share the y and z contractions within a group
+    // near-field targets, longest list first (one block per target: the
+    // longest-processing-time order shortens the tail of the launch); entries
+    // of a target sorted by (z, y, x) table slot so the 3-D kernel can share
+    // the y and z contractions within a group
+    const int ndt = (int)P.dir_tgt.size();
+    std::vector<int> order(ndt);
+    for (int i = 0; i < ndt; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return P.dir_off[x + 1] - P.dir_off[x] > P.dir_off[y + 1] - P.dir_off[y];
+    });
+    std::vector<int> dt(ndt), dl(ndt), doff(ndt + 1, 0);
     std::vector<int> ds(P.dir_src.size()), dc(P.dir_code.size()), df(P.dir_code.size(), 0);
-    for (size_t t2 = 0; t2 + 1 < P.dir_off.size(); ++t2) {
-        const int a = P.dir_off[t2], b = P.dir_off[t2 + 1];
-        std::vector<int> idx(b - a);
-        for (int i = a; i < b; ++i) idx[i - a] = i;
+    for (int q = 0; q < ndt; ++q) {
+        const int t2 = order[q];
+        const int a0 = P.dir_off[t2], b0 = P.dir_off[t2 + 1];
+        dt[q] = P.dir_tgt[t2];
+        dl[q] = P.dir_lev[t2];
+        const int a = doff[q], b = a + (b0 - a0);
+        doff[q + 1] = b;
+        std::vector<int> idx(b0 - a0);
+        for (int i = a0; i < b0; ++i) idx[i - a0] = i;
         std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
             const int cx = map_code(ndim, P.dir_code[x]), cy = map_code(ndim, P.dir_code[y]);
             return (cx >> 4) < (cy >> 4);
@@ -407,6 +418,9 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
             if (last || (dc[i] >> 8) != (dc[i + 1] >> 8)) df[i] |= 2;
         }
     }
+    S.d_dir_tgt.upload(dt);
+    S.d_dir_lev.upload(dl);
+    S.d_dir_off.upload(doff);
     S.d_dir_src.upload(ds);
     S.d_dir_code.upload(dc);
     S.d_dir_flags.upload(df);
