@@ -83,6 +83,77 @@ def measured_traffic(kernel):
     return None
 
 
+def c4_problem(i, base=1000):
+    """Uneven-batch generator: 25% 2-D / 75% 3-D, norder in {8,12,16}, eps in
+    {1e-3,1e-6,1e-9}, delta log-uniform in [1e-5,1e-1], 1-10 Gaussians with
+    centres U[-0.4,0.4], widths log-U[1e-3,1e-2], amplitudes U[0.5,1.5];
+    seed = base + index; free space, Legendre, tree refined to eps."""
+    rng = np.random.default_rng(base + i)
+    ndim = 2 if rng.uniform() < 0.25 else 3
+    n = int(rng.choice([8, 12, 16]))
+    eps = float(rng.choice([1e-3, 1e-6, 1e-9]))
+    delta = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-1))))
+    ng = int(rng.integers(1, 11))
+    c = rng.uniform(-0.4, 0.4, (ng, ndim))
+    s = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), (ng, ndim)))
+    a = rng.uniform(0.5, 1.5, ng)
+    return ndim, n, eps, delta, (c, s, a)
+
+
+def batched_comparison(nprob, nstreams, steps):
+    """nprob problems from the uneven generator (trees above 8000 leaves are
+    skipped, keeping leaf counts in the 50-8000 range): all executed
+    overlapped on nstreams streams (fgt_batch) vs one at a time
+    (device-resident solver per problem, synchronised after each)."""
+    import torch
+    import fgt_amd
+    from fgt_amd.tree import gauss_source
+    t0 = time.time()
+    probs, i = [], 0
+    while len(probs) < nprob:
+        ndim, n, eps, delta, g = c4_problem(i)
+        i += 1
+        tr = fgt_amd.vol_tree_build(ndim, n, 0, 0, eps, gauss_source(*g))
+        if len(tr.leaves) > 8000:
+            continue
+        probs.append((tr, delta, eps))
+    log(f"batch: {nprob} problems ({i} drawn) built in {time.time() - t0:.1f}s")
+    b = fgt_amd.Batch(probs, nstreams=nstreams)
+    for k, (tr, _, _) in enumerate(probs):
+        b.set_fvals(k, tr.fvals)
+    b.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    t_b = (time.perf_counter() - t0) / steps
+    nout = b.stats()["nout"]
+    b.close()
+    solvers = []
+    for tr, delta, eps in probs:
+        sv = fgt_amd.Solver.from_tree(tr, delta, eps)
+        sv.set_fvals(tr.fvals)
+        solvers.append(sv)
+    for sv in solvers:
+        sv.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sv in solvers:
+            sv.execute()
+            torch.cuda.synchronize()
+    t_p = (time.perf_counter() - t0) / steps
+    for sv in solvers:
+        sv.close()
+    dims = sum(1 for tr, _, _ in probs if tr.ndim == 2)
+    return {"problems": nprob, "two_d": dims, "nstreams": nstreams, "output_nodes": int(nout),
+            "leaves_min": int(min(len(t.leaves) for t, _, _ in probs)),
+            "leaves_max": int(max(len(t.leaves) for t, _, _ in probs)),
+            "ms_batched": round(t_b * 1e3, 3), "ms_one_at_a_time": round(t_p * 1e3, 3),
+            "nodes_per_s_batched": round(nout / t_b, 1), "nodes_per_s_one_at_a_time": round(nout / t_p, 1),
+            "speedup": round(t_p / t_b, 3), "dtype": "f64"}
+
+
 def cpu_baseline(tree, cfg):
     """Reference boxfgt (oracle/_ref, OpenMP) on the same tree and fvals."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -131,6 +202,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=128,
+                    help="problems in the uneven-batch comparison (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
     args = ap.parse_args()
@@ -202,6 +275,11 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device copies done")
 
+    batched = None
+    if args.batch > 0:
+        batched = batched_comparison(args.batch, 4, max(1, min(args.steps, 5)))
+        log(f"batched: {batched}")
+
     cpu = None
     if not args.no_cpu_baseline:
         secs, kind, threads = cpu_baseline(tree, cfg)
@@ -230,6 +308,7 @@ def main():
                    "pw_leaves": int(st["npw_leaves"]), "direct_pairs": int(st["ndirect_pairs"])},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     s.close()

@@ -262,7 +262,70 @@ def plan_summary(tree, delta, eps, nd=1):
     return {k: getattr(info, k) for k, _ in _PlanInfo._fields_}
 
 
+class _Problem(C.Structure):
+    _fields_ = [("nd", C.c_int), ("ndim", C.c_int), ("delta", C.c_double), ("eps", C.c_double),
+                ("ipoly", C.c_int), ("iperiod", C.c_int), ("norder", C.c_int), ("nboxes", C.c_int),
+                ("nlevels", C.c_int), ("ltree", C.c_int), ("itree", C.c_void_p), ("iptr", C.c_void_p),
+                ("centers", C.c_void_p), ("boxsize", C.c_void_p)]
+
+
+class Batch:
+    """Many independent box FGT problems planned together and executed
+    overlapped on ``nstreams`` HIP streams (largest problem first).
+
+    ``problems``: sequence of (tree, delta, eps) with ``tree`` a
+    :class:`fgt_amd.tree.Tree` (or any object with the same attributes).
+    """
+
+    def __init__(self, problems, nstreams=4, nd=1):
+        self._h = C.c_void_p()
+        self._keep = []
+        arr = (_Problem * len(problems))()
+        self.shapes = []
+        for i, (t, delta, eps) in enumerate(problems):
+            it = np.ascontiguousarray(t.itree, np.int32)
+            ip = np.ascontiguousarray(t.iptr, np.int32)
+            ce = np.ascontiguousarray(t.centers, np.float64)
+            bs = np.ascontiguousarray(t.boxsize, np.float64)
+            self._keep += [it, ip, ce, bs]
+            arr[i] = _Problem(nd, t.ndim, delta, eps, t.ipoly, t.iperiod, t.norder, t.nboxes, t.nlevels, t.ltree,
+                              it.ctypes.data, ip.ctypes.data, ce.ctypes.data, bs.ctypes.data)
+            npbox = t.norder ** t.ndim
+            self.shapes.append((t.nboxes, npbox) if nd == 1 else (t.nboxes, npbox, nd))
+        _check(lib().fgt_batch_create(C.c_int(len(problems)), arr, C.c_int(nstreams), C.byref(self._h)))
+
+    def set_fvals(self, i, fvals):
+        f = np.ascontiguousarray(fvals, np.float64)
+        if f.size != int(np.prod(self.shapes[i])):
+            raise ValueError("fvals size mismatch")
+        _check(lib().fgt_batch_set_fvals(self._h, C.c_int(i), _p(f)))
+
+    def execute(self):
+        _check(lib().fgt_batch_execute(self._h))
+
+    def get_pot(self, i):
+        out = np.zeros(self.shapes[i])
+        _check(lib().fgt_batch_get_pot(self._h, C.c_int(i), _p(out), C.c_int(0)))
+        return out
+
+    def stats(self):
+        n, ms = C.c_longlong(), C.c_double()
+        _check(lib().fgt_batch_stats(self._h, C.byref(n), C.byref(ms)))
+        return {"nout": n.value, "last_ms": ms.value}
+
+    def close(self):
+        if self._h:
+            lib().fgt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 from .tree import Tree, vol_tree_build  # noqa: E402  (host tree builder)
 
-__all__ = ["boxfgt", "Solver", "FgtError", "plan_summary", "Tree", "vol_tree_build", "lib",
+__all__ = ["boxfgt", "Solver", "Batch", "FgtError", "plan_summary", "Tree", "vol_tree_build", "lib",
            "last_status"]

@@ -86,6 +86,24 @@ struct fgt_solver {
     double stage_ms[8] = {0};
 };
 
+// Many independent problems (uneven sizes, orders, delta) solved together:
+// each has its own plan and device buffers; their executes are spread over
+// a few HIP streams (largest first, each onto the least-loaded stream) so
+// small problems overlap on the device instead of running one after another.
+struct fgt_batch {
+    std::vector<std::unique_ptr<fgt_solver>> solvers;
+    std::vector<hipStream_t> streams;
+    std::vector<int> stream_of;     // per problem
+    std::vector<int> order;         // launch order (largest first)
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double last_ms = 0.0;
+    ~fgt_batch() {
+        for (auto s : streams) (void)hipStreamDestroy(s);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+    }
+};
+
 namespace {
 
 // Map problem-axis sign bits / offsets / table codes onto the (x, y, z) axes
@@ -795,6 +813,97 @@ int fgt_solver_eval_targets(fgt_solver* s, int ntarg, const double* targs, int i
 
 void* fgt_solver_grad_device(fgt_solver* s) { return s ? s->d_grad.p : nullptr; }
 void* fgt_solver_hess_device(fgt_solver* s) { return s ? s->d_hess.p : nullptr; }
+
+int fgt_batch_create(int nprob, const fgt_problem* probs, int nstreams, fgt_batch** out) {
+    if (!out) return 2;
+    *out = nullptr;
+    return guarded([&] {
+        if (nprob < 1 || !probs) throw std::invalid_argument("empty batch");
+        if (nstreams < 1 || nstreams > 16) throw std::invalid_argument("nstreams must be 1..16");
+        auto B = std::make_unique<fgt_batch>();
+        std::vector<double> work(nprob);
+        for (int i = 0; i < nprob; ++i) {
+            const fgt_problem& q = probs[i];
+            if (!q.itree || !q.iptr || !q.centers || !q.boxsize) throw std::invalid_argument("null tree array");
+            int npbox = 1;
+            for (int a = 0; a < q.ndim && a < 3; ++a) npbox *= q.norder;
+            fgt::PlanInput in{q.nd, q.ndim, q.delta, q.eps, q.ipoly, q.iperiod, q.norder, npbox, q.nboxes,
+                              q.nlevels, q.ltree, q.itree, q.iptr, q.centers, q.boxsize};
+            auto S = std::make_unique<fgt_solver>();
+            build_solver(*S, in);
+            work[i] = (double)S->plan.nd * S->plan.npbox * S->plan.nleaves;
+            B->solvers.push_back(std::move(S));
+        }
+        for (int k = 0; k < nstreams; ++k) {
+            hipStream_t st;
+            hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+            B->streams.push_back(st);
+        }
+        B->order.resize(nprob);
+        for (int i = 0; i < nprob; ++i) B->order[i] = i;
+        std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
+        std::vector<double> load(nstreams, 0.0);
+        B->stream_of.assign(nprob, 0);
+        for (int i : B->order) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            B->stream_of[i] = k;
+            load[k] += work[i];
+        }
+        hcheck(hipEventCreate(&B->t0), "hipEventCreate");
+        hcheck(hipEventCreate(&B->t1), "hipEventCreate");
+        *out = B.release();
+    });
+}
+
+int fgt_batch_set_fvals(fgt_batch* b, int i, const double* fvals) {
+    return guarded([&] {
+        if (!b || !fvals || i < 0 || i >= (int)b->solvers.size()) throw std::invalid_argument("bad argument");
+        fgt_solver& S = *b->solvers[i];
+        hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
+    });
+}
+
+int fgt_batch_execute(fgt_batch* b) {
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        // all streams start after t0 and t1 waits for all of them
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hcheck(hipEventRecord(b->t0, b->streams[0]), "hipEventRecord");
+        for (size_t k = 1; k < b->streams.size(); ++k)
+            hcheck(hipStreamWaitEvent(b->streams[k], b->t0, 0), "hipStreamWaitEvent");
+        for (int i : b->order) execute(*b->solvers[i], b->streams[b->stream_of[i]], false);
+        std::vector<hipEvent_t> done(b->streams.size());
+        for (size_t k = 1; k < b->streams.size(); ++k) {
+            hcheck(hipEventCreate(&done[k]), "hipEventCreate");
+            hcheck(hipEventRecord(done[k], b->streams[k]), "hipEventRecord");
+            hcheck(hipStreamWaitEvent(b->streams[0], done[k], 0), "hipStreamWaitEvent");
+        }
+        hcheck(hipEventRecord(b->t1, b->streams[0]), "hipEventRecord");
+        hcheck(hipEventSynchronize(b->t1), "hipEventSynchronize");
+        for (size_t k = 1; k < done.size(); ++k) (void)hipEventDestroy(done[k]);
+        float ms = 0;
+        hcheck(hipEventElapsedTime(&ms, b->t0, b->t1), "hipEventElapsedTime");
+        b->last_ms = ms;
+    });
+}
+
+int fgt_batch_get_pot(fgt_batch* b, int i, double* pot, int accumulate) {
+    return guarded([&] {
+        if (!b || !pot || i < 0 || i >= (int)b->solvers.size()) throw std::invalid_argument("bad argument");
+        download(b->solvers[i]->d_pot, pot, accumulate, "download pot");
+    });
+}
+
+int fgt_batch_stats(const fgt_batch* b, long long* nout, double* last_ms) {
+    if (!b) return 2;
+    long long n = 0;
+    for (const auto& S : b->solvers) n += (long long)S->plan.nd * S->plan.npbox * S->plan.nleaves;
+    if (nout) *nout = n;
+    if (last_ms) *last_ms = b->last_ms;
+    return 0;
+}
+
+void fgt_batch_destroy(fgt_batch* b) { delete b; }
 
 void* fgt_solver_fvals_device(fgt_solver* s) { return s ? s->d_fvals.p : nullptr; }
 void* fgt_solver_pot_device(fgt_solver* s) { return s ? s->d_pot.p : nullptr; }

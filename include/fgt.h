@@ -119,6 +119,28 @@ void *fgt_solver_grad_device(fgt_solver *s);   /* NULL until ifpgh >= 2 */
 void *fgt_solver_hess_device(fgt_solver *s);   /* NULL until ifpgh >= 3 */
 void fgt_solver_destroy(fgt_solver *s);
 
+/* ---- many independent problems (no counterpart in the reference, which
+ *      solves one problem per call) ---- */
+typedef struct {
+    int nd, ndim;
+    double delta, eps;
+    int ipoly, iperiod, norder, nboxes, nlevels, ltree;
+    const int *itree, *iptr;
+    const double *centers, *boxsize;
+} fgt_problem;
+typedef struct fgt_batch fgt_batch;
+/* Plans every problem (arguments as fgt_solver_create) and creates nstreams
+ * (1..16) HIP streams; executes are spread over them, largest first. */
+int fgt_batch_create(int nprob, const fgt_problem *probs, int nstreams,
+                     fgt_batch **out);
+int fgt_batch_set_fvals(fgt_batch *b, int i, const double *fvals);
+/* Runs STEPs 1-6 of every problem, overlapped, and waits for all. */
+int fgt_batch_execute(fgt_batch *b);
+int fgt_batch_get_pot(fgt_batch *b, int i, double *pot, int accumulate);
+/* total output nodes of the batch; device time of the last execute (ms) */
+int fgt_batch_stats(const fgt_batch *b, long long *nout, double *last_ms);
+void fgt_batch_destroy(fgt_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
