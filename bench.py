@@ -100,12 +100,9 @@ def c4_problem(i, base=1000):
     return ndim, n, eps, delta, (c, s, a)
 
 
-def batched_comparison(nprob, nstreams, steps):
-    """nprob problems from the uneven generator (trees above 8000 leaves are
-    skipped, keeping leaf counts in the 50-8000 range): all executed
-    overlapped on nstreams streams (fgt_batch) vs one at a time
-    (device-resident solver per problem, synchronised after each)."""
-    import torch
+def c4_problems(nprob):
+    """nprob problems from the uneven generator; trees above 8000 leaves are
+    skipped, keeping leaf counts in the 50-8000 range."""
     import fgt_amd
     from fgt_amd.tree import gauss_source
     t0 = time.time()
@@ -118,6 +115,16 @@ def batched_comparison(nprob, nstreams, steps):
             continue
         probs.append((tr, delta, eps))
     log(f"batch: {nprob} problems ({i} drawn) built in {time.time() - t0:.1f}s")
+    return probs
+
+
+def batched_comparison(probs, nstreams, steps):
+    """All problems executed overlapped on nstreams streams (fgt_batch) vs
+    one at a time (device-resident solver per problem, synchronised after
+    each)."""
+    import torch
+    import fgt_amd
+    nprob = len(probs)
     b = fgt_amd.Batch(probs, nstreams=nstreams)
     for k, (tr, _, _) in enumerate(probs):
         b.set_fvals(k, tr.fvals)
@@ -152,6 +159,29 @@ def batched_comparison(nprob, nstreams, steps):
             "ms_batched": round(t_b * 1e3, 3), "ms_one_at_a_time": round(t_p * 1e3, 3),
             "nodes_per_s_batched": round(nout / t_b, 1), "nodes_per_s_one_at_a_time": round(nout / t_p, 1),
             "speedup": round(t_p / t_b, 3), "dtype": "f64"}
+
+
+def serving(probs, rate, seconds, nstreams):
+    """Poisson arrivals (seeded) at `rate` per second for `seconds`, each
+    naming one of the problems; served with continuous admission on
+    nstreams streams and, for comparison, one at a time (one stream).
+    Problems are made resident before the clock starts."""
+    from fgt_amd.serve import Server, poisson_arrivals
+    nout = [len(t.leaves) * t.norder ** t.ndim for t, _, _ in probs]
+    arr = poisson_arrivals(rate, seconds, len(probs), seed=7)
+    out = {"rate_per_s": rate, "seconds": seconds, "requests": len(arr), "problems": len(probs)}
+    for label, ns in (("served", nstreams), ("one_at_a_time", 1)):
+        srv = Server(nstreams=ns)
+        for k, (tr, delta, eps) in enumerate(probs):
+            srv.register(k, tr, delta, eps)
+            srv._solver(k)
+        lat, wall = srv.run(arr, seconds)
+        srv.close()
+        v = np.array([lat[r] for _, r, _ in arr]) * 1e3
+        done = sum(nout[k] for _, _, k in arr)
+        out[label] = {"nstreams": ns, "nodes_per_s": round(done / wall, 1), "p50_ms": round(float(np.percentile(v, 50)), 3),
+                      "p99_ms": round(float(np.percentile(v, 99)), 3), "wall_s": round(wall, 3)}
+    return out
 
 
 def cpu_baseline(tree, cfg):
@@ -204,6 +234,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128,
                     help="problems in the uneven-batch comparison (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
     args = ap.parse_args()
@@ -275,10 +307,14 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 1 GiB device copies done")
 
-    batched = None
+    batched = served = None
     if args.batch > 0:
-        batched = batched_comparison(args.batch, 4, max(1, min(args.steps, 5)))
+        probs = c4_problems(args.batch)
+        batched = batched_comparison(probs, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
+        if args.serve_seconds > 0:
+            served = serving(probs, args.serve_rate, args.serve_seconds, 4)
+            log(f"served: {served}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -309,6 +345,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     s.close()
