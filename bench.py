@@ -234,8 +234,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=128,
                     help="problems in the uneven-batch comparison (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=1000.0, help="request arrivals per second")
-    ap.add_argument("--serve-seconds", type=float, default=3.0, help="arrival window of the serving run (0 = skip)")
+    ap.add_argument("--serve-rates", default="1000,3000", help="request arrivals per second (comma list)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
     args = ap.parse_args()
@@ -313,7 +313,7 @@ def main():
         batched = batched_comparison(probs, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
         if args.serve_seconds > 0:
-            served = serving(probs, args.serve_rate, args.serve_seconds, 4)
+            served = [serving(probs, float(r), args.serve_seconds, 4) for r in args.serve_rates.split(",")]
             log(f"served: {served}")
 
     cpu = None
