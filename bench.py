@@ -135,6 +135,20 @@ def batched_comparison(probs, nstreams, steps):
         b.execute()
     t_b = (time.perf_counter() - t0) / steps
     nout = b.stats()["nout"]
+    ref = [b.get_pot(k) for k in range(nprob)]
+    # mixed precision (FP32 expansions) where eps >= 1e-6 and the 3-D path applies
+    nmixed = b.set_precision(1, 1e-6)
+    b.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    t_m = (time.perf_counter() - t0) / steps
+    worst = 0.0
+    for k, (tr, delta, eps) in enumerate(probs):
+        den = np.linalg.norm(ref[k])
+        err = np.linalg.norm(b.get_pot(k) - ref[k]) / (den if den > 0 else 1.0)
+        worst = max(worst, err / eps)
     b.close()
     solvers = []
     for tr, delta, eps in probs:
@@ -158,7 +172,10 @@ def batched_comparison(probs, nstreams, steps):
             "leaves_max": int(max(len(t.leaves) for t, _, _ in probs)),
             "ms_batched": round(t_b * 1e3, 3), "ms_one_at_a_time": round(t_p * 1e3, 3),
             "nodes_per_s_batched": round(nout / t_b, 1), "nodes_per_s_one_at_a_time": round(nout / t_p, 1),
-            "speedup": round(t_p / t_b, 3), "dtype": "f64"}
+            "speedup": round(t_p / t_b, 3), "dtype": "f64",
+            "mixed": {"problems_fp32_expansions": nmixed, "ms": round(t_m * 1e3, 3),
+                      "nodes_per_s": round(nout / t_m, 1), "speedup_vs_one_at_a_time_fp64": round(t_p / t_m, 3),
+                      "max_rel_l2_err_over_eps": float(f"{worst:.3g}")}}
 
 
 def serving(probs, rate, seconds, nstreams):

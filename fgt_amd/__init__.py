@@ -178,6 +178,10 @@ class Solver:
     def execute(self, stream=None, timed=False):
         _check(lib().fgt_solver_execute(self._h, C.c_void_p(stream or 0), C.c_int(1 if timed else 0)))
 
+    def set_precision(self, mode):
+        """0: FP64 expansions; 1: mixed (FP32 expansions, FP64 arithmetic)."""
+        _check(lib().fgt_solver_set_precision(self._h, C.c_int(mode)))
+
     def set_ifpgh(self, ifpgh):
         """1 potential, 2 + gradient, 3 + Hessian for later executes."""
         _check(lib().fgt_solver_set_ifpgh(self._h, C.c_int(ifpgh)))
@@ -293,6 +297,13 @@ class Batch:
             npbox = t.norder ** t.ndim
             self.shapes.append((t.nboxes, npbox) if nd == 1 else (t.nboxes, npbox, nd))
         _check(lib().fgt_batch_create(C.c_int(len(problems)), arr, C.c_int(nstreams), C.byref(self._h)))
+
+    def set_precision(self, mode, eps_min=1e-6):
+        """Mixed precision (mode 1) for the problems that support it and have
+        eps >= eps_min; returns how many switched."""
+        n = C.c_int()
+        _check(lib().fgt_batch_set_precision(self._h, C.c_int(mode), C.c_double(eps_min), C.byref(n)))
+        return n.value
 
     def set_fvals(self, i, fvals):
         f = np.ascontiguousarray(fvals, np.float64)

@@ -132,6 +132,17 @@ std::vector<double2> to_d2(const std::vector<fgt::cplx>& v) {
     return o;
 }
 
+// Expansion workspace: nslots slots of (mp, loc), nd * nexp complex each,
+// elem bytes per complex (16 FP64, 8 FP32); one extra slot past the last
+// stays zero (padding entries of the grouped STEP 3 lists read it).
+void alloc_expansions(fgt_solver& S, size_t elem) {
+    const fgt::Plan& P = S.plan;
+    const size_t slot_elems = 2 * (size_t)P.nd * P.nexp;
+    S.d_expn.alloc(elem * slot_elems * (P.nslots + 1));
+    hcheck(hipMemset(static_cast<char*>(S.d_expn.p) + elem * slot_elems * P.nslots, 0, elem * slot_elems),
+           "zero slot");
+}
+
 void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     std::string err;
     const int rc = fgt::make_plan(in, S.plan, &err);
@@ -179,12 +190,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         S.d_pw2potT.upload(to_d2(tr));
         S.d_wms.upload(to_d2(P.wms));
         S.d_wm2l.upload(to_d2(P.wm2l));
-        // one extra slot past the last stays zero: padding entries of the
-        // grouped STEP 3 lists read it
-        const size_t slot_elems = 2 * (size_t)P.nd * P.nexp;
-        S.d_expn.alloc(sizeof(double2) * slot_elems * (P.nslots + 1));
-        hcheck(hipMemset(S.d_expn.as<double2>() + slot_elems * P.nslots, 0, sizeof(double2) * slot_elems),
-               "zero slot");
+        alloc_expansions(S, sizeof(double2));
     }
     fgt::DevTables& t = S.tabs;
     t.p2pw = S.d_p2pw.as<double2>();
@@ -783,6 +789,19 @@ void download(const DBuf& b, double* out, int accumulate, const char* what) {
 }
 }  // namespace
 
+int fgt_solver_set_precision(fgt_solver* s, int mode) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("null solver");
+        if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
+        if (mode == 1 && !(s->plan.do_pw && s->fast3d))
+            throw std::invalid_argument("mixed precision needs the 3-D register-blocked path (ndim 3, norder 8/12/16)");
+        if (mode == s->tabs.expn_f32) return;
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        alloc_expansions(*s, mode ? sizeof(float2) : sizeof(double2));
+        s->tabs.expn_f32 = mode;
+    });
+}
+
 int fgt_solver_get_grad(fgt_solver* s, double* grad, int accumulate) {
     return guarded([&] {
         if (!s || !grad) throw std::invalid_argument("null argument");
@@ -852,6 +871,20 @@ int fgt_batch_create(int nprob, const fgt_problem* probs, int nstreams, fgt_batc
         hcheck(hipEventCreate(&B->t0), "hipEventCreate");
         hcheck(hipEventCreate(&B->t1), "hipEventCreate");
         *out = B.release();
+    });
+}
+
+int fgt_batch_set_precision(fgt_batch* b, int mode, double eps_min, int* nmixed) {
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
+        int cnt = 0;
+        for (auto& S : b->solvers) {
+            const bool ok = mode == 1 && S->plan.do_pw && S->fast3d && S->plan.eps >= eps_min;
+            if (fgt_solver_set_precision(S.get(), ok ? 1 : 0) != 0) throw std::runtime_error(g_msg);
+            cnt += ok;
+        }
+        if (nmixed) *nmixed = cnt;
     });
 }
 

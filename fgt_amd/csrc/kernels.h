@@ -25,6 +25,13 @@ struct Geom {
 };
 
 #ifdef __HIPCC__
+// Expansion storage: FP64 (double2) or, in the mixed-precision form, FP32
+// (float2) complex numbers; the arithmetic is FP64 either way.
+__device__ __forceinline__ double2 ldE(const double2& v) { return v; }
+__device__ __forceinline__ double2 ldE(const float2& v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void stE(double2* p, double2 v) { *p = v; }
+__device__ __forceinline__ void stE(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
 // range of tasks so that tasks sharing data (children of one parent,
 // neighbouring targets, one element range of many targets) meet in the same L2.
@@ -75,6 +82,9 @@ struct DevTables {
     size_t pw2pot_ks, pw2potS_ks, loctab_ks;
     int n;     // norder
     int npw;
+    // 1: expansions are stored as FP32 complex (mixed precision; 3-D
+    // register-blocked path only), 0: FP64
+    int expn_f32;
     size_t lds_limit;  // device LDS bytes per block
 };
 
@@ -103,9 +113,6 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child,
                   const int* pc_sign, const int* par_acc, int klev, double2* expn, hipStream_t s);
-void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt,
-                const int* off, const int* src, const int* idx, double2* expn,
-                hipStream_t s);
 // STEP 3 with targets grouped per block.  toff[task]: element offset of the
 // target slot (slot * 2 * nd * nexp); poff: CSR over list entries, each
 // target's list padded to a multiple of 2; soff[j]: element offset of the

@@ -117,13 +117,14 @@ __global__ void __launch_bounds__(kBlock) k_p2pw(Geom g, DevTables t, const int*
 }
 
 // ---------------------------------------------------------------- STEP 2
+template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int* __restrict__ par_slot,
                                                   const int* __restrict__ par_first,
                                                   const int* __restrict__ par_count,
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign,
                                                   const int* __restrict__ par_acc, int klev,
-                                                  double2* __restrict__ expn) {
+                                                  ET* __restrict__ expn) {
     const int task = blockIdx.y;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)g.nd * g.nexp;
@@ -133,92 +134,17 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
     const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
     const int first = par_first[task], cnt = par_count[task];
     // par_acc[task] != 0: the parent already holds its leaf children's part
-    double2 acc = (par_acc && par_acc[task]) ? expn[(size_t)par_slot[task] * 2 * tot + e]
+    double2 acc = (par_acc && par_acc[task]) ? ldE(expn[(size_t)par_slot[task] * 2 * tot + e])
                                              : make_double2(0.0, 0.0);
     for (int i = 0; i < cnt; ++i) {
         const int cs = pc_child[first + i];
-        const double2 v = expn[(size_t)cs * 2 * tot + e];
+        const double2 v = ldE(expn[(size_t)cs * 2 * tot + e]);
         acc = cfma(v, ms_phase(w1, pc_sign[first + i], k1, k2, k3, g), acc);
     }
-    expn[(size_t)par_slot[task] * 2 * tot + e] = acc;
+    stE(expn + (size_t)par_slot[task] * 2 * tot + e, acc);
 }
 
 // ---------------------------------------------------------------- STEP 3
-// One block per (element range, target); the linear block index runs over
-// targets fastest and each XCD takes a contiguous run of it, so the sources
-// shared by neighbouring targets (Morton order within a level) are read from
-// HBM about once per element range and then hit in that XCD's L2.
-// Each thread owns kM2lEpt elements (stride blockDim.x).  The 1-D phase
-// factors of its elements (3 offsets per axis) are loaded once into
-// registers and selected per list entry by the wave-uniform offset index;
-// source expansions are fetched four entries ahead of their use.
-constexpr int kM2lEpt = 2;
-
-__device__ __forceinline__ double2 sel3(double2 a, double2 b, double2 c, int o) {
-    return o == 0 ? a : (o == 1 ? b : c);
-}
-
-__global__ void __launch_bounds__(kBlock) k_m2l(Geom g, DevTables t, int ntask,
-                                                const int* __restrict__ tgt,
-                                                const int* __restrict__ off,
-                                                const int* __restrict__ src,
-                                                const int* __restrict__ idx,
-                                                double2* __restrict__ expn) {
-    const int lin = xcd_task(blockIdx.x, gridDim.x);
-    const int task = lin % ntask, chunk = lin / ntask;
-    const long long tot = (long long)g.nd * g.nexp;
-    const int npw = t.npw;
-    long long e[kM2lEpt];
-    bool ok[kM2lEpt];
-    double2 fx[kM2lEpt][3], fy[kM2lEpt][3], fz[kM2lEpt][3], acc[kM2lEpt];
-    const int ts = tgt[task];
-    const double2* __restrict__ base = expn;
-#pragma unroll
-    for (int q = 0; q < kM2lEpt; ++q) {
-        e[q] = ((long long)chunk * kM2lEpt + q) * blockDim.x + threadIdx.x;
-        ok[q] = e[q] < tot;
-        const long long ec = ok[q] ? e[q] : tot - 1;   // valid duplicate, never stored
-        e[q] = ec;
-        const int r = (int)(ec % g.nexp);
-        const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            fx[q][o] = g.px > 1 ? t.wm2l[o * npw + k1] : make_double2(1.0, 0.0);
-            fy[q][o] = g.py > 1 ? t.wm2l[o * npw + k2] : make_double2(1.0, 0.0);
-            fz[q][o] = t.wm2l[o * npw + k3];
-        }
-        acc[q] = base[(size_t)ts * 2 * tot + ec];
-    }
-    const int j0 = off[task], j1 = off[task + 1];
-#define M2L_PHASE(q, o)                                                                     \
-    cmul(cmul(sel3(fx[q][0], fx[q][1], fx[q][2], (o) % 3), sel3(fy[q][0], fy[q][1], fy[q][2], ((o) / 3) % 3)), \
-         sel3(fz[q][0], fz[q][1], fz[q][2], (o) / 9))
-    int j = j0;
-    for (; j + 4 <= j1; j += 4) {
-        double2 v[4][kM2lEpt];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int q = 0; q < kM2lEpt; ++q) v[u][q] = base[(size_t)src[j + u] * 2 * tot + e[q]];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int o = idx[j + u];
-#pragma unroll
-            for (int q = 0; q < kM2lEpt; ++q) acc[q] = cfma(v[u][q], M2L_PHASE(q, o), acc[q]);
-        }
-    }
-    for (; j < j1; ++j) {
-        const int o = idx[j];
-        const size_t sb = (size_t)src[j] * 2 * tot;
-#pragma unroll
-        for (int q = 0; q < kM2lEpt; ++q) acc[q] = cfma(base[sb + e[q]], M2L_PHASE(q, o), acc[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < kM2lEpt; ++q)
-        if (ok[q]) expn[((size_t)ts * 2 + 1) * tot + e[q]] = acc[q];
-}
-#undef M2L_PHASE
-
 // STEP 3, grouped: a block owns one element range (one element per thread)
 // and kM2lGroup consecutive targets (Morton neighbours, so their sources
 // repeat and hit in L2).  Each thread forms the nine products w_x w_y and
@@ -249,12 +175,13 @@ __device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, cons
 }
 #undef M2L_CASE
 
+template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup,
                                                   const long long* __restrict__ toff,
                                                   const int* __restrict__ poff,
                                                   const long long* __restrict__ soff,
                                                   const int* __restrict__ so,
-                                                  double2* __restrict__ expn) {
+                                                  ET* __restrict__ expn) {
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int grp = lin % ngroup, chunk = lin / ngroup;
     const long long tot = (long long)g.nd * g.nexp;
@@ -276,28 +203,29 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask
     }
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
-    const double2* __restrict__ eb = expn + e;
+    const ET* __restrict__ eb = expn + e;
     const int tb = grp * kM2lGroup, te = min(tb + kM2lGroup, ntask);
     for (int task = tb; task < te; ++task) {
         const long long to = toff[task];
-        double2 acc = eb[to];
+        double2 acc = ldE(eb[to]);
         // lists are padded to a multiple of kM2lBatch with a zero expansion
         for (int j = poff[task]; j < poff[task + 1]; j += kM2lBatch) {
             double2 v[kM2lBatch];
 #pragma unroll
-            for (int u = 0; u < kM2lBatch; ++u) v[u] = eb[soff[j + u]];
+            for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[soff[j + u]]);
 #pragma unroll
             for (int u = 0; u < kM2lBatch; ++u) acc = m2l_apply(so[j + u], v[u], acc, fxy, fz);
         }
-        if (ok) expn[to + tot + e] = acc;
+        if (ok) stE(expn + to + tot + e, acc);
     }
 }
 
 // ---------------------------------------------------------------- STEP 4
+template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign, int klev,
-                                                  double2* __restrict__ expn) {
+                                                  ET* __restrict__ expn) {
     const int entry = blockIdx.y;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)g.nd * g.nexp;
@@ -305,8 +233,8 @@ __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
     const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
-    const double2 v = expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e];
-    expn[((size_t)pc_child[entry] * 2 + 1) * tot + e] = cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g));
+    const double2 v = ldE(expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e]);
+    stE(expn + ((size_t)pc_child[entry] * 2 + 1) * tot + e, cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g)));
 }
 
 // ---------------------------------------------------------------- STEP 5
@@ -465,6 +393,7 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, c
         check(hipGetLastError(), "k_p2pw3 launch");
         return;
     }
+    if (t.expn_f32) throw std::runtime_error("FP32 expansions need the 3-D register-blocked kernels");
     const size_t lds = sizeof(double2) * ((size_t)t.npw * t.n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
     hipLaunchKernelGGL(k_p2pw, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, fvals, expn);
     check(hipGetLastError(), "k_p2pw launch");
@@ -474,18 +403,14 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
                   const int* par_first, const int* par_count, const int* pc_child, const int* pc_sign,
                   const int* par_acc, int klev, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
-    hipLaunchKernelGGL(k_merge, dim3(blocks_for((long long)g.nd * g.nexp), ntask), dim3(kBlock), 0, s, g,
-                       t, par_slot, par_first, par_count, pc_child, pc_sign, par_acc, klev, expn);
+    const dim3 grid(blocks_for((long long)g.nd * g.nexp), ntask);
+    if (t.expn_f32)
+        hipLaunchKernelGGL(k_merge<float2>, grid, dim3(kBlock), 0, s, g, t, par_slot, par_first, par_count,
+                           pc_child, pc_sign, par_acc, klev, reinterpret_cast<float2*>(expn));
+    else
+        hipLaunchKernelGGL(k_merge<double2>, grid, dim3(kBlock), 0, s, g, t, par_slot, par_first, par_count,
+                           pc_child, pc_sign, par_acc, klev, expn);
     check(hipGetLastError(), "k_merge launch");
-}
-
-void launch_m2l(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* off,
-                const int* src, const int* idx, double2* expn, hipStream_t s) {
-    if (ntask <= 0) return;
-    const long long nblk = (long long)blocks_for(((long long)g.nd * g.nexp + kM2lEpt - 1) / kM2lEpt) * ntask;
-    if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l: grid too large");
-    hipLaunchKernelGGL(k_m2l, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, tgt, off, src, idx, expn);
-    check(hipGetLastError(), "k_m2l launch");
 }
 
 void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
@@ -494,16 +419,25 @@ void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long*
     const int ngroup = (ntask + kM2lGroup - 1) / kM2lGroup;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
-    hipLaunchKernelGGL(k_m2l27, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff, poff, soff,
-                       so, expn);
+    if (t.expn_f32)
+        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff,
+                           poff, soff, so, reinterpret_cast<float2*>(expn));
+    else
+        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff,
+                           poff, soff, so, expn);
     check(hipGetLastError(), "k_m2l27 launch");
 }
 
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s) {
     if (nentry <= 0) return;
-    hipLaunchKernelGGL(k_split, dim3(blocks_for((long long)g.nd * g.nexp), nentry), dim3(kBlock), 0, s, g,
-                       t, pc_parent, pc_child, pc_sign, klev, expn);
+    const dim3 grid(blocks_for((long long)g.nd * g.nexp), nentry);
+    if (t.expn_f32)
+        hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, pc_parent, pc_child, pc_sign, klev,
+                           reinterpret_cast<float2*>(expn));
+    else
+        hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, pc_parent, pc_child, pc_sign, klev,
+                           expn);
     check(hipGetLastError(), "k_split launch");
 }
 
@@ -514,6 +448,7 @@ void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
         check(hipGetLastError(), "k_pw2pot3 launch");
         return;
     }
+    if (t.expn_f32) throw std::runtime_error("FP32 expansions need the 3-D register-blocked kernels");
     const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
     hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, kd, od);
     check(hipGetLastError(), "k_pw2pot launch");

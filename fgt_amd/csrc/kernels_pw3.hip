@@ -38,12 +38,12 @@ __device__ __forceinline__ void cmfma(double2 a, double2 b, dbl4& re, dbl4& im) 
 
 // ---------------------------------------------------------------- STEP 1
 // LDS: sff [m2][s][m1] (N*4*N complex), sff2 [s][m1][k2] (4*N*npw complex)
-template <int N>
+template <int N, typename ET>
 __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab, int npw,
                                               const int* __restrict__ box, const int* __restrict__ lev,
                                               const int* __restrict__ slot,
                                               const double* __restrict__ fvals,
-                                              double2* __restrict__ expn) {
+                                              ET* __restrict__ expn) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
     double2* sff2 = sff + 4 * N * N;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
             for (int m3 = 0; m3 < N; ++m3)
                 fr[m3] = fvals[((size_t)b * g.npbox + tid + N * N * m3) * nd + ind];
         }
-        double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
+        ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
         for (int r = 0; r < nh; r += 4) {
             // z: ff[s][m1][m2] = sum_m3 T[k3_s][m3] f[m1,m2,m3], thread = (m1,m2)
             if (tid < N * N) {
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
             {
                 const int k3 = r + wave;
                 if (k3 < nh) {
-                    double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     for (int c = 0; c < npw; c += 64) {
                         const int k2 = c + lane;
                         const bool act = k2 < npw;
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
                             double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
                             for (int m1 = 0; m1 < N; ++m1) acc = zfma(T[k1 * N + m1], f2[m1], acc);
-                            if (act) Ek[(size_t)k1 * npw + k2] = acc;
+                            if (act) stE(Ek + (size_t)k1 * npw + k2, acc);
                         }
                     }
                 }
@@ -143,12 +143,12 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // one buffer: sH is fully read into registers before sF is written.
 constexpr int kFusedParts = 4;
 
-template <int N>
+template <int N, typename ET>
 __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS, int npw,
                                                const int* __restrict__ plev, const int* __restrict__ pslot,
                                                const int* __restrict__ kids, const int* __restrict__ ksg,
                                                const double* __restrict__ fvals,
-                                               double2* __restrict__ expn) {
+                                               ET* __restrict__ expn) {
     extern __shared__ double2 sm[];
     double2* sH = sm;                     // 4*2*N*N
     double2* sF = sm;                     // 2*2*N*npw (aliases sH)
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     const int nround = (nh + 1) / 2;
     const int rb = 2 * (part * nround / kFusedParts), re = 2 * ((part + 1) * nround / kFusedParts);
     for (int ind = 0; ind < nd; ++ind) {
-        double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
+        ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
         for (int r = rb; r < re; r += 2) {
             // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
             if (tid < N * N) {
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 const int s = wave & 1, hlf = wave >> 1;
                 const int k3 = r + s;
                 if (k3 < nh) {
-                    double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     const int k1a = hlf ? (npw + 1) / 2 : 0, k1b = hlf ? npw : (npw + 1) / 2;
                     const double2* __restrict__ Txp = T + base0 + 1 * per;
                     const double2* __restrict__ Txm = T + base0 + 2 * per;
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                             for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txp[k1 * N + m1], fp[m1], acc);
 #pragma unroll
                             for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txm[k1 * N + m1], fm[m1], acc);
-                            if (act) Ek[(size_t)k1 * npw + k2] = acc;
+                            if (act) stE(Ek + (size_t)k1 * npw + k2, acc);
                         }
                     }
                 }
@@ -277,12 +277,12 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // operand.  (For STEP 1 the matrix-core form is slower: its 50 x 50 output
 // pads to 64 x 64 tiles, where the VALU form wastes only the lane padding.)
 
-template <int N, bool MF>
+template <int N, bool MF, typename ET>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
                                                 const double2* __restrict__ tabS, int npw,
                                                 const int* __restrict__ box, const int* __restrict__ lev,
                                                 const int* __restrict__ slot, const int* __restrict__ sel,
-                                                const double2* __restrict__ expn, size_t ks, Kinds kd,
+                                                const ET* __restrict__ expn, size_t ks, Kinds kd,
                                                 OutDesc od) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
             for (int k = 0; k < N; ++k) G[q][k] = 0.0;
-        const double2* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
+        const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
         for (int r = 0; r < nh; r += 4) {
             __syncthreads();
             // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2.
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             if constexpr (MF) {
                 const int k3 = r + wave;
                 if (k3 < nh) {
-                    const double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    const ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     const int li = lane & 15, lk = lane >> 4;
                     dbl4 are[4], aim[4];
 #pragma unroll
@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             const int k2 = 16 * t + li;
-                            e[t] = Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)];
+                            e[t] = ldE(Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)]);
                         }
 #pragma unroll
                         for (int t = 0; t < 4; ++t) cmfma(u, e[t], are[t], aim[t]);
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             } else {
                 const int k3 = r + wave;
                 if (k3 < nh) {
-                    const double2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    const ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     for (int c = 0; c < npw; c += 64) {
                         const int k2 = c + lane;
                         const bool act = k2 < npw;
@@ -380,14 +380,14 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                         for (; k1 + PF <= npw; k1 += PF) {
                             double2 e[PF];
 #pragma unroll
-                            for (int u = 0; u < PF; ++u) e[u] = Ek[(size_t)(k1 + u) * npw + k2c];
+                            for (int u = 0; u < PF; ++u) e[u] = ldE(Ek[(size_t)(k1 + u) * npw + k2c]);
 #pragma unroll
                             for (int u = 0; u < PF; ++u)
 #pragma unroll
                                 for (int k = 0; k < N; ++k) acc[k] = zfma(UX[(k1 + u) * N + k], e[u], acc[k]);
                         }
                         for (; k1 < npw; ++k1) {
-                            const double2 e = Ek[(size_t)k1 * npw + k2c];
+                            const double2 e = ldE(Ek[(size_t)k1 * npw + k2c]);
 #pragma unroll
                             for (int k = 0; k < N; ++k) acc[k] = zfma(UX[k1 * N + k], e, acc[k]);
                         }
@@ -451,9 +451,25 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
                const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
     const size_t lds = sizeof(double2) * (4 * (size_t)N * N + 4 * (size_t)N * t.npw);
     if (lds > lds_limit) return false;
-    hipLaunchKernelGGL(k_p2pw3<N>, dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot, fvals,
-                       expn);
+    if (t.expn_f32)
+        hipLaunchKernelGGL((k_p2pw3<N, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
+                           fvals, reinterpret_cast<float2*>(expn));
+    else
+        hipLaunchKernelGGL((k_p2pw3<N, double2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
+                           fvals, expn);
     return true;
+}
+
+template <int N, bool MF>
+void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                          const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s,
+                          size_t lds) {
+    if (t.expn_f32)
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS,
+                           t.npw, box, lev, slot, sel, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+    else
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS,
+                           t.npw, box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
 }
 
 template <int N>
@@ -464,12 +480,22 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, c
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
     if (N == 16 && t.npw <= 64 && !pw2pot_valu_only())
-        hipLaunchKernelGGL((k_pw2pot3<N, true>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw,
-                           box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
+        launch_pw2pot3_typed<N, true>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds);
     else
-        hipLaunchKernelGGL((k_pw2pot3<N, false>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS, t.npw,
-                           box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
+        launch_pw2pot3_typed<N, false>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds);
     return true;
+}
+
+template <int N>
+void launch_p2pwF3_typed(const Geom& g, const DevTables& t, unsigned nblk, size_t lds, const int* plev,
+                         const int* pslot, const int* kids, const int* ksg, const double* fvals, double2* expn,
+                         hipStream_t s) {
+    if (t.expn_f32)
+        hipLaunchKernelGGL((k_p2pwF3<N, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+                           kids, ksg, fvals, reinterpret_cast<float2*>(expn));
+    else
+        hipLaunchKernelGGL((k_p2pwF3<N, double2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+                           kids, ksg, fvals, expn);
 }
 
 }  // namespace
@@ -504,20 +530,10 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
     if ((long long)ntask * kFusedParts > 0x7fffffffLL) return false;
     const unsigned nblk = (unsigned)(ntask * kFusedParts);
     switch (N) {
-        case 8:
-            hipLaunchKernelGGL(k_p2pwF3<8>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot, kids,
-                               ksg, fvals, expn);
-            return true;
-        case 12:
-            hipLaunchKernelGGL(k_p2pwF3<12>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
-                               kids, ksg, fvals, expn);
-            return true;
-        case 16:
-            hipLaunchKernelGGL(k_p2pwF3<16>, dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
-                               kids, ksg, fvals, expn);
-            return true;
-        default:
-            return false;
+        case 8: launch_p2pwF3_typed<8>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
+        case 12: launch_p2pwF3_typed<12>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
+        case 16: launch_p2pwF3_typed<16>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
+        default: return false;
     }
 }
 

@@ -98,6 +98,13 @@ int fgt_solver_get_pot(fgt_solver *s, double *pot, int accumulate);
  * arrays (Hessian order 1-D xx; 2-D xx,xy,yy; 3-D xx,yy,zz,xy,xz,yz, as
  * src/bfgt/bfgt_pwrouts.f:612-1065).  Default 1. */
 int fgt_solver_set_ifpgh(fgt_solver *s, int ifpgh);
+/* Expansion storage: mode 0 FP64 (default), 1 mixed precision -- the
+ * plane-wave expansions are stored as FP32 complex while tables, phases,
+ * arithmetic and outputs stay FP64 (for eps >= 1e-6; error against the
+ * FP64 result stays below eps).  Mode 1 needs the 3-D register-blocked
+ * path (ndim 3, norder 8/12/16); otherwise status 2.  No counterpart in the
+ * reference (FP64 only). */
+int fgt_solver_set_precision(fgt_solver *s, int mode);
 /* device gradient / Hessian -> host (accumulate != 0 adds, reference +=);
  * status 2 when not requested by fgt_solver_set_ifpgh. */
 int fgt_solver_get_grad(fgt_solver *s, double *grad, int accumulate);
@@ -133,6 +140,10 @@ typedef struct fgt_batch fgt_batch;
  * (1..16) HIP streams; executes are spread over them, largest first. */
 int fgt_batch_create(int nprob, const fgt_problem *probs, int nstreams,
                      fgt_batch **out);
+/* mode 1: mixed precision for every problem that supports it and has
+ * eps >= eps_min (the others stay FP64); *nmixed (nullable) = their count. */
+int fgt_batch_set_precision(fgt_batch *b, int mode, double eps_min,
+                            int *nmixed);
 int fgt_batch_set_fvals(fgt_batch *b, int i, const double *fvals);
 /* Runs STEPs 1-6 of every problem, overlapped, and waits for all. */
 int fgt_batch_execute(fgt_batch *b);
