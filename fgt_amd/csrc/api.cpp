@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/fgt.h"
@@ -75,7 +76,8 @@ struct fgt_solver {
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
-    DBuf d_s5_src, d_s5_sel;
+    DBuf d_s5_tasks;   // fgt::PotTask list of the 3-D STEP 5 kernel
+    int n_s5 = 0;
     DBuf d_p2pwS, d_fz_lev, d_fz_slot, d_fz_kids, d_fz_ksg;          // fused STEP 1+2 parents
     DBuf d_s1_box, d_s1_lev, d_s1_slot;                              // standalone STEP 1 leaves
     DBuf d_mp_slot, d_mp_first, d_mp_count, d_mp_child, d_mp_sign, d_mp_acc;  // merges of non-leaf children
@@ -227,23 +229,56 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         S.d_pw2potS.upload(to_d2(ts));
         t.pw2potS_ks = sks;
         const fgt::TreeView& T = P.tree;
-        std::vector<int> src(P.leaf_box.size()), sel(P.leaf_box.size());
+        // STEP 5 tasks: leaves below the first PW level evaluate their
+        // parent's expansion through the split-phased tables; siblings with
+        // the same x split sign form one task (shared x contraction).  Those
+        // tasks come first (more work per task), then one task per leaf
+        // evaluating its own expansion.
+        std::vector<fgt::PotTask> fam, own;
+        std::unordered_map<long long, int> fam_of;
         for (size_t i = 0; i < P.leaf_box.size(); ++i) {
             const int b = P.leaf_box[i], l = P.leaf_lev[i];
             if (l > P.nlevstart) {
                 const int par = T.parent[b];
-                int sg = 0;
-                for (int a = 0; a < 3; ++a)
-                    if (T.centers[3 * b + a] - T.centers[3 * par + a] < 0) sg |= 1 << a;
-                src[i] = P.slot_of_box[par];
-                sel[i] = 1 | (sg << 1);
+                int sg[3];
+                for (int a = 0; a < 3; ++a) sg[a] = T.centers[3 * b + a] - T.centers[3 * par + a] < 0 ? 1 : 0;
+                const long long key = 2LL * par + sg[0];
+                auto it = fam_of.find(key);
+                if (it == fam_of.end() || fam[it->second].box[3] >= 0) {
+                    fgt::PotTask pt{};
+                    pt.slot = P.slot_of_box[par];
+                    pt.lev = l;
+                    pt.vx = 1 + sg[0];
+                    pt.nvy = 0;
+                    for (int c = 0; c < 4; ++c) pt.box[c] = -1, pt.code[c] = 0;
+                    fam_of[key] = (int)fam.size();
+                    fam.push_back(pt);
+                    it = fam_of.find(key);
+                }
+                fgt::PotTask& pt = fam[it->second];
+                const int vy = 1 + sg[1];
+                int j = 0;
+                while (j < pt.nvy && pt.vy[j] != vy) ++j;
+                if (j == pt.nvy) pt.vy[pt.nvy++] = vy;
+                int c = 0;
+                while (pt.box[c] >= 0) ++c;
+                pt.box[c] = b;
+                pt.code[c] = j | ((1 + sg[2]) << 1);
             } else {
-                src[i] = P.leaf_slot[i];
-                sel[i] = 0;
+                fgt::PotTask pt{};
+                pt.slot = P.leaf_slot[i];
+                pt.lev = l;
+                pt.vx = 0;
+                pt.nvy = 1;
+                pt.vy[0] = pt.vy[1] = 0;
+                for (int c = 0; c < 4; ++c) pt.box[c] = -1, pt.code[c] = 0;
+                pt.box[0] = b;
+                own.push_back(pt);
             }
         }
-        S.d_s5_src.upload(src);
-        S.d_s5_sel.upload(sel);
+        fam.insert(fam.end(), own.begin(), own.end());
+        S.n_s5 = (int)fam.size();
+        S.d_s5_tasks.upload(fam);
         // split entries, keeping non-leaf children only
         std::vector<int> pp, pcc, psg;
         S.pcs_level_off.assign(nl + 2, 0);
@@ -554,12 +589,11 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         // STEP 5, once per output component (pot, grad, hess entries)
         for (const Comp& c : comps) {
             if (S.fast3d)
-                fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                                   S.d_s5_src.as<int>(), S.d_s5_sel.as<int>(), S.d_expn.as<double2>(), c.kd,
-                                   c.od, st);
+                fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.d_s5_tasks.as<fgt::PotTask>(), S.d_expn.as<double2>(),
+                                         c.kd, c.od, st);
             else
                 fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                                   S.d_leaf_slot.as<int>(), nullptr, S.d_expn.as<double2>(), c.kd, c.od, st);
+                                   S.d_leaf_slot.as<int>(), S.d_expn.as<double2>(), c.kd, c.od, st);
         }
         mark(5);
     } else {

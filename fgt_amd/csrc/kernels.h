@@ -56,6 +56,21 @@ struct Kinds {
     int x, y, z;
 };
 
+// One task of the 3-D STEP 5 kernel: the local expansion in `slot`,
+// evaluated at up to four leaves of level `lev` that share the x table
+// variant vx (0 plain table; 1 / 2 table times the + / conjugate
+// parent->child split phase).  Leaf c: box[c] (-1 = unused), y variant
+// vy[code[c] & 1], z variant code[c] >> 1.  A leaf evaluating its own
+// expansion is a task with one leaf and all variants 0; the leaves of one
+// parent with the same x split sign form one task, so the x contraction of
+// the parent's expansion is done once for all of them.
+struct PotTask {
+    int slot, lev, vx, nvy;
+    int vy[2];
+    int box[4];
+    int code[4];
+};
+
 // Device-resident tables (all pointers are device pointers).
 struct DevTables {
     // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)
@@ -92,10 +107,8 @@ struct DevTables {
 // false (nothing launched) when the shape is not covered.
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                     const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
-// sel: per task 0 (own loc) or 1 | split-sign bits << 1 (evaluate the parent's loc)
-bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od,
-                      hipStream_t s, size_t lds_limit);
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit);
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 // true when FGT_PW2POT_VALU=1: STEP 5 keeps its x stage on the VALU
 bool pw2pot_valu_only();
@@ -124,8 +137,11 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
-                   const int* lev, const int* slot, const int* sel, const double2* expn,
+                   const int* lev, const int* slot, const double2* expn,
                    Kinds kd, OutDesc od, hipStream_t s);
+// 3-D STEP 5 over PotTask lists (throws when the shape is not covered)
+void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s);
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,

@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace fgt {
@@ -153,6 +154,15 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 // on it picks the phase factors from fixed registers (scalar branches, no
 // per-lane selects), and four source loads are issued ahead of their use.
 constexpr int kM2lGroup = 8;
+// targets per block; FGT_M2L_GROUP overrides (for tuning runs)
+int m2l_group() {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_M2L_GROUP");
+        const int g = e ? std::atoi(e) : 0;
+        return g > 0 && g <= 64 ? g : kM2lGroup;
+    }();
+    return v;
+}
 constexpr int kM2lBatch = 2;   // source loads issued together (list padding unit)
 
 // (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
@@ -176,7 +186,7 @@ __device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, cons
 #undef M2L_CASE
 
 template <typename ET>
-__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup,
+__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup, int group,
                                                   const long long* __restrict__ toff,
                                                   const int* __restrict__ poff,
                                                   const long long* __restrict__ soff,
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
     const ET* __restrict__ eb = expn + e;
-    const int tb = grp * kM2lGroup, te = min(tb + kM2lGroup, ntask);
+    const int tb = grp * group, te = min(tb + group, ntask);
     for (int task = tb; task < te; ++task) {
         const long long to = toff[task];
         double2 acc = ldE(eb[to]);
@@ -416,15 +426,16 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
                   const long long* soff, const int* so, double2* expn, hipStream_t s) {
     if (ntask <= 0) return;
-    const int ngroup = (ntask + kM2lGroup - 1) / kM2lGroup;
+    const int group = m2l_group();
+    const int ngroup = (ntask + group - 1) / group;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
     if (t.expn_f32)
-        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff,
-                           poff, soff, so, reinterpret_cast<float2*>(expn));
+        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, group,
+                           toff, poff, soff, so, reinterpret_cast<float2*>(expn));
     else
-        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, toff,
-                           poff, soff, so, expn);
+        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, group,
+                           toff, poff, soff, so, expn);
     check(hipGetLastError(), "k_m2l27 launch");
 }
 
@@ -441,13 +452,17 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
     check(hipGetLastError(), "k_split launch");
 }
 
-void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                   const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
+void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;
-    if (launch_pw2pot_3d(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, t.lds_limit)) {
-        check(hipGetLastError(), "k_pw2pot3 launch");
-        return;
-    }
+    if (!launch_pw2pot_3d(g, t, ntask, tasks, expn, kd, od, s, t.lds_limit))
+        throw std::runtime_error("3-D STEP 5 kernel not available for this shape");
+    check(hipGetLastError(), "k_pw2pot3 launch");
+}
+
+void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
+                   const int* slot, const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
+    if (ntask <= 0) return;
     if (t.expn_f32) throw std::runtime_error("FP32 expansions need the 3-D register-blocked kernels");
     const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
     hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, kd, od);

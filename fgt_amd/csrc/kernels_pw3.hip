@@ -264,12 +264,20 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 }
 
 // ---------------------------------------------------------------- STEP 5
-// tabT[l][k][i] = pw2pot table transposed (frequency k, node i).
-// LDS: sff [k2][s][16] (npw*64 complex), sU [i][k] (N*npw complex).
-// sel per task: 0 = evaluate the leaf's own local expansion with the plain
-// tables; otherwise bit0 set and bits 1..3 = split signs (x,y,z): evaluate the
-// parent's local expansion with the tables pre-multiplied by the parent->child
-// phase (tabS[l][sign][k][i]) -- STEP 4 folded into STEP 5 for leaves.
+// Tables: tabS[kind][l][v][k][i] (frequency k, node i), v = 0 the plain
+// PW->value table, 1 / 2 the table times the + / conjugate parent->child
+// split phase.  A task (PotTask) evaluates one expansion at up to four
+// leaves: for a leaf evaluating its parent's expansion the phase is folded
+// into the tables (STEP 4 folded into STEP 5), and the x variant depends only
+// on the x split sign, so four siblings share one x contraction.
+// Per round of four k3 slabs:
+//   x: ff[k2][s][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2]   (wave = slab)
+//   y: yres[s][kk1][kk2] = sum_k2 Uy[kk2][k2] ff[k2][s][kk1], per y variant
+//   z: G_c[kk3] += Re(Uz_c[kk3][k3] yres[s][zk1][zk2]), per leaf c, with one
+//      thread per output column (zk1, zk2) holding all kk3 in registers.
+// LDS: sff [k2][s][16] (npw*64 complex), yres [s][kk1][16] (1024 complex,
+// entry kk2 at kk2 ^ kk1), sUr [v][s][kk3] (12*N complex: the round's z
+// table rows of the three variants, zero past nh).
 // MF: x stage on the FP64 matrix cores (N = 16, npw <= 64): per slab the
 // complex product ff[kk1][k2] = sum_k1 U[kk1][k1] E[k1][k2] runs as
 // 16x16x4 tiles (rows kk1, columns 16 k2, K = 4 k1).  The table is read per
@@ -278,58 +286,57 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // pads to 64 x 64 tiles, where the VALU form wastes only the lane padding.)
 
 template <int N, bool MF, typename ET>
-__global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabT,
-                                                const double2* __restrict__ tabS, int npw,
-                                                const int* __restrict__ box, const int* __restrict__ lev,
-                                                const int* __restrict__ slot, const int* __restrict__ sel,
-                                                const ET* __restrict__ expn, size_t ks, Kinds kd,
-                                                OutDesc od) {
+__global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS, int npw,
+                                                const PotTask* __restrict__ tasks, const ET* __restrict__ expn,
+                                                size_t ks, Kinds kd, OutDesc od) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
-    const size_t sff_len = max((size_t)npw * 64, (size_t)(4 * 4 * 16 * N) / 2);
-    double2* sU = sff + sff_len;
+    double2* yres = sff + (size_t)npw * 64;
+    double2* sUr = yres + 4 * 16 * 16;
     constexpr int NQ = N / 4;
     constexpr int PF = (N == 12) ? 6 : 8;
-    const int task = xcd_task(blockIdx.x, gridDim.x);
-    const int b = box[task], l = lev[task], sl = slot[task], se = sel[task];
-    // UT[k][i] = U[i][k] for each axis: tabS[l][v] holds v = 0 the plain
-    // table, v = 1 / 2 the table times the + / conjugate split phase.  One base
-    // pointer plus wave-uniform offsets keeps the reads on the scalar path.
+    const PotTask* __restrict__ T = tasks + xcd_task(blockIdx.x, gridDim.x);
+    const int sl = T->slot, l = T->lev, nvy = T->nvy;
+    int cbox[4], ccode[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        cbox[c] = T->box[c];
+        ccode[c] = T->code[c];
+    }
+    // table kind per axis (value / first / second derivative) at stride ks;
+    // offsets are wave-uniform so the table reads stay on the scalar path
     const int per = npw * N;
-    const int vx = (se & 1) ? 1 + ((se >> 1) & 1) : 0;
-    const int vy = (se & 1) ? 1 + ((se >> 2) & 1) : 0;
-    const int vz = (se & 1) ? 1 + ((se >> 3) & 1) : 0;
-    // table kind per axis (value / first / second derivative): stride ks
-    const int ox = __builtin_amdgcn_readfirstlane((int)(kd.x * ks) + (3 * l + vx) * per);
-    const int oy = __builtin_amdgcn_readfirstlane((int)(kd.y * ks) + (3 * l + vy) * per);
-    const int oz = __builtin_amdgcn_readfirstlane((int)(kd.z * ks) + (3 * l + vz) * per);
+    const int ox = __builtin_amdgcn_readfirstlane((int)(kd.x * ks) + (3 * l + T->vx) * per);
+    const int oz = __builtin_amdgcn_readfirstlane((int)(kd.z * ks) + 3 * l * per);
     const double2* __restrict__ UX = tabS + ox;
-    const double2* __restrict__ UY = tabS + oy;
     const double2* __restrict__ UZ = tabS + oz;
-    (void)tabT;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int nh = g.pz, nd = g.nd;
-    for (int i = tid; i < N * npw; i += blockDim.x) {
-        const int k = i / N, ii = i % N;
-        sU[ii * npw + k] = UZ[i];
-    }
     const int kk1 = lane & 15, s = lane >> 4;
     const bool halve = (npw & 1) != 0;
+    // z stage: one thread per output column (zk1, zk2), all kk3 in registers
+    const bool zact = tid < N * N;
+    const int zk1 = tid % N, zk2 = tid / N;
     for (int ind = 0; ind < nd; ++ind) {
-        double G[NQ][N];
+        double G[4][N];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+        for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int k = 0; k < N; ++k) G[q][k] = 0.0;
+            for (int k = 0; k < N; ++k) G[c][k] = 0.0;
         const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
         for (int r = 0; r < nh; r += 4) {
             __syncthreads();
-            // x: ff[k2][s][kk1] = sum_k1 U[kk1][k1] E[k3][k1][k2]; wave = slab, lane = k2.
-            // Rows of E are fetched PF at a time (all loads issued before the
-            // FMAs that use them) so enough reads are in flight to cover HBM
-            // latency; lanes past npw read a valid duplicate and discard it.
+            // the round's z table rows, all three variants (zero past nh)
+            if (tid < 12 * N) {
+                const int v = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
+                sUr[tid] = (r + s2 < nh) ? UZ[v * per + (r + s2) * N + k] : make_double2(0.0, 0.0);
+            }
+            // x: wave = slab, lane = k2.  Rows of E are fetched PF at a time
+            // (all loads issued before the FMAs that use them) so enough reads
+            // are in flight to cover HBM latency; lanes past npw read a valid
+            // duplicate and discard it.
             if constexpr (MF) {
                 const int k3 = r + wave;
                 if (k3 < nh) {
@@ -400,49 +407,63 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                 }
             }
             __syncthreads();
-            // y: ff2[s][kk1][kk2] = sum_k2 U[kk2][k2] ff[k2][s][kk1]; lane = (kk1, s),
-            // wave owns kk2 in [wave*NQ, wave*NQ+NQ); then z: G += Re(U[kk3][k3] ff2)
-            const int k3 = r + s;
-            if (kk1 < N && k3 < nh) {
-                double2 y[NQ];
+            for (int j = 0; j < nvy; ++j) {
+                if (j) __syncthreads();   // the z stage of the previous y variant is done with yres
+                // y: lane = (kk1, s), wave owns kk2 in [wave*NQ, wave*NQ+NQ)
+                const int oy = __builtin_amdgcn_readfirstlane((int)(kd.y * ks) + (3 * l + T->vy[j]) * per);
+                const double2* __restrict__ UY = tabS + oy;
+                const int k3 = r + s;
+                if (kk1 < N && k3 < nh) {
+                    double2 y[NQ];
 #pragma unroll
-                for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
-                for (int k2 = 0; k2 < npw; ++k2) {
-                    const double2 f = sff[(k2 * 4 + s) * 16 + (kk1 ^ (k2 & 15))];
+                    for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
+                    for (int k2 = 0; k2 < npw; ++k2) {
+                        const double2 f = sff[(k2 * 4 + s) * 16 + (kk1 ^ (k2 & 15))];
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
+                        for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) yres[(s * 16 + kk1) * 16 + ((wave * NQ + q) ^ kk1)] = y[q];
                 }
-                const double w = (halve && k3 == nh - 1) ? 0.5 : 1.0;
+                __syncthreads();
+                // z, for the leaves that use this y variant
+                if (zact) {
+#pragma unroll 1
+                    for (int s2 = 0; s2 < 4; ++s2) {
+                        const int k3z = r + s2;
+                        if (k3z >= nh) break;
+                        const double2 yv = yres[(s2 * 16 + zk1) * 16 + (zk2 ^ zk1)];
+                        const double w = (halve && k3z == nh - 1) ? 0.5 : 1.0;
+                        const double yr = yv.x * w, yi = yv.y * w;
 #pragma unroll
-                for (int kk3 = 0; kk3 < N; ++kk3) {
-                    const double2 u = sU[kk3 * npw + k3];
+                        for (int c = 0; c < 4; ++c) {
+                            if (cbox[c] >= 0 && (ccode[c] & 1) == j) {
+                                const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * 4 + s2) * N;
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) G[q][kk3] += (u.x * y[q].x - u.y * y[q].y) * w;
+                                for (int kk3 = 0; kk3 < N; ++kk3) {
+                                    const double2 u = uz[kk3];
+                                    G[c][kk3] += u.x * yr - u.y * yi;
+                                }
+                            }
+                        }
+                    }
                 }
             }
         }
-        // reduce the four slab groups through LDS, one kk2 column at a time
-        // (keeps the live register set small) and accumulate into pot
-        double* red = reinterpret_cast<double*>(sm);   // [wave][s][kk3][16]
+        if (zact) {
+            // the empty asm makes col opaque here, so the 64 output addresses
+            // are formed now and not hoisted out of the loops (where they
+            // would be held, or spilled, through the whole task)
+            int col = zk1 + N * zk2;
+            asm volatile("" : "+v"(col));
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            __syncthreads();
-            if (kk1 < N)
+            for (int c = 0; c < 4; ++c)
+                if (cbox[c] >= 0) {
+                    double* __restrict__ o = od.p + (size_t)cbox[c] * g.npbox * od.ns + od.coff + ind;
 #pragma unroll
-                for (int k = 0; k < N; ++k) red[((wave * 4 + s) * N + k) * 16 + kk1] = G[q][k];
-            __syncthreads();
-            for (int o = tid; o < 4 * 16 * N; o += blockDim.x) {
-                const int k1o = o & 15, k = (o >> 4) % N, w2 = o / (16 * N);
-                if (k1o < N) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int s2 = 0; s2 < 4; ++s2) v += red[((w2 * 4 + s2) * N + k) * 16 + k1o];
-                    const int kk2 = w2 * NQ + q;
-                    od.p[((size_t)b * g.npbox + k1o + N * (kk2 + N * k)) * od.ns + od.coff + ind] += 2.0 * v;
+                    for (int kk3 = 0; kk3 < N; ++kk3) o[(size_t)(col + N * N * kk3) * od.ns] += 2.0 * G[c][kk3];
                 }
-            }
         }
-        __syncthreads();
     }
 }
 
@@ -460,29 +481,31 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
     return true;
 }
 
+size_t pw2pot3_lds(int n, int npw) {
+    return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + 12 * (size_t)n);
+}
+
 template <int N, bool MF>
-void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                          const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s,
-                          size_t lds) {
+void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
+                          Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
     if (t.expn_f32)
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS,
-                           t.npw, box, lev, slot, sel, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw, tasks,
+                           reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
     else
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potT, t.pw2potS,
-                           t.npw, box, lev, slot, sel, expn, t.pw2potS_ks, kd, od);
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+                           tasks, expn, t.pw2potS_ks, kd, od);
 }
 
 template <int N>
-bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
-                 const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
-    const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * N);
-    const size_t lds = sff + sizeof(double2) * (size_t)N * t.npw;
+bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
+                 Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+    const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
     if (N == 16 && t.npw <= 64 && !pw2pot_valu_only())
-        launch_pw2pot3_typed<N, true>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds);
+        launch_pw2pot3_typed<N, true>(g, t, ntask, tasks, expn, kd, od, s, lds);
     else
-        launch_pw2pot3_typed<N, false>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds);
+        launch_pw2pot3_typed<N, false>(g, t, ntask, tasks, expn, kd, od, s, lds);
     return true;
 }
 
@@ -539,21 +562,19 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
 
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     if (g.ndim != 3 || (t.n != 8 && t.n != 12 && t.n != 16)) return false;
-    const size_t sff = std::max(sizeof(double2) * (size_t)t.npw * 64, sizeof(double) * 4 * 4 * 16 * t.n);
-    const size_t lds = sff + sizeof(double2) * (size_t)t.n * t.npw;
+    const size_t lds = pw2pot3_lds(t.n, t.npw);
     const size_t lds1 = sizeof(double2) * (4 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
     const size_t lds2 = sizeof(double2) * std::max(8 * (size_t)t.n * t.n, 4 * (size_t)t.n * t.npw);
     return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
 
-bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                      const int* slot, const int* sel, const double2* expn, Kinds kd, OutDesc od, hipStream_t s,
-                      size_t lds_limit) {
-    if (g.ndim != 3 || ntask <= 0 || !sel) return false;
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
+                      Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+    if (g.ndim != 3 || ntask <= 0) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
-        case 12: return try_pw2pot3<12>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
-        case 16: return try_pw2pot3<16>(g, t, ntask, box, lev, slot, sel, expn, kd, od, s, lds_limit);
+        case 8: return try_pw2pot3<8>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
+        case 12: return try_pw2pot3<12>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
+        case 16: return try_pw2pot3<16>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
         default: return false;
     }
 }
