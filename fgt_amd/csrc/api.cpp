@@ -72,7 +72,8 @@ struct fgt_solver {
     DBuf d_leaf_box, d_leaf_lev, d_leaf_slot;
     DBuf d_par_slot, d_par_first, d_par_count, d_pc_parent, d_pc_child, d_pc_sign, d_pc_sign_split;
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
-    DBuf d_m2l_toff, d_m2l_poff, d_m2l_soff, d_m2l_so;   // grouped STEP 3 lists (padded)
+    DBuf d_m2l_toff, d_m2l_poff, d_m2l_soff, d_m2l_so;   // STEP 3 entry streams per group
+    int n_m2l_groups = 0;
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
@@ -421,25 +422,39 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     for (size_t i = 0; i < mi.size(); ++i) mi[i] = map_m2l(ndim, P.m2l_idx[i]);
     S.d_m2l_idx.upload(mi);
     if (P.do_pw) {
+        // STEP 3 entry streams: kM2lGroup consecutive targets per group; per
+        // target its own expansion (unit phase, o = 13), then its list, the
+        // final entry flagged; each group padded with zero-slot entries
         const long long se = 2LL * P.nd * P.nexp;
-        std::vector<long long> toff(P.m2l_tgt.size()), soff;
-        std::vector<int> poff(P.m2l_tgt.size() + 1, 0), so;
-        for (size_t tk = 0; tk < P.m2l_tgt.size(); ++tk) {
-            toff[tk] = se * P.m2l_tgt[tk];
-            for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
-                soff.push_back(se * P.m2l_src[j]);
-                so.push_back(mi[j]);
+        const int nt = (int)P.m2l_tgt.size();
+        const int ng = (nt + fgt::kM2lGroup - 1) / fgt::kM2lGroup;
+        std::vector<long long> toff((size_t)ng * fgt::kM2lGroup, 0), soff;
+        std::vector<int> gent(ng + 1, 0), code;
+        for (int gi = 0; gi < ng; ++gi) {
+            gent[gi] = (int)soff.size();
+            for (int tk = gi * fgt::kM2lGroup; tk < std::min(nt, (gi + 1) * fgt::kM2lGroup); ++tk) {
+                toff[tk] = se * P.m2l_tgt[tk];
+                soff.push_back(toff[tk]);
+                code.push_back(13);
+                for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
+                    soff.push_back(se * P.m2l_src[j]);
+                    code.push_back(mi[j]);
+                }
+                code.back() |= 1 << 5;
             }
-            while (soff.size() % 2) {
+            while ((soff.size() - gent[gi]) % fgt::kM2lBatch) {
                 soff.push_back(se * P.nslots);   // the zero slot
-                so.push_back(13);                // centre offset (unit phase)
+                code.push_back(13);
             }
-            poff[tk + 1] = (int)soff.size();
+            if ((int)soff.size() - gent[gi] > fgt::kM2lMaxEntries)
+                throw std::runtime_error("STEP 3 group entry stream longer than kM2lMaxEntries");
         }
+        gent[ng] = (int)soff.size();
+        S.n_m2l_groups = ng;
         S.d_m2l_toff.upload(toff);
-        S.d_m2l_poff.upload(poff);
+        S.d_m2l_poff.upload(gent);
         S.d_m2l_soff.upload(soff);
-        S.d_m2l_so.upload(so);
+        S.d_m2l_so.upload(code);
     }
 
     // near-field targets, longest list first (one block per target: the
@@ -569,8 +584,9 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
             mark(2);
         }
         // STEP 3
-        fgt::launch_m2l27(g, t, (int)P.m2l_tgt.size(), S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(),
-                          S.d_m2l_soff.as<long long>(), S.d_m2l_so.as<int>(), S.d_expn.as<double2>(), st);
+        fgt::launch_m2l27(g, t, S.n_m2l_groups, 2LL * P.nd * P.nexp * P.nslots, S.d_m2l_toff.as<long long>(),
+                          S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(), S.d_m2l_so.as<int>(),
+                          S.d_expn.as<double2>(), st);
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)

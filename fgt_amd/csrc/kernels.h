@@ -126,13 +126,20 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child,
                   const int* pc_sign, const int* par_acc, int klev, double2* expn, hipStream_t s);
-// STEP 3 with targets grouped per block.  toff[task]: element offset of the
-// target slot (slot * 2 * nd * nexp); poff: CSR over list entries, each
-// target's list padded to a multiple of 2; soff[j]: element offset of the
-// source slot (padding points at an all-zero slot); so[j]: offset index
-// o = ox + 3 oy + 9 oz in 0..26.
-void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
-                  const long long* soff, const int* so, double2* expn, hipStream_t s);
+// STEP 3, kM2lGroup targets per block.  toff[i]: element offset of target
+// i's slot (slot * 2 * nd * nexp); gent: per group, the range of its entry
+// stream (each target's own expansion, then its list; padded to a multiple
+// of kM2lBatch); soff[j]: element offset of the entry's source slot (padding
+// points at an all-zero slot, whose element offset is zero_off); scode[j] =
+// o | last << 5 with o = ox + 3 oy + 9 oz in 0..26 and last set on each
+// target's final entry.
+constexpr int kM2lGroup = 8;
+constexpr int kM2lBatch = 4;
+// longest entry stream of a group: kM2lGroup targets x (own + 26 list
+// entries), padded
+constexpr int kM2lMaxEntries = kM2lGroup * 27 + kM2lBatch;
+void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, long long zero_off, const long long* toff,
+                  const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s);
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);

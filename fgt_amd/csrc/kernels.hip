@@ -13,7 +13,6 @@
 #include "kernels.h"
 
 #include <cstdio>
-#include <cstdlib>
 #include <stdexcept>
 
 namespace fgt {
@@ -150,21 +149,13 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 // and kM2lGroup consecutive targets (Morton neighbours, so their sources
 // repeat and hit in L2).  Each thread forms the nine products w_x w_y and
 // three w_z of its element once and reuses them for every list entry of the
-// group.  The entry's offset o = ox + 3 oy + 9 oz is wave-uniform; a switch
-// on it picks the phase factors from fixed registers (scalar branches, no
-// per-lane selects), and four source loads are issued ahead of their use.
-constexpr int kM2lGroup = 8;
-// targets per block; FGT_M2L_GROUP overrides (for tuning runs)
-int m2l_group() {
-    static const int v = [] {
-        const char* e = std::getenv("FGT_M2L_GROUP");
-        const int g = e ? std::atoi(e) : 0;
-        return g > 0 && g <= 64 ? g : kM2lGroup;
-    }();
-    return v;
-}
-constexpr int kM2lBatch = 2;   // source loads issued together (list padding unit)
-
+// group.  The group's entries (each target's own expansion first, then its
+// list) form one stream; source loads go kM2lBatch at a time into two
+// register batches, the next batch issued before the current one is
+// applied, so a wave keeps loads in flight across its whole stream.  The
+// entry's offset o = ox + 3 oy + 9 oz is wave-uniform; a switch on it picks
+// the phase factors from fixed registers (scalar branches, no per-lane
+// selects).
 // (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
 // so the kernel stays small in registers and more waves are resident
 #define M2L_CASE(o)                                                     \
@@ -186,11 +177,11 @@ __device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, cons
 #undef M2L_CASE
 
 template <typename ET>
-__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask, int ngroup, int group,
+__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, long long zero_off,
                                                   const long long* __restrict__ toff,
-                                                  const int* __restrict__ poff,
+                                                  const int* __restrict__ gent,
                                                   const long long* __restrict__ soff,
-                                                  const int* __restrict__ so,
+                                                  const int* __restrict__ scode,
                                                   ET* __restrict__ expn) {
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int grp = lin % ngroup, chunk = lin / ngroup;
@@ -214,20 +205,48 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ntask
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
     const ET* __restrict__ eb = expn + e;
-    const int tb = grp * group, te = min(tb + group, ntask);
-    for (int task = tb; task < te; ++task) {
-        const long long to = toff[task];
-        double2 acc = ldE(eb[to]);
-        // lists are padded to a multiple of kM2lBatch with a zero expansion
-        for (int j = poff[task]; j < poff[task + 1]; j += kM2lBatch) {
-            double2 v[kM2lBatch];
-#pragma unroll
-            for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[soff[j + u]]);
-#pragma unroll
-            for (int u = 0; u < kM2lBatch; ++u) acc = m2l_apply(so[j + u], v[u], acc, fxy, fz);
-        }
-        if (ok) stE(expn + to + tot + e, acc);
+    // the group's entry stream, staged in LDS: read back at a wave-uniform
+    // address (broadcast), it keeps the vector-memory counter for the source
+    // loads alone
+    __shared__ long long s_soff[kM2lMaxEntries + kM2lBatch];
+    __shared__ int s_code[kM2lMaxEntries];
+    const int g0 = gent[grp], n = gent[grp + 1] - g0;   // a positive multiple of kM2lBatch
+    for (int i = threadIdx.x; i < n + kM2lBatch; i += blockDim.x) {
+        s_soff[i] = i < n ? soff[g0 + i] : zero_off;   // one batch of read-ahead past the end
+        if (i < n) s_code[i] = scode[g0 + i];
     }
+    __syncthreads();
+    const int j0 = 0, j1 = n;
+    const long long* __restrict__ to = toff + (size_t)grp * kM2lGroup;
+    int ti = 0;
+    double2 acc = make_double2(0.0, 0.0);
+    double2 va[kM2lBatch], vb[kM2lBatch];
+#define M2L_LOAD(v, j)                                                                   \
+    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[s_soff[(j) + u]]);
+    // code = o | last-entry-of-its-target << 5
+#define M2L_APPLY(v, j)                                                                  \
+    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
+        const int code = s_code[(j) + u];                                                \
+        acc = m2l_apply(code & 31, v[u], acc, fxy, fz);                                  \
+        if (code >> 5) {                                                                 \
+            if (ok) stE(expn + to[ti] + tot + e, acc);                                   \
+            ++ti;                                                                        \
+            acc = make_double2(0.0, 0.0);                                                \
+        }                                                                                \
+    }
+    // The next batch is always loaded before the current one is applied
+    // (past the end of the stream it reads the zero slot), so the wait for
+    // a batch leaves exactly one batch in flight on every path.
+    M2L_LOAD(va, j0);
+    for (int j = j0; j < j1; j += 2 * kM2lBatch) {
+        M2L_LOAD(vb, j + kM2lBatch);
+        M2L_APPLY(va, j);
+        if (j + kM2lBatch >= j1) break;
+        M2L_LOAD(va, j + 2 * kM2lBatch);
+        M2L_APPLY(vb, j + kM2lBatch);
+    }
+#undef M2L_APPLY
+#undef M2L_LOAD
 }
 
 // ---------------------------------------------------------------- STEP 4
@@ -423,19 +442,17 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
     check(hipGetLastError(), "k_merge launch");
 }
 
-void launch_m2l27(const Geom& g, const DevTables& t, int ntask, const long long* toff, const int* poff,
-                  const long long* soff, const int* so, double2* expn, hipStream_t s) {
-    if (ntask <= 0) return;
-    const int group = m2l_group();
-    const int ngroup = (ntask + group - 1) / group;
+void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, long long zero_off, const long long* toff,
+                  const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s) {
+    if (ngroup <= 0) return;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
     if (t.expn_f32)
-        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, group,
-                           toff, poff, soff, so, reinterpret_cast<float2*>(expn));
+        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, zero_off, toff, gent,
+                           soff, scode, reinterpret_cast<float2*>(expn));
     else
-        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ntask, ngroup, group,
-                           toff, poff, soff, so, expn);
+        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, zero_off, toff, gent,
+                           soff, scode, expn);
     check(hipGetLastError(), "k_m2l27 launch");
 }
 
