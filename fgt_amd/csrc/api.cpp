@@ -78,7 +78,7 @@ struct fgt_solver {
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
     DBuf d_s5_tasks;   // fgt::PotTask list of the 3-D STEP 5 kernel
-    int n_s5 = 0;
+    int n_s5 = 0, n_s5_sib = 0;   // tasks, of which the first n_s5_sib are sibling tasks
     DBuf d_p2pwS, d_fz_lev, d_fz_slot, d_fz_kids, d_fz_ksg;          // fused STEP 1+2 parents
     DBuf d_s1_box, d_s1_lev, d_s1_slot;                              // standalone STEP 1 leaves
     DBuf d_mp_slot, d_mp_first, d_mp_count, d_mp_child, d_mp_sign, d_mp_acc;  // merges of non-leaf children
@@ -277,6 +277,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
                 own.push_back(pt);
             }
         }
+        S.n_s5_sib = (int)fam.size();
         fam.insert(fam.end(), own.begin(), own.end());
         S.n_s5 = (int)fam.size();
         S.d_s5_tasks.upload(fam);
@@ -605,8 +606,8 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         // STEP 5, once per output component (pot, grad, hess entries)
         for (const Comp& c : comps) {
             if (S.fast3d)
-                fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.d_s5_tasks.as<fgt::PotTask>(), S.d_expn.as<double2>(),
-                                         c.kd, c.od, st);
+                fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.n_s5_sib, S.d_s5_tasks.as<fgt::PotTask>(),
+                                         S.d_expn.as<double2>(), c.kd, c.od, st);
             else
                 fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
                                    S.d_leaf_slot.as<int>(), S.d_expn.as<double2>(), c.kd, c.od, st);

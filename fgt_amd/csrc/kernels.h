@@ -107,7 +107,8 @@ struct DevTables {
 // false (nothing launched) when the shape is not covered.
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                     const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
-bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+// tasks [0, nsib): sibling tasks (up to four leaves); [nsib, ntask): one leaf
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                       const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit);
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 // true when FGT_PW2POT_VALU=1: STEP 5 keeps its x stage on the VALU
@@ -147,7 +148,7 @@ void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
                    const int* lev, const int* slot, const double2* expn,
                    Kinds kd, OutDesc od, hipStream_t s);
 // 3-D STEP 5 over PotTask lists (throws when the shape is not covered)
-void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s);
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.

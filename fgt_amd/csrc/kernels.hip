@@ -469,10 +469,10 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
     check(hipGetLastError(), "k_split launch");
 }
 
-void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks,
+void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;
-    if (!launch_pw2pot_3d(g, t, ntask, tasks, expn, kd, od, s, t.lds_limit))
+    if (!launch_pw2pot_3d(g, t, ntask, nsib, tasks, expn, kd, od, s, t.lds_limit))
         throw std::runtime_error("3-D STEP 5 kernel not available for this shape");
     check(hipGetLastError(), "k_pw2pot3 launch");
 }

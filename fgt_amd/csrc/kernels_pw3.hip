@@ -285,7 +285,10 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // operand.  (For STEP 1 the matrix-core form is slower: its 50 x 50 output
 // pads to 64 x 64 tiles, where the VALU form wastes only the lane padding.)
 
-template <int N, bool MF, typename ET>
+// C: most leaves per task (4 for sibling tasks, 1 for a leaf's own
+// expansion; with one leaf the freed registers carry the x stage's next
+// K-step loads while the current one runs).
+template <int N, bool MF, int C, typename ET>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS, int npw,
                                                 const PotTask* __restrict__ tasks, const ET* __restrict__ expn,
                                                 size_t ks, Kinds kd, OutDesc od) {
@@ -297,9 +300,9 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     constexpr int PF = (N == 12) ? 6 : 8;
     const PotTask* __restrict__ T = tasks + xcd_task(blockIdx.x, gridDim.x);
     const int sl = T->slot, l = T->lev, nvy = T->nvy;
-    int cbox[4], ccode[4];
+    int cbox[C], ccode[C];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < C; ++c) {
         cbox[c] = T->box[c];
         ccode[c] = T->code[c];
     }
@@ -320,9 +323,9 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const bool zact = tid < N * N;
     const int zk1 = tid % N, zk2 = tid / N;
     for (int ind = 0; ind < nd; ++ind) {
-        double G[4][N];
+        double G[C][N];
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < C; ++c)
 #pragma unroll
             for (int k = 0; k < N; ++k) G[c][k] = 0.0;
         const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
@@ -346,20 +349,53 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                     for (int t = 0; t < 4; ++t) are[t] = aim[t] = dbl4{0.0, 0.0, 0.0, 0.0};
                     const int ksteps = (npw + 3) >> 2;
-                    for (int st = 0; st < ksteps; ++st) {
-                        const int k1 = 4 * st + lk;
-                        const bool kv = k1 < npw;
-                        const int k1c = kv ? k1 : npw - 1;
-                        double2 u = UX[k1c * N + li];
-                        if (!kv) u = make_double2(0.0, 0.0);
-                        double2 e[4];
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            const int k2 = 16 * t + li;
-                            e[t] = ldE(Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)]);
+                    if constexpr (C == 1) {
+                        // the next K-step's operands are loaded (always, from a
+                        // valid clamped address) before this step's products,
+                        // into two alternating register sets
+                        double2 ua, ub, ea[4], eb[4];
+#define PW2POT_XLOAD(u_, e_, st)                                                             \
+    {                                                                                        \
+        const int k1c_ = min(4 * (st) + lk, npw - 1);                                        \
+        u_ = UX[k1c_ * N + li];                                                              \
+        _Pragma("unroll") for (int t = 0; t < 4; ++t)                                        \
+            e_[t] = ldE(Ek[(size_t)k1c_ * npw + min(16 * t + li, npw - 1)]);                 \
+    }
+#define PW2POT_XSTEP(u_, e_, st)                                                             \
+    {                                                                                        \
+        const double2 um_ = (4 * (st) + lk < npw) ? u_ : make_double2(0.0, 0.0);             \
+        _Pragma("unroll") for (int t = 0; t < 4; ++t) cmfma(um_, e_[t], are[t], aim[t]);     \
+    }
+                        PW2POT_XLOAD(ua, ea, 0);
+                        for (int st = 0; st < ksteps; st += 2) {
+                            PW2POT_XLOAD(ub, eb, min(st + 1, ksteps - 1));
+                            // keep the loads above this step's products (the
+                            // scheduler would otherwise sink them to their use)
+                            __builtin_amdgcn_sched_barrier(0);
+                            PW2POT_XSTEP(ua, ea, st);
+                            if (st + 1 >= ksteps) break;
+                            PW2POT_XLOAD(ua, ea, min(st + 2, ksteps - 1));
+                            __builtin_amdgcn_sched_barrier(0);
+                            PW2POT_XSTEP(ub, eb, st + 1);
                         }
+#undef PW2POT_XSTEP
+#undef PW2POT_XLOAD
+                    } else {
+                        for (int st = 0; st < ksteps; ++st) {
+                            const int k1 = 4 * st + lk;
+                            const bool kv = k1 < npw;
+                            const int k1c = kv ? k1 : npw - 1;
+                            double2 u = UX[k1c * N + li];
+                            if (!kv) u = make_double2(0.0, 0.0);
+                            double2 e[4];
 #pragma unroll
-                        for (int t = 0; t < 4; ++t) cmfma(u, e[t], are[t], aim[t]);
+                            for (int t = 0; t < 4; ++t) {
+                                const int k2 = 16 * t + li;
+                                e[t] = ldE(Ek[(size_t)k1c * npw + (k2 < npw ? k2 : npw - 1)]);
+                            }
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) cmfma(u, e[t], are[t], aim[t]);
+                        }
                     }
                     // lane holds kk1 = lk + 4v for column k2 = 16t + li
 #pragma unroll
@@ -436,7 +472,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                         const double w = (halve && k3z == nh - 1) ? 0.5 : 1.0;
                         const double yr = yv.x * w, yi = yv.y * w;
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
+                        for (int c = 0; c < C; ++c) {
                             if (cbox[c] >= 0 && (ccode[c] & 1) == j) {
                                 const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * 4 + s2) * N;
 #pragma unroll
@@ -457,7 +493,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             int col = zk1 + N * zk2;
             asm volatile("" : "+v"(col));
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < C; ++c)
                 if (cbox[c] >= 0) {
                     double* __restrict__ o = od.p + (size_t)cbox[c] * g.npbox * od.ns + od.coff + ind;
 #pragma unroll
@@ -485,27 +521,36 @@ size_t pw2pot3_lds(int n, int npw) {
     return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + 12 * (size_t)n);
 }
 
-template <int N, bool MF>
-void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
-                          Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
+template <int N, bool MF, int C>
+void launch_pw2pot3_c(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
+                      Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
+    if (ntask <= 0) return;
     if (t.expn_f32)
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw, tasks,
-                           reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+                           tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
     else
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+        hipLaunchKernelGGL((k_pw2pot3<N, MF, C, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
                            tasks, expn, t.pw2potS_ks, kd, od);
 }
 
+// tasks [0, nsib) hold up to four leaves, [nsib, ntask) one leaf each
+template <int N, bool MF>
+void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
+    launch_pw2pot3_c<N, MF, 4>(g, t, nsib, tasks, expn, kd, od, s, lds);
+    launch_pw2pot3_c<N, MF, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds);
+}
+
 template <int N>
-bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
-                 Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                 const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
     const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
     if (N == 16 && t.npw <= 64 && !pw2pot_valu_only())
-        launch_pw2pot3_typed<N, true>(g, t, ntask, tasks, expn, kd, od, s, lds);
+        launch_pw2pot3_typed<N, true>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
     else
-        launch_pw2pot3_typed<N, false>(g, t, ntask, tasks, expn, kd, od, s, lds);
+        launch_pw2pot3_typed<N, false>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
     return true;
 }
 
@@ -568,13 +613,13 @@ bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
 
-bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
-                      Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
     if (g.ndim != 3 || ntask <= 0) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
-        case 12: return try_pw2pot3<12>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
-        case 16: return try_pw2pot3<16>(g, t, ntask, tasks, expn, kd, od, s, lds_limit);
+        case 8: return try_pw2pot3<8>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
+        case 12: return try_pw2pot3<12>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
+        case 16: return try_pw2pot3<16>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
         default: return false;
     }
 }
