@@ -21,7 +21,7 @@ namespace {
 
 // flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group
 template <int N>
-__global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restrict__ loctab,
+__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab,
                                                 const int* __restrict__ tgt, const int* __restrict__ lev,
                                                 const int* __restrict__ off, const int* __restrict__ src,
                                                 const int* __restrict__ code, const int* __restrict__ flags,
@@ -52,8 +52,20 @@ __global__ void __launch_bounds__(256) k_direct3(Geom g, const double* __restric
             if (act) {
                 double f[N];
                 const double* fs = fvals + ((size_t)sb * g.npbox + (size_t)N * tid) * nd + ind;
+                if (nd == 1) {
+                    // one density: the thread's N values are contiguous and
+                    // 16-byte aligned; read them as pairs
+                    const double2* fs2 = reinterpret_cast<const double2*>(fs);
 #pragma unroll
-                for (int i = 0; i < N; ++i) f[i] = fs[(size_t)i * nd];
+                    for (int i = 0; i < N / 2; ++i) {
+                        const double2 v = fs2[i];
+                        f[2 * i] = v.x;
+                        f[2 * i + 1] = v.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) f[i] = fs[(size_t)i * nd];
+                }
 #pragma unroll
                 for (int k1 = 0; k1 < N; ++k1) {
                     double s = X[k1];
