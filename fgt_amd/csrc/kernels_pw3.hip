@@ -37,7 +37,9 @@ __device__ __forceinline__ void cmfma(double2 a, double2 b, dbl4& re, dbl4& im) 
 }
 
 // ---------------------------------------------------------------- STEP 1
-// LDS: sff [m2][s][m1] (N*4*N complex), sff2 [s][m1][k2] (4*N*npw complex)
+// LDS: sff [m2][s][m1] (N*4*N complex) and sff2 [s][m1][k2] (4*N*npw
+// complex) share one buffer: the y stage reads its sff column into registers
+// and passes a barrier before writing sff2 (more blocks fit on a CU).
 template <int N, typename ET>
 __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab, int npw,
                                               const int* __restrict__ box, const int* __restrict__ lev,
@@ -46,7 +48,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
                                               ET* __restrict__ expn) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
-    double2* sff2 = sff + 4 * N * N;
+    double2* sff2 = sm;
     const int task = blockIdx.x;
     const int b = box[task], l = lev[task], sl = slot[task];
     const double2* __restrict__ T = tab + (size_t)l * npw * N;   // T[k][m]
@@ -90,6 +92,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 #pragma unroll
                 for (int m2 = 0; m2 < N; ++m2)
                     ff[m2] = act ? sff[(m2 * 4 + s) * N + m1] : make_double2(0.0, 0.0);
+                __syncthreads();   // sff consumed; sff2 (same memory) may be written
                 for (int k2 = wave; k2 < npw; k2 += 4) {
                     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
@@ -506,7 +509,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 template <int N>
 bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
                const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
-    const size_t lds = sizeof(double2) * (4 * (size_t)N * N + 4 * (size_t)N * t.npw);
+    const size_t lds = sizeof(double2) * std::max(4 * (size_t)N * N, 4 * (size_t)N * t.npw);
     if (lds > lds_limit) return false;
     if (t.expn_f32)
         hipLaunchKernelGGL((k_p2pw3<N, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
@@ -608,7 +611,7 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     if (g.ndim != 3 || (t.n != 8 && t.n != 12 && t.n != 16)) return false;
     const size_t lds = pw2pot3_lds(t.n, t.npw);
-    const size_t lds1 = sizeof(double2) * (4 * (size_t)t.n * t.n + 4 * (size_t)t.n * t.npw);
+    const size_t lds1 = sizeof(double2) * std::max(4 * (size_t)t.n * t.n, 4 * (size_t)t.n * t.npw);
     const size_t lds2 = sizeof(double2) * std::max(8 * (size_t)t.n * t.n, 4 * (size_t)t.n * t.npw);
     return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
