@@ -94,10 +94,14 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
                     ff[m2] = act ? sff[(m2 * 4 + s) * N + m1] : make_double2(0.0, 0.0);
                 __syncthreads();   // sff consumed; sff2 (same memory) may be written
                 for (int k2 = wave; k2 < npw; k2 += 4) {
-                    double2 acc = make_double2(0.0, 0.0);
+                    // two partial sums: four independent FMA chains
+                    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
 #pragma unroll
-                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(T[k2 * N + m2], ff[m2], acc);
-                    if (act) sff2[(s * N + m1) * npw + k2] = acc;
+                    for (int m2 = 0; m2 < N; m2 += 2) {
+                        a0 = zfma(T[k2 * N + m2], ff[m2], a0);
+                        a1 = zfma(T[k2 * N + m2 + 1], ff[m2 + 1], a1);
+                    }
+                    if (act) sff2[(s * N + m1) * npw + k2] = make_double2(a0.x + a1.x, a0.y + a1.y);
                 }
             }
             __syncthreads();
@@ -114,10 +118,13 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
                         for (int m1 = 0; m1 < N; ++m1)
                             f2[m1] = act ? sff2[(wave * N + m1) * npw + k2] : make_double2(0.0, 0.0);
                         for (int k1 = 0; k1 < npw; ++k1) {
-                            double2 acc = make_double2(0.0, 0.0);
+                            double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
 #pragma unroll
-                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(T[k1 * N + m1], f2[m1], acc);
-                            if (act) stE(Ek + (size_t)k1 * npw + k2, acc);
+                            for (int m1 = 0; m1 < N; m1 += 2) {
+                                a0 = zfma(T[k1 * N + m1], f2[m1], a0);
+                                a1 = zfma(T[k1 * N + m1 + 1], f2[m1 + 1], a1);
+                            }
+                            if (act) stE(Ek + (size_t)k1 * npw + k2, make_double2(a0.x + a1.x, a0.y + a1.y));
                         }
                     }
                 }
@@ -223,12 +230,13 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 const double2* __restrict__ Tym = T + base0 + 2 * per;
                 __syncthreads();   // sH consumed; sF (same memory) may be written
                 for (int k2 = wave; k2 < npw; k2 += 4) {
-                    double2 acc = make_double2(0.0, 0.0);
+                    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
 #pragma unroll
-                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(Typ[k2 * N + m2], hp[m2], acc);
-#pragma unroll
-                    for (int m2 = 0; m2 < N; ++m2) acc = zfma(Tym[k2 * N + m2], hm[m2], acc);
-                    if (act) sF[((sx * 2 + s) * N + m1) * npw + k2] = acc;
+                    for (int m2 = 0; m2 < N; ++m2) {
+                        a0 = zfma(Typ[k2 * N + m2], hp[m2], a0);
+                        a1 = zfma(Tym[k2 * N + m2], hm[m2], a1);
+                    }
+                    if (act) sF[((sx * 2 + s) * N + m1) * npw + k2] = make_double2(a0.x + a1.x, a0.y + a1.y);
                 }
             }
             __syncthreads();
@@ -251,12 +259,13 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                             fm[m1] = act ? sF[((1 * 2 + s) * N + m1) * npw + k2] : make_double2(0.0, 0.0);
                         }
                         for (int k1 = k1a; k1 < k1b; ++k1) {
-                            double2 acc = make_double2(0.0, 0.0);
+                            double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
 #pragma unroll
-                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txp[k1 * N + m1], fp[m1], acc);
-#pragma unroll
-                            for (int m1 = 0; m1 < N; ++m1) acc = zfma(Txm[k1 * N + m1], fm[m1], acc);
-                            if (act) stE(Ek + (size_t)k1 * npw + k2, acc);
+                            for (int m1 = 0; m1 < N; ++m1) {
+                                a0 = zfma(Txp[k1 * N + m1], fp[m1], a0);
+                                a1 = zfma(Txm[k1 * N + m1], fm[m1], a1);
+                            }
+                            if (act) stE(Ek + (size_t)k1 * npw + k2, make_double2(a0.x + a1.x, a0.y + a1.y));
                         }
                     }
                 }
