@@ -428,12 +428,13 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
         // final entry flagged; each group padded with zero-slot entries
         const long long se = 2LL * P.nd * P.nexp;
         const int nt = (int)P.m2l_tgt.size();
-        const int ng = (nt + fgt::kM2lGroup - 1) / fgt::kM2lGroup;
-        std::vector<long long> toff((size_t)ng * fgt::kM2lGroup, 0), soff;
+        const int G = fgt::m2l_group();
+        const int ng = (nt + G - 1) / G;
+        std::vector<long long> toff((size_t)ng * G, 0), soff;
         std::vector<int> gent(ng + 1, 0), code;
         for (int gi = 0; gi < ng; ++gi) {
             gent[gi] = (int)soff.size();
-            for (int tk = gi * fgt::kM2lGroup; tk < std::min(nt, (gi + 1) * fgt::kM2lGroup); ++tk) {
+            for (int tk = gi * G; tk < std::min(nt, (gi + 1) * G); ++tk) {
                 toff[tk] = se * P.m2l_tgt[tk];
                 soff.push_back(toff[tk]);
                 code.push_back(13);
@@ -585,9 +586,9 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
             mark(2);
         }
         // STEP 3
-        fgt::launch_m2l27(g, t, S.n_m2l_groups, 2LL * P.nd * P.nexp * P.nslots, S.d_m2l_toff.as<long long>(),
-                          S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(), S.d_m2l_so.as<int>(),
-                          S.d_expn.as<double2>(), st);
+        fgt::launch_m2l27(g, t, S.n_m2l_groups, fgt::m2l_group(), 2LL * P.nd * P.nexp * P.nslots,
+                          S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(),
+                          S.d_m2l_so.as<int>(), S.d_expn.as<double2>(), st);
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)

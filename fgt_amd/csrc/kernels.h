@@ -135,12 +135,17 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 // o | last << 5 with o = ox + 3 oy + 9 oz in 0..26 and last set on each
 // target's final entry.
 constexpr int kM2lGroup = 8;
+constexpr int kM2lGroupMax = 16;
+// targets per STEP 3 block: kM2lGroup, or FGT_M2L_GROUP (1..kM2lGroupMax,
+// tuning runs)
+int m2l_group();
 constexpr int kM2lBatch = 4;
-// longest entry stream of a group: kM2lGroup targets x (own + 26 list
+// longest entry stream of a group: kM2lGroupMax targets x (own + 26 list
 // entries), padded
-constexpr int kM2lMaxEntries = kM2lGroup * 27 + kM2lBatch;
-void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, long long zero_off, const long long* toff,
-                  const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s);
+constexpr int kM2lMaxEntries = kM2lGroupMax * 27 + kM2lBatch;
+void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
+                  const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
+                  hipStream_t s);
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s);

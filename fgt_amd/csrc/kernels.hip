@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 
 namespace fgt {
@@ -177,7 +178,7 @@ __device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, cons
 #undef M2L_CASE
 
 template <typename ET>
-__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, long long zero_off,
+__global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, int group, long long zero_off,
                                                   const long long* __restrict__ toff,
                                                   const int* __restrict__ gent,
                                                   const long long* __restrict__ soff,
@@ -217,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     }
     __syncthreads();
     const int j0 = 0, j1 = n;
-    const long long* __restrict__ to = toff + (size_t)grp * kM2lGroup;
+    const long long* __restrict__ to = toff + (size_t)grp * group;
     int ti = 0;
     double2 acc = make_double2(0.0, 0.0);
     double2 va[kM2lBatch], vb[kM2lBatch];
@@ -442,16 +443,26 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
     check(hipGetLastError(), "k_merge launch");
 }
 
-void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, long long zero_off, const long long* toff,
-                  const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s) {
+int m2l_group() {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_M2L_GROUP");
+        const int g = e ? std::atoi(e) : 0;
+        return g >= 1 && g <= kM2lGroupMax ? g : kM2lGroup;
+    }();
+    return v;
+}
+
+void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
+                  const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
+                  hipStream_t s) {
     if (ngroup <= 0) return;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
     if (t.expn_f32)
-        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, zero_off, toff, gent,
+        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
                            soff, scode, reinterpret_cast<float2*>(expn));
     else
-        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, zero_off, toff, gent,
+        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
                            soff, scode, expn);
     check(hipGetLastError(), "k_m2l27 launch");
 }
