@@ -2,13 +2,15 @@
 //
 // STEP 1 (values -> PW, g3d_tens_prod_to_pw, bfgt_pwrouts.f:153-222) and
 // STEP 5 (PW -> values, g3d_pw2pot, bfgt_pwrouts.f:719-790) for norder
-// N in {8, 12, 16}, one 256-thread block (4 waves) per leaf.
+// N in {8, 12, 16}, one 256-thread block (4 waves) per task (a leaf, a
+// parent with leaf children, or up to four sibling leaves).
 //
-// Design: every contraction is arranged so that the 1-D table operand is
-// wave-uniform (its address depends only on loop counters and the wave
-// index), which lets it come from the scalar cache into SGPRs; the other
+// Design: on the VALU, every contraction is arranged so that the 1-D table
+// operand is wave-uniform (its address depends only on loop counters and the
+// wave index), which lets it come from the scalar cache into SGPRs; the other
 // operand lives in VGPRs, so each v_fma_f64 has one scalar and one vector
-// source and LDS only carries the transposes between stages.  The half
+// source and LDS only carries the transposes between stages.  STEP 5's x
+// contraction for N = 16 runs on the FP64 matrix cores instead.  The half
 // (z) axis is processed in rounds of four slabs, one per wave.
 #include "kernels.h"
 
