@@ -151,12 +151,12 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 // repeat and hit in L2).  Each thread forms the nine products w_x w_y and
 // three w_z of its element once and reuses them for every list entry of the
 // group.  The group's entries (each target's own expansion first, then its
-// list) form one stream; source loads go kM2lBatch at a time into two
-// register batches, the next batch issued before the current one is
+// list) form one stream; source loads go kM2lBatch at a time into three
+// register batches, the batch two ahead issued before the current one is
 // applied, so a wave keeps loads in flight across its whole stream.  The
 // entry's offset o = ox + 3 oy + 9 oz is wave-uniform; a switch on it picks
 // the phase factors from fixed registers (scalar branches, no per-lane
-// selects).
+// selects; reading the factors from LDS instead was measured slower).
 // (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
 // so the kernel stays small in registers and more waves are resident
 #define M2L_CASE(o)                                                     \
@@ -209,11 +209,11 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     // the group's entry stream, staged in LDS: read back at a wave-uniform
     // address (broadcast), it keeps the vector-memory counter for the source
     // loads alone
-    __shared__ long long s_soff[kM2lMaxEntries + kM2lBatch];
+    __shared__ long long s_soff[kM2lMaxEntries + 2 * kM2lBatch];
     __shared__ int s_code[kM2lMaxEntries];
     const int g0 = gent[grp], n = gent[grp + 1] - g0;   // a positive multiple of kM2lBatch
-    for (int i = threadIdx.x; i < n + kM2lBatch; i += blockDim.x) {
-        s_soff[i] = i < n ? soff[g0 + i] : zero_off;   // one batch of read-ahead past the end
+    for (int i = threadIdx.x; i < n + 2 * kM2lBatch; i += blockDim.x) {
+        s_soff[i] = i < n ? soff[g0 + i] : zero_off;   // two batches of read-ahead past the end
         if (i < n) s_code[i] = scode[g0 + i];
     }
     __syncthreads();
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     const long long* __restrict__ to = toff + (size_t)grp * group;
     int ti = 0;
     double2 acc = make_double2(0.0, 0.0);
-    double2 va[kM2lBatch], vb[kM2lBatch];
+    double2 va[kM2lBatch], vb[kM2lBatch], vc[kM2lBatch];
 #define M2L_LOAD(v, j)                                                                   \
     _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[s_soff[(j) + u]]);
     // code = o | last-entry-of-its-target << 5
@@ -235,16 +235,21 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
             acc = make_double2(0.0, 0.0);                                                \
         }                                                                                \
     }
-    // The next batch is always loaded before the current one is applied
-    // (past the end of the stream it reads the zero slot), so the wait for
-    // a batch leaves exactly one batch in flight on every path.
+    // The batch two ahead is always loaded before the current one is
+    // applied (past the end of the stream it reads the zero slot), so the
+    // wait for a batch leaves exactly two batches in flight on every path.
+    constexpr int B = kM2lBatch;
     M2L_LOAD(va, j0);
-    for (int j = j0; j < j1; j += 2 * kM2lBatch) {
-        M2L_LOAD(vb, j + kM2lBatch);
+    M2L_LOAD(vb, j0 + B);
+    for (int j = j0; j < j1; j += 3 * B) {
+        M2L_LOAD(vc, j + 2 * B);
         M2L_APPLY(va, j);
-        if (j + kM2lBatch >= j1) break;
-        M2L_LOAD(va, j + 2 * kM2lBatch);
-        M2L_APPLY(vb, j + kM2lBatch);
+        if (j + B >= j1) break;
+        M2L_LOAD(va, j + 3 * B);
+        M2L_APPLY(vb, j + B);
+        if (j + 2 * B >= j1) break;
+        M2L_LOAD(vb, j + 4 * B);
+        M2L_APPLY(vc, j + 2 * B);
     }
 #undef M2L_APPLY
 #undef M2L_LOAD
