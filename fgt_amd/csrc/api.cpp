@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -829,15 +830,86 @@ int fgt_solver_set_ifpgh(fgt_solver* s, int ifpgh) {
 }
 
 namespace {
-void download(const DBuf& b, double* out, int accumulate, const char* what) {
+// Host <-> device transfers of the drop-in's host arrays go through a pinned
+// staging buffer (allocated once per process, grown on demand, two halves):
+// chunk i is copied by DMA while the host threads move or accumulate chunk
+// i-1.  Pageable hipMemcpy plus a single-threaded += ran at ~2.4 GB/s.
+constexpr size_t kStageChunk = size_t(16) << 20;   // bytes per half
+struct PinnedStage {
+    std::mutex m;
+    char* p = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+PinnedStage& pinned_stage() {
+    static PinnedStage g;
+    std::lock_guard<std::mutex> lk(g.m);
+    if (!g.p) {
+        void* q = nullptr;
+        hcheck(hipHostMalloc(&q, 2 * kStageChunk, hipHostMallocDefault), "hipHostMalloc staging");
+        g.p = static_cast<char*>(q);
+        hcheck(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking), "hipStreamCreate staging");
+        for (auto& e : g.ev) hcheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate staging");
+    }
+    return g;
+}
+
+void host_move(char* dst, const char* src, size_t bytes, bool accumulate) {
     if (!accumulate) {
-        hcheck(hipMemcpy(out, b.p, b.bytes, hipMemcpyDeviceToHost), what);
+        const long long nb = (long long)((bytes + 65535) / 65536);
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < nb; ++i) {
+            const size_t off = (size_t)i * 65536, len = std::min<size_t>(65536, bytes - off);
+            std::memcpy(dst + off, src + off, len);
+        }
         return;
     }
-    const size_t cnt = b.bytes / sizeof(double);
-    std::vector<double> tmp(cnt);
-    hcheck(hipMemcpy(tmp.data(), b.p, b.bytes, hipMemcpyDeviceToHost), what);
-    for (size_t i = 0; i < cnt; ++i) out[i] += tmp[i];
+    double* d = reinterpret_cast<double*>(dst);
+    const double* q = reinterpret_cast<const double*>(src);
+    const long long n = (long long)(bytes / sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) d[i] += q[i];
+}
+
+// device -> host (out = or += device values); waits for all device work
+// first, like a blocking hipMemcpy (the staging stream is non-blocking)
+void download(const DBuf& b, double* out, int accumulate, const char* what) {
+    hcheck(hipDeviceSynchronize(), what);
+    PinnedStage& g = pinned_stage();
+    std::lock_guard<std::mutex> lk(g.m);
+    const size_t total = b.bytes;
+    const size_t nch = (total + kStageChunk - 1) / kStageChunk;
+    auto issue = [&](size_t c) {
+        const size_t off = c * kStageChunk, len = std::min(kStageChunk, total - off);
+        char* half = g.p + (c & 1) * kStageChunk;
+        hcheck(hipMemcpyAsync(half, static_cast<const char*>(b.p) + off, len, hipMemcpyDeviceToHost, g.st), what);
+        hcheck(hipEventRecord(g.ev[c & 1], g.st), what);
+    };
+    if (nch > 0) issue(0);
+    for (size_t c = 0; c < nch; ++c) {
+        hcheck(hipEventSynchronize(g.ev[c & 1]), what);
+        if (c + 1 < nch) issue(c + 1);   // the other half, free since chunk c-1 was consumed
+        const size_t off = c * kStageChunk, len = std::min(kStageChunk, total - off);
+        host_move(reinterpret_cast<char*>(out) + off, g.p + (c & 1) * kStageChunk, len, accumulate != 0);
+    }
+}
+
+// host -> device; waits for all device work first
+void upload_host(const DBuf& b, const double* in, const char* what) {
+    hcheck(hipDeviceSynchronize(), what);
+    PinnedStage& g = pinned_stage();
+    std::lock_guard<std::mutex> lk(g.m);
+    const size_t total = b.bytes;
+    const size_t nch = (total + kStageChunk - 1) / kStageChunk;
+    for (size_t c = 0; c < nch; ++c) {
+        const size_t off = c * kStageChunk, len = std::min(kStageChunk, total - off);
+        char* half = g.p + (c & 1) * kStageChunk;
+        if (c >= 2) hcheck(hipEventSynchronize(g.ev[c & 1]), what);   // DMA of chunk c-2 is done
+        host_move(half, reinterpret_cast<const char*>(in) + off, len, false);
+        hcheck(hipMemcpyAsync(static_cast<char*>(b.p) + off, half, len, hipMemcpyHostToDevice, g.st), what);
+        hcheck(hipEventRecord(g.ev[c & 1], g.st), what);
+    }
+    hcheck(hipStreamSynchronize(g.st), what);
 }
 }  // namespace
 
@@ -1029,7 +1101,7 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
             const int rc = fgt_solver_set_ifpgh(&S, ifpgh_dev);
             if (rc) throw std::runtime_error(g_msg);
         }
-        hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
+        upload_host(S.d_fvals, fvals, "upload fvals");
         const double t1 = now_s();
         execute(S, nullptr, timeinfo != nullptr);
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -1043,11 +1115,9 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
             // STEP 11 evaluates what the reference holds at that point: the
             // caller's accumulated arrays for the outputs it requested, fresh
             // values for the ones formed only for the targets
-            hcheck(hipMemcpy(S.d_pot.p, pot, S.d_pot.bytes, hipMemcpyHostToDevice), "upload pot");
-            if (*ifpgh >= 2 && *ifpghtarg >= 2)
-                hcheck(hipMemcpy(S.d_grad.p, grad, S.d_grad.bytes, hipMemcpyHostToDevice), "upload grad");
-            if (*ifpgh >= 3 && *ifpghtarg >= 3)
-                hcheck(hipMemcpy(S.d_hess.p, hess, S.d_hess.bytes, hipMemcpyHostToDevice), "upload hess");
+            upload_host(S.d_pot, pot, "upload pot");
+            if (*ifpgh >= 2 && *ifpghtarg >= 2) upload_host(S.d_grad, grad, "upload grad");
+            if (*ifpgh >= 3 && *ifpghtarg >= 3) upload_host(S.d_hess, hess, "upload hess");
             eval_targets(S, nt, targs, *ifpghtarg, pote, grade, hesse, nullptr);
             if (timeinfo) timeinfo[11] = now_s() - t3;
         }
