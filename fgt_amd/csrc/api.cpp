@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -88,6 +89,15 @@ struct fgt_solver {
     DBuf d_pcs_parent, d_pcs_child, d_pcs_sign;   // splits into non-leaf children only
     std::vector<int> pcs_level_off;
     double stage_ms[8] = {0};
+    // STEP 6 on a second stream, overlapping STEPs 1-4 (it reads only fvals;
+    // STEP 5 waits for it, so pot gets each sum once): made on first use
+    hipStream_t st_near = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    ~fgt_solver() {
+        if (st_near) (void)hipStreamDestroy(st_near);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+    }
 };
 
 // Many independent problems (uneven sizes, orders, delta) solved together:
@@ -538,18 +548,44 @@ std::vector<Comp> components(fgt_solver& S) {
     return c;
 }
 
-void execute(fgt_solver& S, hipStream_t st, bool timed) {
+// fork_near: run STEP 6 on the solver's second stream, overlapping STEPs
+// 1-4 on `st` (single-problem executes; a batch already overlaps problems).
+// The outputs are the same bit for bit: each output node receives one STEP 5
+// and one STEP 6 sum onto zero, and two-term addition is commutative.
+void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) {
     const fgt::Plan& P = S.plan;
     const fgt::Geom& g = S.geom;
     const fgt::DevTables& t = S.tabs;
-    hipEvent_t ev[8];
-    if (timed)
+    hipEvent_t ev[8], evn[2];
+    if (timed) {
         for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
+        for (auto& e : evn) hcheck(hipEventCreate(&e), "hipEventCreate");
+    }
     auto mark = [&](int i) { if (timed) hcheck(hipEventRecord(ev[i], st), "hipEventRecord"); };
     hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
     if (S.ifpgh >= 2) hcheck(hipMemsetAsync(S.d_grad.p, 0, S.d_grad.bytes, st), "hipMemsetAsync grad");
     if (S.ifpgh >= 3) hcheck(hipMemsetAsync(S.d_hess.p, 0, S.d_hess.bytes, st), "hipMemsetAsync hess");
     const std::vector<Comp> comps = components(S);
+    auto near_field = [&](hipStream_t sn) {
+        for (const Comp& c : comps)
+            fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
+                               S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
+                               S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, sn);
+    };
+    const bool fork = fork_near && P.do_pw && !P.dir_tgt.empty();
+    if (fork) {
+        if (!S.st_near) {
+            hcheck(hipStreamCreateWithFlags(&S.st_near, hipStreamNonBlocking), "hipStreamCreate near");
+            hcheck(hipEventCreateWithFlags(&S.ev_fork, hipEventDisableTiming), "hipEventCreate");
+            hcheck(hipEventCreateWithFlags(&S.ev_join, hipEventDisableTiming), "hipEventCreate");
+        }
+        hcheck(hipEventRecord(S.ev_fork, st), "hipEventRecord");   // after the zeroing
+        hcheck(hipStreamWaitEvent(S.st_near, S.ev_fork, 0), "hipStreamWaitEvent");
+        if (timed) hcheck(hipEventRecord(evn[0], S.st_near), "hipEventRecord");
+        near_field(S.st_near);
+        if (timed) hcheck(hipEventRecord(evn[1], S.st_near), "hipEventRecord");
+        hcheck(hipEventRecord(S.ev_join, S.st_near), "hipEventRecord");
+    }
     mark(0);
     if (P.do_pw) {
         if (S.fast3d) {
@@ -606,6 +642,10 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         }
         mark(4);
         // STEP 5, once per output component (pot, grad, hess entries)
+        if (fork) {
+            hcheck(hipStreamWaitEvent(st, S.ev_join, 0), "hipStreamWaitEvent");
+            mark(7);   // STEP 5 starts here (after the wait for STEP 6)
+        }
         for (const Comp& c : comps) {
             if (S.fast3d)
                 fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.n_s5_sib, S.d_s5_tasks.as<fgt::PotTask>(),
@@ -619,19 +659,26 @@ void execute(fgt_solver& S, hipStream_t st, bool timed) {
         for (int i = 1; i <= 5; ++i) mark(i);
     }
     // STEP 6
-    for (const Comp& c : comps)
-        fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
-                           S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
-                           S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, st);
+    if (!fork) near_field(st);
     mark(6);
     if (timed) {
         hcheck(hipEventSynchronize(ev[6]), "hipEventSynchronize");
         for (int i = 0; i < 6; ++i) {
             float ms = 0;
-            hcheck(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "hipEventElapsedTime");
+            // with the fork: STEP 5 from the end of the wait, STEP 6 on its own stream
+            const hipEvent_t a = (fork && i == 4) ? ev[7] : ev[i], b = (fork && i == 5) ? evn[1] : ev[i + 1];
+            hcheck(hipEventElapsedTime(&ms, (fork && i == 5) ? evn[0] : a, b), "hipEventElapsedTime");
             S.stage_ms[i] = ms;
         }
+        S.stage_ms[6] = 0.0;
+        if (fork) {
+            // whole execute, first to last event of either stream
+            float ms = 0;
+            hcheck(hipEventElapsedTime(&ms, ev[0], ev[6]), "hipEventElapsedTime");
+            S.stage_ms[6] = ms;
+        }
         for (auto& e : ev) (void)hipEventDestroy(e);
+        for (auto& e : evn) (void)hipEventDestroy(e);
     }
 }
 
@@ -776,7 +823,7 @@ int fgt_solver_set_fvals(fgt_solver* s, const double* fvals) {
 int fgt_solver_execute(fgt_solver* s, void* stream, int timed) {
     return guarded([&] {
         if (!s) throw std::invalid_argument("null solver");
-        execute(*s, static_cast<hipStream_t>(stream), timed != 0);
+        execute(*s, static_cast<hipStream_t>(stream), timed != 0, true);
     });
 }
 
@@ -1103,7 +1150,7 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         }
         upload_host(S.d_fvals, fvals, "upload fvals");
         const double t1 = now_s();
-        execute(S, nullptr, timeinfo != nullptr);
+        execute(S, nullptr, timeinfo != nullptr, true);
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const double t2 = now_s();
         // accumulate into the caller's arrays (+=), as the reference does
