@@ -154,28 +154,9 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 // list) form one stream; source loads go kM2lBatch at a time into three
 // register batches, the batch two ahead issued before the current one is
 // applied, so a wave keeps loads in flight across its whole stream.  The
-// entry's offset o = ox + 3 oy + 9 oz is wave-uniform; a switch on it picks
-// the phase factors from fixed registers (scalar branches, no per-lane
-// selects; reading the factors from LDS instead was measured slower).
-// (v w_xy) w_z per entry: no 27 loop-invariant phase products to keep live,
-// so the kernel stays small in registers and more waves are resident
-#define M2L_CASE(o)                                                     \
-    case o:                                                             \
-        acc = cfma(cmul(v, fxy[(o) % 9]), fz[(o) / 9], acc);             \
-        break;
-
-__device__ __forceinline__ double2 m2l_apply(int o, double2 v, double2 acc, const double2 (&fxy)[9],
-                                             const double2 (&fz)[3]) {
-    switch (o) {
-        M2L_CASE(0) M2L_CASE(1) M2L_CASE(2) M2L_CASE(3) M2L_CASE(4) M2L_CASE(5) M2L_CASE(6) M2L_CASE(7)
-        M2L_CASE(8) M2L_CASE(9) M2L_CASE(10) M2L_CASE(11) M2L_CASE(12) M2L_CASE(13) M2L_CASE(14)
-        M2L_CASE(15) M2L_CASE(16) M2L_CASE(17) M2L_CASE(18) M2L_CASE(19) M2L_CASE(20) M2L_CASE(21)
-        M2L_CASE(22) M2L_CASE(23) M2L_CASE(24) M2L_CASE(25) M2L_CASE(26)
-        default: break;
-    }
-    return acc;
-}
-#undef M2L_CASE
+// entry's offset o = ox + 3 oy + 9 oz is wave-uniform: it picks the w_x w_y
+// row in LDS and w_z by a uniform select (a switch on o, compiled to a
+// binary search of scalar branches, cost ~10 branches per entry).
 
 template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, int group, long long zero_off,
@@ -193,18 +174,20 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     if (!ok) e = tot - 1;   // valid duplicate, never stored
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    double2 fxy[9], fz[3];
+    // w_x w_y of the element in LDS (one column per thread, read at a
+    // wave-uniform row), w_z in registers picked by a uniform select
+    __shared__ double2 s_fxy[9][kBlock];
+    const int tid = threadIdx.x;
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy) {
         const double2 wy = g.py > 1 ? t.wm2l[oy * npw + k2] : make_double2(1.0, 0.0);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
             const double2 wx = g.px > 1 ? t.wm2l[ox * npw + k1] : make_double2(1.0, 0.0);
-            fxy[ox + 3 * oy] = cmul(wx, wy);
+            s_fxy[ox + 3 * oy][tid] = cmul(wx, wy);
         }
     }
-#pragma unroll
-    for (int oz = 0; oz < 3; ++oz) fz[oz] = t.wm2l[oz * npw + k3];
+    const double2 fz0 = t.wm2l[k3], fz1 = t.wm2l[npw + k3], fz2 = t.wm2l[2 * npw + k3];
     const ET* __restrict__ eb = expn + e;
     // the group's entry stream, staged in LDS: read back at a wave-uniform
     // address (broadcast), it keeps the vector-memory counter for the source
@@ -227,8 +210,10 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     // code = o | last-entry-of-its-target << 5
 #define M2L_APPLY(v, j)                                                                  \
     _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
-        const int code = s_code[(j) + u];                                                \
-        acc = m2l_apply(code & 31, v[u], acc, fxy, fz);                                  \
+        const int code = __builtin_amdgcn_readfirstlane(s_code[(j) + u]);                \
+        const int o = code & 31, oz = o / 9;                                             \
+        const double2 zf = oz == 0 ? fz0 : (oz == 1 ? fz1 : fz2);                        \
+        acc = cfma(cmul(v[u], s_fxy[o - 9 * oz][tid]), zf, acc);                         \
         if (code >> 5) {                                                                 \
             if (ok) stE(expn + to[ti] + tot + e, acc);                                   \
             ++ti;                                                                        \
