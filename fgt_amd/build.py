@@ -53,7 +53,38 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, OUT)
+    build_fortran_driver(verbose)
     return OUT
+
+
+FORTRAN_DRIVER = os.path.join(HERE, "_build", "drop_in_driver")
+
+
+def flang():
+    for cand in (shutil.which("amdflang"), "/opt/rocm/bin/amdflang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran_driver(verbose: bool = False):
+    """The Fortran drop-in test driver (tests/fortran), linked against the
+    library through the reference's implicit ``boxfgt`` interface and the
+    ``fgt_amd`` module (include/fgt_amd.f90).  Skipped without a Fortran
+    compiler (the tests that run it skip too)."""
+    fc = flang()
+    if fc is None:
+        return None
+    root = os.path.dirname(HERE)
+    moddir = os.path.join(HERE, "_build", "fmod")
+    os.makedirs(moddir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", moddir, os.path.join(root, "include", "fgt_amd.f90"),
+           os.path.join(root, "tests", "fortran", "drop_in_driver.f90"), "-o", FORTRAN_DRIVER,
+           "-L", HERE, "-lfgt_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=moddir)
+    return FORTRAN_DRIVER
 
 
 if __name__ == "__main__":
