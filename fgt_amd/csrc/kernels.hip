@@ -257,6 +257,63 @@ __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int
     stE(expn + ((size_t)pc_child[entry] * 2 + 1) * tot + e, cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g)));
 }
 
+// ---------------------------------------------------------------- STEP 5, 2-D
+// g2d_pw2pot (bfgt_pwrouts.f:473-526): pot[kk1][kk2] += 2 Re sum_{k1, k3}
+// Ux[kk1][k1] Uz[kk2][k3] loc[k3][k1] over the half axis k3 (the last one
+// halved when npw is odd).  A 2-D leaf's expansion is small (npw x nh
+// complex), so the block stages it in LDS with coalesced loads and spreads
+// both contractions over all its threads: x with one thread per (kk1, k3),
+// then the half axis with one thread per output node.  (The generic kernel
+// gave the half axis one barrier-separated iteration each with n active
+// threads; 2-D problems ran ~40x slower per node than 3-D ones.)
+// LDS: sE [k3][k1] (nh*npw complex), sff [kk1][k3] (n*nh complex).
+__global__ void __launch_bounds__(kBlock) k_pw2pot2(Geom g, DevTables t, const int* __restrict__ box,
+                                                    const int* __restrict__ lev, const int* __restrict__ slot,
+                                                    const double2* __restrict__ expn, Kinds kd, OutDesc od) {
+    extern __shared__ double2 sm[];
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], sl = slot[task];
+    const int n = t.n, npw = t.npw, nh = g.pz;
+    double2* sE = sm;
+    double2* sff = sE + (size_t)nh * npw;
+    const size_t lo = (size_t)l * n * npw;
+    const double2* __restrict__ Ux = t.pw2pot + kd.x * t.pw2pot_ks + lo;   // [i][k]
+    const double2* __restrict__ Uz = t.pw2pot + kd.z * t.pw2pot_ks + lo;
+    const bool halve_last = (npw % 2) == 1;
+    const int tid = threadIdx.x;
+    for (int ind = 0; ind < g.nd; ++ind) {
+        const double2* __restrict__ E = expn + ((size_t)sl * 2 * g.nd + g.nd + ind) * g.nexp;
+        for (int i = tid; i < nh * npw; i += blockDim.x) sE[i] = E[i];
+        __syncthreads();
+        for (int o = tid; o < n * nh; o += blockDim.x) {
+            const int kk1 = o % n, k3 = o / n;
+            const double2* __restrict__ u = Ux + (size_t)kk1 * npw;
+            const double2* __restrict__ e = sE + (size_t)k3 * npw;
+            double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+            int k1 = 0;
+            for (; k1 + 1 < npw; k1 += 2) {
+                v0 = cfma(u[k1], e[k1], v0);
+                v1 = cfma(u[k1 + 1], e[k1 + 1], v1);
+            }
+            if (k1 < npw) v0 = cfma(u[k1], e[k1], v0);
+            sff[kk1 * nh + k3] = make_double2(v0.x + v1.x, v0.y + v1.y);
+        }
+        __syncthreads();
+        for (int o = tid; o < n * n; o += blockDim.x) {
+            const int kk1 = o % n, kk2 = o / n;
+            const double2* __restrict__ u = Uz + (size_t)kk2 * npw;
+            const double2* __restrict__ f = sff + (size_t)kk1 * nh;
+            double acc = 0.0;
+            for (int k3 = 0; k3 < nh; ++k3) {
+                const double w = (halve_last && k3 == nh - 1) ? 0.5 : 1.0;
+                acc += (u[k3].x * f[k3].x - u[k3].y * f[k3].y) * w;
+            }
+            od.p[((size_t)b * g.npbox + kk1 + n * kk2) * od.ns + od.coff + ind] += 2.0 * acc;
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- STEP 5
 // Tables per axis by kind (kd); the z table is staged in LDS, the x/y
 // tables come from LDS when they are the same kind, else from global memory.
@@ -482,6 +539,14 @@ void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
                    const int* slot, const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;
     if (t.expn_f32) throw std::runtime_error("FP32 expansions need the 3-D register-blocked kernels");
+    if (g.ndim == 2) {
+        const size_t lds2 = sizeof(double2) * ((size_t)g.pz * t.npw + (size_t)t.n * g.pz);
+        if (lds2 <= t.lds_limit) {
+            hipLaunchKernelGGL(k_pw2pot2, dim3(ntask), dim3(kBlock), lds2, s, g, t, box, lev, slot, expn, kd, od);
+            check(hipGetLastError(), "k_pw2pot2 launch");
+            return;
+        }
+    }
     const size_t lds = sizeof(double2) * ((size_t)t.n * t.npw + (size_t)g.nx * g.py);
     hipLaunchKernelGGL(k_pw2pot, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, expn, kd, od);
     check(hipGetLastError(), "k_pw2pot launch");
