@@ -8,6 +8,7 @@
 //   k_split   STEP 4, parent loc -> child loc     (boxfgt.f:827-844)
 //   k_pw2pot  STEP 5, loc -> node values          (boxfgt.f:865-881,
 //             g3d_pw2pot bfgt_pwrouts.f:719-790)
+//   k_pw2pot2 STEP 5 for 2-D                      (g2d_pw2pot bfgt_pwrouts.f:473-526)
 //   k_direct  STEP 6, banded near field           (boxfgt.f:904-928,
 //             g3d_tens_prod_to_potloc_fast bfgt_local.f:1322-1618)
 #include "kernels.h"
