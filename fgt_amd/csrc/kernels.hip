@@ -3,6 +3,7 @@
 // Stage map (reference loops they replace):
 //   k_p2pw    STEP 1, values -> PW per PW leaf    (boxfgt.f:710-726,
 //             g3d_tens_prod_to_pw bfgt_pwrouts.f:153-222)
+//   k_p2pw2   STEP 1 for 2-D                      (g2d_tens_prod_to_pw bfgt_pwrouts.f:98-146)
 //   k_merge   STEP 2, child mp -> parent mp       (boxfgt.f:746-765)
 //   k_m2l     STEP 3, mp -> loc at cutoff level   (boxfgt.f:787-804)
 //   k_split   STEP 4, parent loc -> child loc     (boxfgt.f:827-844)
@@ -11,6 +12,7 @@
 //   k_pw2pot2 STEP 5 for 2-D                      (g2d_pw2pot bfgt_pwrouts.f:473-526)
 //   k_direct  STEP 6, banded near field           (boxfgt.f:904-928,
 //             g3d_tens_prod_to_potloc_fast bfgt_local.f:1322-1618)
+//   k_direct2 STEP 6 for 2-D                      (g2d_tens_prod_to_potloc_fast bfgt_local.f:1230-1315)
 #include "kernels.h"
 
 #include <cstdio>
@@ -116,6 +118,56 @@ __global__ void __launch_bounds__(kBlock) k_p2pw(Geom g, DevTables t, const int*
         }
     }
     (void)one;
+}
+
+// ---------------------------------------------------------------- STEP 1, 2-D
+// g2d_tens_prod_to_pw (bfgt_pwrouts.f:98-146): E[k3][k1] = sum_{m1, m2}
+// T[k1][m1] T[k3][m2] f[m1][m2] over the half axis k3.  The block stages
+// the level's table and the leaf's n x n values in LDS and spreads each
+// contraction over all its threads: the half axis with one thread per
+// (m1, k3), then x with one thread per output (k1 fastest, so the
+// expansion row is written coalesced).  (The generic kernel ran one
+// barrier-separated pass per half-axis frequency with n active threads.)
+// LDS: sT [k][m] (npw*n complex), ff [m1][k3] (n*nh complex), sf [m2][m1]
+// (n*n real).
+__global__ void __launch_bounds__(kBlock) k_p2pw2(Geom g, DevTables t, const int* __restrict__ box,
+                                                  const int* __restrict__ lev, const int* __restrict__ slot,
+                                                  const double* __restrict__ fvals, double2* __restrict__ expn) {
+    extern __shared__ double2 sm[];
+    const int task = blockIdx.x;
+    const int b = box[task], l = lev[task], sl = slot[task];
+    const int n = t.n, npw = t.npw, nh = g.pz;
+    double2* sT = sm;
+    double2* ff = sT + (size_t)npw * n;
+    double* sf = reinterpret_cast<double*>(ff + (size_t)n * nh);
+    const double2* __restrict__ T = t.p2pw + (size_t)l * npw * n;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < npw * n; i += blockDim.x) sT[i] = T[i];
+    for (int ind = 0; ind < g.nd; ++ind) {
+        for (int i = tid; i < n * n; i += blockDim.x) sf[i] = fvals[((size_t)b * g.npbox + i) * g.nd + ind];
+        __syncthreads();
+        for (int o = tid; o < n * nh; o += blockDim.x) {
+            const int m1 = o % n, k3 = o / n;
+            const double2* __restrict__ tz = sT + (size_t)k3 * n;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int m2 = 0; m2 < n; ++m2) {
+                const double f = sf[m1 + n * m2];
+                acc.x += tz[m2].x * f;
+                acc.y += tz[m2].y * f;
+            }
+            ff[m1 * nh + k3] = acc;
+        }
+        __syncthreads();
+        double2* __restrict__ E = expn + ((size_t)sl * 2 * g.nd + ind) * g.nexp;
+        for (int o = tid; o < npw * nh; o += blockDim.x) {
+            const int k1 = o % npw, k3 = o / npw;
+            const double2* __restrict__ tx = sT + (size_t)k1 * n;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int m1 = 0; m1 < n; ++m1) acc = cfma(tx[m1], ff[m1 * nh + k3], acc);
+            E[o] = acc;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- STEP 2
@@ -452,6 +504,62 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
     }
 }
 
+// STEP 6 for 2-D (g2d_tens_prod_to_potloc_fast, bfgt_local.f:1230-1315): per source,
+// h[k1][j3] = sum_j1 Ax[k1][j1] f[j1][j3] over the x band, then
+// pot[k1][k3] += sum_j3 Az[k3][j3] h[k1][j3], each with one thread per
+// (k1, j3) / (k1, k3) pair; the n x n outputs stay in registers across the
+// target's sources.  Two barriers per source instead of two per source row.
+// LDS: sf [j3][j1] (n*n), h [j3][k1] (n*n), Ax, Az (n*n each), x band (2n).
+__global__ void __launch_bounds__(kBlock) k_direct2(Geom g, DevTables t, const int* __restrict__ tgt,
+                                                    const int* __restrict__ lev, const int* __restrict__ off,
+                                                    const int* __restrict__ src, const int* __restrict__ code,
+                                                    const double* __restrict__ fvals, Kinds kd, OutDesc od) {
+    extern __shared__ double dsm[];
+    const int task = blockIdx.x;
+    const int tb = tgt[task], l = lev[task];
+    const int n = t.n;
+    const size_t nn = (size_t)n * n;
+    double* sf = dsm;
+    double* h = sf + nn;
+    double* Ax = h + nn;
+    double* Az = Ax + nn;
+    int* band = (int*)(Az + nn);
+    const int* bandl = t.locband + (size_t)l * 13 * 2 * n;
+    const int tid = threadIdx.x;
+    const bool act = tid < (int)nn;
+    const int r = tid % n, c = tid / n;   // (k1, j3) in the first stage, (k1, k3) in the second
+    for (int ind = 0; ind < g.nd; ++ind) {
+        double acc = 0.0;
+        for (int j = off[task]; j < off[task + 1]; ++j) {
+            const int sb = src[j], cd = code[j];
+            const int slx = cd & 15, slz = (cd >> 8) & 15;
+            __syncthreads();
+            const double* __restrict__ A = t.loctab + kd.x * t.loctab_ks + ((size_t)l * 13 + slx) * nn;
+            const double* __restrict__ Z = t.loctab + kd.z * t.loctab_ks + ((size_t)l * 13 + slz) * nn;
+            if (act) {
+                sf[tid] = fvals[((size_t)sb * g.npbox + tid) * g.nd + ind];
+                Ax[tid] = A[tid];
+                Az[tid] = Z[tid];
+            }
+            if (tid < 2 * n) band[tid] = bandl[(size_t)slx * 2 * n + tid];
+            __syncthreads();
+            if (act) {
+                const int lo = band[2 * r], hi = band[2 * r + 1];
+                const double* __restrict__ fs = sf + (size_t)n * c;
+                double v = 0.0;
+                for (int j1 = lo; j1 <= hi; ++j1) v += Ax[r * n + j1] * fs[j1];
+                h[c * n + r] = v;
+            }
+            __syncthreads();
+            if (act) {
+                const double* __restrict__ az = Az + (size_t)c * n;
+                for (int j3 = 0; j3 < n; ++j3) acc += az[j3] * h[j3 * n + r];
+            }
+        }
+        if (act) od.p[((size_t)tb * g.npbox + tid) * od.ns + od.coff + ind] += acc;
+    }
+}
+
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) {
         char buf[256];
@@ -472,6 +580,14 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, c
         return;
     }
     if (t.expn_f32) throw std::runtime_error("FP32 expansions need the 3-D register-blocked kernels");
+    if (g.ndim == 2) {
+        const size_t lds2 = sizeof(double2) * ((size_t)t.npw * t.n + (size_t)t.n * g.pz) + sizeof(double) * t.n * t.n;
+        if (lds2 <= t.lds_limit) {
+            hipLaunchKernelGGL(k_p2pw2, dim3(ntask), dim3(kBlock), lds2, s, g, t, box, lev, slot, fvals, expn);
+            check(hipGetLastError(), "k_p2pw2 launch");
+            return;
+        }
+    }
     const size_t lds = sizeof(double2) * ((size_t)t.npw * t.n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
     hipLaunchKernelGGL(k_p2pw, dim3(ntask), dim3(kBlock), lds, s, g, t, box, lev, slot, fvals, expn);
     check(hipGetLastError(), "k_p2pw launch");
@@ -559,6 +675,13 @@ void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
     if (ntask <= 0) return;
     if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s)) {
         check(hipGetLastError(), "k_direct3 launch");
+        return;
+    }
+    if (g.ndim == 2) {
+        const size_t lds2 = sizeof(double) * 4 * (size_t)t.n * t.n + sizeof(int) * 2 * (size_t)t.n;
+        hipLaunchKernelGGL(k_direct2, dim3(ntask), dim3(kBlock), lds2, s, g, t, tgt, lev, off, src, code, fvals,
+                           kd, od);
+        check(hipGetLastError(), "k_direct2 launch");
         return;
     }
     const size_t lds = sizeof(double) * ((size_t)g.npbox + (size_t)g.nx * g.ny + 3 * (size_t)t.n * t.n) +
