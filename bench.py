@@ -252,7 +252,7 @@ def main():
     ap.add_argument("--no-drop-in", action="store_true", help="skip the end-to-end boxfgt_ call timing")
     ap.add_argument("--batch", type=int, default=128,
                     help="problems in the uneven-batch comparison (0 = skip)")
-    ap.add_argument("--serve-rates", default="1000,3000", help="request arrivals per second (comma list)")
+    ap.add_argument("--serve-rates", default="1000,3000,5000", help="request arrivals per second (comma list)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
@@ -330,16 +330,20 @@ def main():
     e2e = None
     if not args.no_drop_in:
         pot = np.zeros_like(tree.fvals)
-        best = None
+        best, split = None, None
         for _ in range(2):
             pot[...] = 0.0
+            ti = np.zeros(20)
             t0 = time.perf_counter()
             fgt_amd.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, cfg["iperiod"], n, tree.npbox, tree.nboxes,
                            tree.nlevels, tree.ltree, tree.itree, tree.iptr, tree.centers, tree.boxsize,
-                           tree.fvals, 1, pot)
+                           tree.fvals, 1, pot, timeinfo=ti)
             dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-        e2e = {"ms": round(best * 1e3, 2), "nodes_per_s": round(nout / best, 1),
+            if best is None or dt < best:
+                # timeinfo: [0] plan + fvals upload, [1..6] device stages, [7] download and +=
+                best, split = dt, {"plan_and_upload_ms": round(ti[0] * 1e3, 2),
+                                   "download_and_accumulate_ms": round(ti[7] * 1e3, 2)}
+        e2e = {"ms": round(best * 1e3, 2), "nodes_per_s": round(nout / best, 1), **split,
                "includes": "plan + H2D + STEPs 1-6 + D2H, one boxfgt_ call with host arrays, best of 2"}
         log(f"drop-in end to end: {best * 1e3:.1f} ms")
 
@@ -348,8 +352,9 @@ def main():
         probs = c4_problems(args.batch)
         batched = batched_comparison(probs, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
-        if args.serve_seconds > 0:
-            served = [serving(probs, float(r), args.serve_seconds, 4) for r in args.serve_rates.split(",")]
+        rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+        if args.serve_seconds > 0 and rates:
+            served = [serving(probs, r, args.serve_seconds, 4) for r in rates]
             log(f"served: {served}")
 
     cpu = None

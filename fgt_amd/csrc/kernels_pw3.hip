@@ -292,9 +292,12 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // LDS: sff [k2][s][16] (npw*64 complex), yres [s][kk1][16] (1024 complex,
 // entry kk2 at kk2 ^ kk1), sUr [v][s][kk3] (12*N complex: the round's z
 // table rows of the three variants, zero past nh).
-// MF: x stage on the FP64 matrix cores (N = 16, npw <= 64): per slab the
+// MF: x stage on the FP64 matrix cores (N >= 12, npw <= 64): per slab the
 // complex product ff[kk1][k2] = sum_k1 U[kk1][k1] E[k1][k2] runs as
-// 16x16x4 tiles (rows kk1, columns 16 k2, K = 4 k1).  The table is read per
+// 16x16x4 tiles (rows kk1, columns 16 k2, K = 4 k1; for N = 12 the rows
+// kk1 >= N carry zero table entries).  For N = 8 half of each tile would be
+// padding and the VALU form is faster (STEP 5 over 21 norder-8 problems:
+// 4.34 vs 5.10 ms; norder 12: 5.17 VALU vs 4.53 ms here).  The table is read per
 // lane from global memory (cached); E rows stream straight into the B
 // operand.  (For STEP 1 the matrix-core form is slower: its 50 x 50 output
 // pads to 64 x 64 tiles, where the VALU form wastes only the lane padding.)
@@ -359,6 +362,8 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                 if (k3 < nh) {
                     const ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     const int li = lane & 15, lk = lane >> 4;
+                    // rows kk1 >= N (N < 16) read a valid table entry and are zeroed
+                    const int lir = li < N ? li : N - 1;
                     dbl4 are[4], aim[4];
 #pragma unroll
                     for (int t = 0; t < 4; ++t) are[t] = aim[t] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -371,13 +376,13 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #define PW2POT_XLOAD(u_, e_, st)                                                             \
     {                                                                                        \
         const int k1c_ = min(4 * (st) + lk, npw - 1);                                        \
-        u_ = UX[k1c_ * N + li];                                                              \
+        u_ = UX[k1c_ * N + lir];                                                             \
         _Pragma("unroll") for (int t = 0; t < 4; ++t)                                        \
             e_[t] = ldE(Ek[(size_t)k1c_ * npw + min(16 * t + li, npw - 1)]);                 \
     }
 #define PW2POT_XSTEP(u_, e_, st)                                                             \
     {                                                                                        \
-        const double2 um_ = (4 * (st) + lk < npw) ? u_ : make_double2(0.0, 0.0);             \
+        const double2 um_ = (4 * (st) + lk < npw && li < N) ? u_ : make_double2(0.0, 0.0);   \
         _Pragma("unroll") for (int t = 0; t < 4; ++t) cmfma(um_, e_[t], are[t], aim[t]);     \
     }
                         PW2POT_XLOAD(ua, ea, 0);
@@ -399,8 +404,8 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                             const int k1 = 4 * st + lk;
                             const bool kv = k1 < npw;
                             const int k1c = kv ? k1 : npw - 1;
-                            double2 u = UX[k1c * N + li];
-                            if (!kv) u = make_double2(0.0, 0.0);
+                            double2 u = UX[k1c * N + lir];
+                            if (!kv || li >= N) u = make_double2(0.0, 0.0);
                             double2 e[4];
 #pragma unroll
                             for (int t = 0; t < 4; ++t) {
@@ -561,10 +566,13 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
-    if (N == 16 && t.npw <= 64 && !pw2pot_valu_only())
-        launch_pw2pot3_typed<N, true>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
-    else
-        launch_pw2pot3_typed<N, false>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+    if constexpr (N >= 12) {
+        if (t.npw <= 64 && !pw2pot_valu_only()) {
+            launch_pw2pot3_typed<N, true>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+            return true;
+        }
+    }
+    launch_pw2pot3_typed<N, false>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
     return true;
 }
 

@@ -87,3 +87,33 @@ def test_pgh_3d_vs_oracle_and_closed_form(iperiod, delta):
     s.execute()
     assert np.array_equal(s.get_pot(), pot)
     s.close()
+
+
+@pytest.mark.parametrize("norder,ipoly,iperiod,delta", [(12, 0, 0, 5e-4), (16, 1, 1, 1e-3)])
+def test_pgh_2d_three_densities_vs_oracle(norder, ipoly, iperiod, delta):
+    """2-D kernels (STEPs 1, 5, 6) with nd = 3 independent densities, all
+    components, against the oracle (no reference fixture has 2-D nd > 1)."""
+    print(f"[gpu] pgh 2-D nd=3 n={norder} ipoly={ipoly} iperiod={iperiod}", flush=True)
+    gauss = (np.array([[0.1, -0.2], [-0.3, 0.25]]), np.array([[2e-2, 1e-2], [8e-3, 1.5e-2]]),
+             np.array([1.0, 0.7]))
+    eps = 1e-9
+    tr = fgt_amd.vol_tree_build(2, norder, ipoly, iperiod, eps, gauss_source(*gauss))
+    rng = np.random.default_rng(7)
+    fv = np.stack([tr.fvals, -0.5 * tr.fvals, rng.uniform(-1.0, 1.0, tr.fvals.shape)], axis=-1)
+    s = fgt_amd.Solver.from_tree(tr, delta, eps, nd=3)
+    s.set_ifpgh(3)
+    s.set_fvals(fv)
+    s.execute()
+    pot, grad, hess = s.get_pot(), s.get_grad(), s.get_hess()
+    s.close()
+    rp, rg, rh, info = oracle.boxfgt_pgh(3, 2, delta, eps, ipoly, iperiod, norder, tr.nboxes, tr.nlevels,
+                                         tr.itree, tr.iptr, tr.centers, tr.boxsize, fv, 3)
+    assert 1 <= info.npwlevel < tr.nlevels
+    for d in range(3):
+        assert rel_l2(pot[..., d], rp[..., d]) < 1e-12, f"pot {d}"
+        for k in range(2):
+            assert rel_l2(grad[:, :, k, d], rg[:, :, k, d]) < 1e-12, f"grad {k} density {d}"
+        for k in range(3):
+            assert rel_l2(hess[:, :, k, d], rh[:, :, k, d]) < 1e-11, f"hess {k} density {d}"
+    # density 1 is -0.5 x density 0: the transform is linear per density
+    assert np.allclose(pot[..., 1], -0.5 * pot[..., 0], rtol=0, atol=1e-13 * np.abs(pot[..., 0]).max())
