@@ -876,6 +876,8 @@ int fgt_solver_set_ifpgh(fgt_solver* s, int ifpgh) {
     });
 }
 
+}  // extern "C"
+
 namespace {
 // Host <-> device transfers of the drop-in's host arrays go through a pinned
 // staging buffer (allocated once per process, grown on demand, two halves):
@@ -959,6 +961,8 @@ void upload_host(const DBuf& b, const double* in, const char* what) {
     hcheck(hipStreamSynchronize(g.st), what);
 }
 }  // namespace
+
+extern "C" {
 
 int fgt_solver_set_precision(fgt_solver* s, int mode) {
     return guarded([&] {
