@@ -716,6 +716,8 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
     P.locband.assign(static_cast<size_t>(2) * n * kLocSlots * (nl + 1), 0);
     P.loccols.assign(static_cast<size_t>(2) * kLocSlots * (nl + 1), 0);
     const int nlevend = std::min(npwlevel, nl);
+    // levels are independent (each writes its own slices)
+#pragma omp parallel for schedule(dynamic, 1)
     for (int l = 0; l <= nlevend; ++l)
         make_loctab_level(in.eps, in.ipoly, n, in.delta, T.boxsize[l], &P.loctab[nn * kLocSlots * l],
                           &P.locband[static_cast<size_t>(2) * n * kLocSlots * l], &P.loccols[2 * kLocSlots * l],
@@ -757,11 +759,22 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
             srcs.push_back(src);
             codes.push_back(code);
         };
-        std::vector<int> srcs, codes;
-        for (int l = 0; l <= nlevend; ++l) {
+        // boxes are independent: each fills its own lists (in parallel), and
+        // the lists are appended in level order, box order within a level
+        std::vector<int> order_box, order_lev;
+        for (int l = 0; l <= nlevend; ++l)
             for (int b = T.lfirst[l]; b <= T.llast[l]; ++b) {
-                srcs.clear();
-                codes.clear();
+                order_box.push_back(b);
+                order_lev.push_back(l);
+            }
+        const int nq = static_cast<int>(order_box.size());
+        std::vector<std::vector<int>> qsrc(nq), qcode(nq);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int q = 0; q < nq; ++q) {
+            const int b = order_box[q], l = order_lev[q];
+            std::vector<int>& srcs = qsrc[q];
+            std::vector<int>& codes = qcode[q];
+            {
                 if (l == 0) {
                     if (T.nchild[0] == 0) add_pair(0, 0, srcs, codes);
                 } else if (T.nchild[b] == 0 && !P.all_pw) {
@@ -792,14 +805,15 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
                         if (ok) add_pair(b, jb, srcs, codes);
                     }
                 }
-                if (!srcs.empty()) {
-                    P.dir_tgt.push_back(b);
-                    P.dir_lev.push_back(l);
-                    P.dir_off.push_back(static_cast<int>(P.dir_src.size()));
-                    P.dir_src.insert(P.dir_src.end(), srcs.begin(), srcs.end());
-                    P.dir_code.insert(P.dir_code.end(), codes.begin(), codes.end());
-                }
             }
+        }
+        for (int q = 0; q < nq; ++q) {
+            if (qsrc[q].empty()) continue;
+            P.dir_tgt.push_back(order_box[q]);
+            P.dir_lev.push_back(order_lev[q]);
+            P.dir_off.push_back(static_cast<int>(P.dir_src.size()));
+            P.dir_src.insert(P.dir_src.end(), qsrc[q].begin(), qsrc[q].end());
+            P.dir_code.insert(P.dir_code.end(), qcode[q].begin(), qcode[q].end());
         }
         P.dir_off.push_back(static_cast<int>(P.dir_src.size()));
         P.ndirect_pairs = static_cast<int64_t>(P.dir_src.size());
