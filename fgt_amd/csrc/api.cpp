@@ -179,11 +179,13 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     // device limits for the LDS-staged kernels
     int dev = 0;
     hcheck(hipGetDevice(&dev), "hipGetDevice");
-    hipDeviceProp_t prop;
-    hcheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    int lds_max = 0;   // one attribute query (the full property struct is slow to fill)
+    hcheck(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+           "hipDeviceGetAttribute");
+    const size_t lds_limit = static_cast<size_t>(lds_max);
     const size_t lds_p2pw = sizeof(double2) * ((size_t)P.npw * n + (size_t)g.nx * g.ny + (size_t)g.py * g.nx);
     const size_t lds_pw2pot = sizeof(double2) * ((size_t)n * P.npw + (size_t)g.nx * g.py);
-    if (P.do_pw && (lds_p2pw > prop.sharedMemPerBlock || lds_pw2pot > prop.sharedMemPerBlock))
+    if (P.do_pw && (lds_p2pw > lds_limit || lds_pw2pot > lds_limit))
         throw std::invalid_argument("plane-wave count too large for the LDS-staged kernels");
 
     S.d_fvals.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
@@ -212,7 +214,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     t.pw2potT = S.d_pw2potT.as<double2>();
     t.pw2pot_ks = P.pw2pot_ks;
     t.loctab_ks = P.loctab_ks;
-    t.lds_limit = prop.sharedMemPerBlock;
+    t.lds_limit = lds_limit;
     t.n = n;
     t.npw = P.npw;
     S.fast3d = P.do_pw && fgt::pw2pot_3d_supported(g, t);
