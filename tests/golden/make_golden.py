@@ -58,6 +58,7 @@ PGH_CASES = {
     "pgh2_per_n8":   (2, 8, 0, 1, 1e-6, 1e-6, 1e-3, 13, 4, (1e-3, 1e-2), 1, 3),
     "pgh2_n12":      (2, 12, 0, 0, 1e-6, 1e-9, 1e-4, 14, 3, (1e-3, 1e-2), 1, 3),
     "pgh1_n8":       (1, 8, 0, 0, 1e-9, 1e-6, 1e-3, 15, 3, (1e-4, 1e-2), 1, 3),
+    "pgh2_cheb_nd2_n16": (2, 16, 1, 1, 1e-6, 1e-9, 1e-3, 20, 3, (1e-3, 1e-2), 2, 3),
 }
 
 
