@@ -201,15 +201,25 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
 // ---------------------------------------------------------------- STEP 3
 // STEP 3, grouped: a block owns one element range (one element per thread)
 // and kM2lGroup consecutive targets (Morton neighbours, so their sources
-// repeat and hit in L2).  Each thread forms the nine products w_x w_y and
+// repeat and hit in L2).  Each thread forms the w_x w_y products and the
 // three w_z of its element once and reuses them for every list entry of the
-// group.  The group's entries (each target's own expansion first, then its
+// group.  The 1-D phases are conj(z), 1, z for offsets 0, 1, 2, so the nine
+// products w_x w_y are a b, a conj(b), a, b, 1 (a = z[k1], b = z[k2]) or
+// their conjugates: five LDS rows, and a conjugate is an (exact) sign flip
+// of the imaginary part -- 20 KB of LDS per block instead of 36 KB, six
+// blocks per CU instead of four.  The group's entries (each target's own expansion first, then its
 // list) form one stream; source loads go kM2lBatch at a time into three
 // register batches, the batch two ahead issued before the current one is
 // applied, so a wave keeps loads in flight across its whole stream.  The
 // entry's offset o = ox + 3 oy + 9 oz is wave-uniform: it picks the w_x w_y
-// row in LDS and w_z by a uniform select (a switch on o, compiled to a
-// binary search of scalar branches, cost ~10 branches per entry).
+// row in LDS and its conjugation, and w_z by a uniform select (no branches).
+
+// (ox, oy) -> row of s_fxy, 3 bits each at oxy = ox + 3 oy:
+//   (0,0) ab*  (1,0) b*  (2,0) (a* b)*  (0,1) a*  (1,1) 1  (2,1) a  (0,2) a* b  (1,2) b  (2,2) ab
+constexpr unsigned kM2lRow = (0u << 0) | (3u << 3) | (1u << 6) | (2u << 9) | (4u << 12) | (2u << 15) |
+                             (1u << 18) | (3u << 21) | (0u << 24);
+// conjugated at oxy = 0, 1, 2 (oy = 0) and 3 ((ox, oy) = (0, 1))
+constexpr unsigned kM2lConj = 0x0fu;
 
 template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, int group, long long zero_off,
@@ -228,17 +238,19 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
     // w_x w_y of the element in LDS (one column per thread, read at a
-    // wave-uniform row), w_z in registers picked by a uniform select
-    __shared__ double2 s_fxy[9][kBlock];
+    // wave-uniform row), w_z in registers picked by a uniform select.
+    // Rows: 0 a b, 1 conj(a) b, 2 a, 3 b, 4 one; per (ox, oy) the row is
+    // kM2lRow and the conjugation kM2lConj (a = w_x(2), b = w_y(2)).
+    __shared__ double2 s_fxy[5][kBlock];
     const int tid = threadIdx.x;
-#pragma unroll
-    for (int oy = 0; oy < 3; ++oy) {
-        const double2 wy = g.py > 1 ? t.wm2l[oy * npw + k2] : make_double2(1.0, 0.0);
-#pragma unroll
-        for (int ox = 0; ox < 3; ++ox) {
-            const double2 wx = g.px > 1 ? t.wm2l[ox * npw + k1] : make_double2(1.0, 0.0);
-            s_fxy[ox + 3 * oy][tid] = cmul(wx, wy);
-        }
+    {
+        const double2 a = g.px > 1 ? t.wm2l[2 * npw + k1] : make_double2(1.0, 0.0);
+        const double2 b = g.py > 1 ? t.wm2l[2 * npw + k2] : make_double2(1.0, 0.0);
+        s_fxy[0][tid] = cmul(a, b);
+        s_fxy[1][tid] = cmul(conjd(a), b);
+        s_fxy[2][tid] = a;
+        s_fxy[3][tid] = b;
+        s_fxy[4][tid] = make_double2(1.0, 0.0);
     }
     const double2 fz0 = t.wm2l[k3], fz1 = t.wm2l[npw + k3], fz2 = t.wm2l[2 * npw + k3];
     const ET* __restrict__ eb = expn + e;
@@ -264,9 +276,12 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
 #define M2L_APPLY(v, j)                                                                  \
     _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
         const int code = __builtin_amdgcn_readfirstlane(s_code[(j) + u]);                \
-        const int o = code & 31, oz = o / 9;                                             \
+        const int o = code & 31, oz = o / 9, oxy = o - 9 * oz;                           \
         const double2 zf = oz == 0 ? fz0 : (oz == 1 ? fz1 : fz2);                        \
-        acc = cfma(cmul(v[u], s_fxy[o - 9 * oz][tid]), zf, acc);                         \
+        double2 fxy = s_fxy[(kM2lRow >> (3 * oxy)) & 7][tid];                            \
+        fxy.y = __longlong_as_double(__double_as_longlong(fxy.y) ^                       \
+                                     ((long long)((kM2lConj >> oxy) & 1) << 63));        \
+        acc = cfma(cmul(v[u], fxy), zf, acc);                                            \
         if (code >> 5) {                                                                 \
             if (ok) stE(expn + to[ti] + tot + e, acc);                                   \
             ++ti;                                                                        \
