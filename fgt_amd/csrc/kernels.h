@@ -134,7 +134,7 @@ void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_s
 // points at an all-zero slot, whose element offset is zero_off); scode[j] =
 // o | last << 5 with o = ox + 3 oy + 9 oz in 0..26 and last set on each
 // target's final entry.
-constexpr int kM2lGroup = 8;
+constexpr int kM2lGroup = 4;   // best at 4-5 with six blocks per CU (8 was best at four)
 constexpr int kM2lGroupMax = 12;
 // targets per STEP 3 block: kM2lGroup, or FGT_M2L_GROUP (1..kM2lGroupMax,
 // tuning runs)
