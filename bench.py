@@ -118,6 +118,52 @@ def c4_problems(nprob):
     return probs
 
 
+def mixed_sweep(steps):
+    """Mixed-precision form on the single-problem generator (3-D periodic,
+    20 anisotropic Gaussians, seed 1) at norder 12: eps in {1e-3, 1e-6} x
+    delta in {1e-2, 1e-3, 1e-4}.  Each point times the FP64 path and the
+    FP32-expansion path on the same tree and states the mixed result's
+    relative l2 and max error against the FP64 result (which matches the
+    reference to ~1e-15)."""
+    import torch
+    import fgt_amd
+    rows = []
+    for eps in (1e-3, 1e-6):
+        tree, _ = make_workload(n=12, eps=eps)
+        for delta in (1e-2, 1e-3, 1e-4):
+            sv = fgt_amd.Solver.from_tree(tree, delta, eps)
+            sv.set_fvals(tree.fvals)
+            nout = len(tree.leaves) * tree.npbox
+            res = {"eps": eps, "delta": delta, "output_nodes": int(nout)}
+            for mode in (0, 1):
+                try:
+                    sv.set_precision(mode)
+                except fgt_amd.FgtError as ex:
+                    res["mixed"] = f"not available: {ex}"
+                    break
+                sv.execute()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    sv.execute()
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) / steps * 1e3
+                pot = sv.get_pot()[tree.leaves]
+                if mode == 0:
+                    ref = pot
+                    res["fp64"] = {"ms": round(ms, 3), "nodes_per_s": round(nout / ms * 1e3, 1)}
+                else:
+                    den = max(np.linalg.norm(ref), 1e-300)
+                    res["mixed"] = {"ms": round(ms, 3), "nodes_per_s": round(nout / ms * 1e3, 1),
+                                    "speedup": round(res["fp64"]["ms"] / ms, 3),
+                                    "rel_l2_vs_fp64": float(f"{np.linalg.norm(pot - ref) / den:.3g}"),
+                                    "max_err_vs_fp64_rel_max": float(
+                                        f"{np.abs(pot - ref).max() / max(np.abs(ref).max(), 1e-300):.3g}")}
+            sv.close()
+            rows.append(res)
+    return rows
+
+
 def batched_comparison(probs, nstreams, steps):
     """All problems executed overlapped on nstreams streams (fgt_batch) vs
     one at a time (device-resident solver per problem, synchronised after
@@ -254,6 +300,8 @@ def main():
                     help="problems in the uneven-batch comparison (0 = skip)")
     ap.add_argument("--serve-rates", default="1000,3000,5000", help="request arrivals per second (comma list)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
+    ap.add_argument("--no-mixed-sweep", action="store_true",
+                    help="skip the norder-12 eps x delta mixed-precision sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
     args = ap.parse_args()
@@ -347,6 +395,10 @@ def main():
                "includes": "plan + H2D + STEPs 1-6 + D2H, one boxfgt_ call with host arrays, best of 2"}
         log(f"drop-in end to end: {best * 1e3:.1f} ms")
 
+    sweep = None
+    if not args.no_mixed_sweep:
+        sweep = mixed_sweep(max(1, min(args.steps, 5)))
+        log(f"mixed sweep: {sweep}")
     batched = served = None
     if args.batch > 0:
         probs = c4_problems(args.batch)
@@ -386,6 +438,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "drop_in_end_to_end": e2e,
+        "mixed_precision_sweep": sweep,
         "batched": batched,
         "served": served,
     }
