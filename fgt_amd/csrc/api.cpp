@@ -970,7 +970,9 @@ int fgt_solver_set_precision(fgt_solver* s, int mode) {
     return guarded([&] {
         if (!s) throw std::invalid_argument("null solver");
         if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
-        if (mode == 1 && !(s->plan.do_pw && s->fast3d))
+        if (mode == 1 && !s->plan.do_pw)
+            throw std::invalid_argument("mixed precision: this plan forms no plane-wave expansions (near field only)");
+        if (mode == 1 && !s->fast3d)
             throw std::invalid_argument("mixed precision needs the 3-D register-blocked path (ndim 3, norder 8/12/16)");
         if (mode == s->tabs.expn_f32) return;
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
