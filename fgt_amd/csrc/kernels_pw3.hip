@@ -10,7 +10,7 @@
 // wave index), which lets it come from the scalar cache into SGPRs; the other
 // operand lives in VGPRs, so each v_fma_f64 has one scalar and one vector
 // source and LDS only carries the transposes between stages.  STEP 5's x
-// contraction for N = 16 runs on the FP64 matrix cores instead.  The half
+// contraction for N = 12 and 16 runs on the FP64 matrix cores instead.  The half
 // (z) axis is processed in rounds of four slabs, one per wave.
 #include "kernels.h"
 
