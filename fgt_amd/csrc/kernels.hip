@@ -15,6 +15,7 @@
 //   k_direct2 STEP 6 for 2-D                      (g2d_tens_prod_to_potloc_fast bfgt_local.f:1230-1315)
 #include "kernels.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -24,6 +25,8 @@ namespace fgt {
 namespace {
 
 constexpr int kBlock = 256;
+// rows of grid y per launch (HIP's y dimension is limited to 65535)
+constexpr int kMaxGridY = 65535;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -611,15 +614,20 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, c
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child, const int* pc_sign,
                   const int* par_acc, int klev, double2* expn, hipStream_t s) {
-    if (ntask <= 0) return;
-    const dim3 grid(blocks_for((long long)g.nd * g.nexp), ntask);
-    if (t.expn_f32)
-        hipLaunchKernelGGL(k_merge<float2>, grid, dim3(kBlock), 0, s, g, t, par_slot, par_first, par_count,
-                           pc_child, pc_sign, par_acc, klev, reinterpret_cast<float2*>(expn));
-    else
-        hipLaunchKernelGGL(k_merge<double2>, grid, dim3(kBlock), 0, s, g, t, par_slot, par_first, par_count,
-                           pc_child, pc_sign, par_acc, klev, expn);
-    check(hipGetLastError(), "k_merge launch");
+    // tasks go on grid y, which holds at most kMaxGridY rows: longer lists
+    // are launched in chunks
+    for (int a = 0; a < ntask; a += kMaxGridY) {
+        const int cnt = std::min(kMaxGridY, ntask - a);
+        const dim3 grid(blocks_for((long long)g.nd * g.nexp), cnt);
+        const int* acc = par_acc ? par_acc + a : nullptr;
+        if (t.expn_f32)
+            hipLaunchKernelGGL(k_merge<float2>, grid, dim3(kBlock), 0, s, g, t, par_slot + a, par_first + a,
+                               par_count + a, pc_child, pc_sign, acc, klev, reinterpret_cast<float2*>(expn));
+        else
+            hipLaunchKernelGGL(k_merge<double2>, grid, dim3(kBlock), 0, s, g, t, par_slot + a, par_first + a,
+                               par_count + a, pc_child, pc_sign, acc, klev, expn);
+        check(hipGetLastError(), "k_merge launch");
+    }
 }
 
 int m2l_group() {
@@ -648,15 +656,17 @@ void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long
 
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s) {
-    if (nentry <= 0) return;
-    const dim3 grid(blocks_for((long long)g.nd * g.nexp), nentry);
-    if (t.expn_f32)
-        hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, pc_parent, pc_child, pc_sign, klev,
-                           reinterpret_cast<float2*>(expn));
-    else
-        hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, pc_parent, pc_child, pc_sign, klev,
-                           expn);
-    check(hipGetLastError(), "k_split launch");
+    for (int a = 0; a < nentry; a += kMaxGridY) {
+        const int cnt = std::min(kMaxGridY, nentry - a);
+        const dim3 grid(blocks_for((long long)g.nd * g.nexp), cnt);
+        if (t.expn_f32)
+            hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
+                               pc_sign + a, klev, reinterpret_cast<float2*>(expn));
+        else
+            hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
+                               pc_sign + a, klev, expn);
+        check(hipGetLastError(), "k_split launch");
+    }
 }
 
 void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,

@@ -601,6 +601,23 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
         for (int j = 0; j < T.nchild[b]; ++j)
             if (T.child[static_cast<size_t>(mc) * b + j] < 0) { set_err(err, "missing child id"); return 2; }
     }
+    // levels agree with laddr; every box below the root has an in-range
+    // parent one level up, and every child sits one level down
+    for (int l = 0; l <= nl; ++l)
+        for (int b = T.lfirst[l]; b >= 0 && b <= T.llast[l]; ++b)
+            if (T.level[b] != l) { set_err(err, "box level disagrees with laddr"); return 2; }
+    for (int b = 0; b < nb; ++b) {
+        const int l = T.level[b];
+        if (b < T.lfirst[l] || b > T.llast[l]) { set_err(err, "box outside its level's laddr range"); return 2; }
+        if (l >= 1) {
+            const int p = T.parent[b];
+            if (p < 0 || p >= nb || T.level[p] != l - 1) { set_err(err, "invalid iparent entry"); return 2; }
+        }
+        for (int j = 0; j < T.nchild[b]; ++j)
+            if (T.level[T.child[static_cast<size_t>(mc) * b + j]] != l + 1) {
+                set_err(err, "child not one level below its parent"); return 2;
+            }
+    }
     T.centers.assign(in.centers, in.centers + static_cast<size_t>(ndim) * nb);
     T.boxsize.assign(in.boxsize, in.boxsize + nl + 1);
 

@@ -216,3 +216,32 @@ def test_corrupt_tree_is_rejected():
     rc, _ = _call_boxfgt(c)
     assert rc == 2
     assert "iptr" in fgt_amd.last_status()[1]
+
+
+@pytest.mark.parametrize("what", ["parent_out_of_range", "parent_wrong_level", "level_vs_laddr"])
+def test_corrupt_parent_or_level_is_rejected(what):
+    """A leaf's parent id out of range (or one level off), or a level entry
+    that disagrees with laddr, is refused with status 2 before any list is
+    built (an out-of-range parent used to be read as an index)."""
+    c = load_case("c1_small")
+    it = c.itree.copy()
+    nlev = c.nlevels
+    ilevel = c.iptr[1] - 1
+    iparent = c.iptr[2] - 1
+    nchild = c.iptr[3] - 1
+    lev = it[ilevel:ilevel + c.nboxes]
+    leaves = [b for b in range(c.nboxes) if it[nchild + b] == 0 and lev[b] >= 2]
+    b = leaves[len(leaves) // 2]
+    if what == "parent_out_of_range":
+        it[iparent + b] = 100000000
+    elif what == "parent_wrong_level":
+        it[iparent + b] = 1          # the root, two or more levels up
+    else:
+        it[ilevel + b] = (lev[b] + 1) % (nlev + 1)
+    c.itree = it
+    with pytest.raises(fgt_amd.FgtError):
+        fgt_amd.plan_summary(c, c.delta, c.eps)
+    rc, _ = _call_boxfgt(c)
+    assert rc == 2
+    msg = fgt_amd.last_status()[1]
+    assert "iparent" in msg or "level" in msg
