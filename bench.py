@@ -1,13 +1,20 @@
 """Benchmark: box Gauss transform throughput (output nodes/s) on one MI355X.
 
-Workload (default): 3-D periodic box FGT at FP64, Legendre nodes, norder=16,
-eps=1e-9, delta=1e-4; source = 20 anisotropic Gaussians with seeded centres
-in the unit box, per-axis widths log-uniform in [5e-4, 5e-3], amplitudes
-uniform in [-1, 1]; adaptive tree refined to eps by the library's builder
-(same arrays as the reference's vol_tree_build).  A "step" is one full
-transform (STEPs 1-6) of the resident tree: fvals are in HBM and the
-potential is written to HBM inside the timed region; the plan (tables,
-lists) is built once before it.
+Headline workload (`value`): the eps = 1e-6, 3-D slice of the first 512
+draws of the uneven-batch generator (norder 8/12/16, delta log-uniform in
+[1e-5, 1e-1], 1-10 seeded Gaussians, free space, trees refined to eps by the
+library's builder -- the same arrays as the reference's vol_tree_build).  A
+"step" is one batched, mixed-precision transform (STEPs 1-6) of every
+problem with fvals resident in HBM and the potentials written to HBM; the
+plans (tables, lists) are built once before the timed region.  Beside it,
+in the same run: the FP64 batch, the device-resident FP64 solver one
+problem at a time and the plain FP64 boxfgt_ drop-in one problem at a time,
+and the error of the mixed result against FP64.
+
+Secondary legs: the single-problem plain FP64 workload (3-D periodic,
+norder 16, eps 1e-9, delta 1e-4, 20 anisotropic Gaussians) with the
+roofline of its dominant kernel, and the norder-12 eps x delta
+mixed-precision sweep.
 
 Prints ONE JSON line on stdout (everything else goes to stderr).
 """
@@ -101,21 +108,136 @@ def c4_problem(i, base=1000):
 
 
 def c4_problems(nprob):
-    """nprob problems from the uneven generator; trees above 8000 leaves are
-    skipped, keeping leaf counts in the 50-8000 range."""
+    """The first nprob draws of the uneven generator, every draw kept."""
     import fgt_amd
     from fgt_amd.tree import gauss_source
     t0 = time.time()
-    probs, i = [], 0
-    while len(probs) < nprob:
+    probs = []
+    for i in range(nprob):
         ndim, n, eps, delta, g = c4_problem(i)
-        i += 1
         tr = fgt_amd.vol_tree_build(ndim, n, 0, 0, eps, gauss_source(*g))
-        if len(tr.leaves) > 8000:
-            continue
         probs.append((tr, delta, eps))
-    log(f"batch: {nprob} problems ({i} drawn) built in {time.time() - t0:.1f}s")
+    log(f"batch: {nprob} problems built in {time.time() - t0:.1f}s")
     return probs
+
+
+def headline_problems(ndraw=512):
+    """The eps = 1e-6, 3-D slice of the first ndraw draws of the uneven
+    generator (norder 8/12/16, delta log-uniform in [1e-5, 1e-1], 1-10
+    Gaussians, free space): every such draw, none skipped."""
+    import fgt_amd
+    from fgt_amd.tree import gauss_source
+    t0 = time.time()
+    probs = []
+    for i in range(ndraw):
+        ndim, n, eps, delta, g = c4_problem(i)
+        if ndim != 3 or eps != 1e-6:
+            continue
+        tr = fgt_amd.vol_tree_build(ndim, n, 0, 0, eps, gauss_source(*g))
+        probs.append((tr, delta, eps))
+    log(f"headline: {len(probs)} problems (eps 1e-6, 3-D, of {ndraw} draws) built in {time.time() - t0:.1f}s")
+    return probs
+
+
+def pot_error(pot, ref):
+    den = np.linalg.norm(ref)
+    l2 = np.linalg.norm(pot - ref) / (den if den > 0 else 1.0)
+    mx = np.abs(pot - ref).max() / max(np.abs(ref).max(), 1e-300)
+    return l2, mx
+
+
+def headline(probs, steps, warmup, mode, drop_in=True):
+    """North-star comparison on one problem set.
+
+    value: the batched mixed-precision form (precision `mode` for every
+    problem whose eps allows it; fvals resident in HBM, potential written to
+    HBM); baselines in the same run: the plain FP64 boxfgt_ drop-in one
+    problem at a time (host arrays: plan + H2D + STEPs 1-6 + D2H per call)
+    and the device-resident FP64 solver one problem at a time (synchronised
+    after each).  Errors are stated against the FP64 result of each problem.
+    """
+    import torch
+    import fgt_amd
+    nprob = len(probs)
+    nout = sum(len(t.leaves) * t.npbox for t, _, _ in probs)
+    b = fgt_amd.Batch(probs, nstreams=4)
+    for k, (tr, _, _) in enumerate(probs):
+        b.set_fvals(k, tr.fvals)
+    # FP64 batch: reference values and the batched FP64 rate
+    for _ in range(max(1, warmup)):
+        b.execute()
+    t0 = time.perf_counter()
+    for _ in range(max(1, min(steps, 5))):
+        b.execute()
+    t_b64 = (time.perf_counter() - t0) / max(1, min(steps, 5))
+    ref = [b.get_pot(k)[tr.leaves] for k, (tr, _, _) in enumerate(probs)]
+    nmixed = b.set_precision(mode)
+    for _ in range(warmup):
+        b.execute()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.execute()
+    torch.cuda.synchronize()
+    t_mix = (time.perf_counter() - t0) / steps
+    worst_l2 = worst_max = 0.0
+    for k, (tr, _, eps) in enumerate(probs):
+        l2, mx = pot_error(b.get_pot(k)[tr.leaves], ref[k])
+        worst_l2 = max(worst_l2, l2 / eps)
+        worst_max = max(worst_max, mx / eps)
+    b.close()
+    # device-resident FP64 solver, one problem at a time
+    solvers = []
+    for tr, delta, eps in probs:
+        sv = fgt_amd.Solver.from_tree(tr, delta, eps)
+        sv.set_fvals(tr.fvals)
+        solvers.append(sv)
+    for sv in solvers:
+        sv.execute()
+    torch.cuda.synchronize()
+    npass = 2
+    t0 = time.perf_counter()
+    for _ in range(npass):
+        for sv in solvers:
+            sv.execute()
+            torch.cuda.synchronize()
+    t_one = (time.perf_counter() - t0) / npass
+    for sv in solvers:
+        sv.close()
+    # the plain FP64 drop-in as a Fortran caller sees it, one problem at a time
+    t_drop = None
+    if drop_in:
+        pots = [np.zeros_like(tr.fvals) for tr, _, _ in probs]
+        tr, delta, eps = probs[0]
+        fgt_amd.boxfgt(1, 3, delta, eps, 0, 0, tr.norder, tr.npbox, tr.nboxes, tr.nlevels, tr.ltree, tr.itree,
+                       tr.iptr, tr.centers, tr.boxsize, tr.fvals, 1, np.zeros_like(tr.fvals))
+        t0 = time.perf_counter()
+        for k, (tr, delta, eps) in enumerate(probs):
+            fgt_amd.boxfgt(1, 3, delta, eps, 0, 0, tr.norder, tr.npbox, tr.nboxes, tr.nlevels, tr.ltree,
+                           tr.itree, tr.iptr, tr.centers, tr.boxsize, tr.fvals, 1, pots[k])
+        t_drop = time.perf_counter() - t0
+        for k, (tr, _, _) in enumerate(probs):
+            l2, _ = pot_error(pots[k][tr.leaves], ref[k])
+            if l2 > 1e-12:
+                raise AssertionError(f"drop-in and batch FP64 disagree on problem {k}: {l2:.3e}")
+    lv = [len(t.leaves) for t, _, _ in probs]
+    out = {
+        "problems": nprob, "output_nodes": int(nout), "leaves_min": int(min(lv)), "leaves_max": int(max(lv)),
+        "norders": sorted({int(t.norder) for t, _, _ in probs}),
+        "ms_batched_mixed": round(t_mix * 1e3, 3), "nodes_per_s_batched_mixed": round(nout / t_mix, 1),
+        "problems_mixed": int(nmixed), "precision_mode": mode,
+        "max_rel_l2_err_over_eps": float(f"{worst_l2:.3g}"), "max_err_over_eps": float(f"{worst_max:.3g}"),
+        "error_reference": "FP64 batch on the same trees (equal to the reference boxfgt to ~1e-15)",
+        "ms_batched_fp64": round(t_b64 * 1e3, 3), "nodes_per_s_batched_fp64": round(nout / t_b64, 1),
+        "ms_device_one_at_a_time_fp64": round(t_one * 1e3, 3),
+        "nodes_per_s_device_one_at_a_time_fp64": round(nout / t_one, 1),
+        "speedup_vs_device_one_at_a_time_fp64": round(t_one / t_mix, 3),
+    }
+    if t_drop is not None:
+        out.update({"ms_drop_in_one_at_a_time_fp64": round(t_drop * 1e3, 3),
+                    "nodes_per_s_drop_in_one_at_a_time_fp64": round(nout / t_drop, 1),
+                    "speedup_vs_drop_in_one_at_a_time_fp64": round(t_drop / t_mix, 3)})
+    return out
 
 
 def mixed_sweep(steps):
@@ -247,74 +369,71 @@ def serving(probs, rate, seconds, nstreams):
     return out
 
 
-def cpu_baseline(tree, cfg):
-    """Reference boxfgt (oracle/_ref, OpenMP) on the same tree and fvals."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def host_cpus():
     try:
-        from oracle import refrun
-        if not refrun.available():
-            raise RuntimeError("reference build missing")
+        return len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(probs, budget_s=15.0):
+    """The reference boxfgt (oracle/_ref, OpenMP FP64) on the GPU box's host,
+    over the leading problems of `probs` (same trees and fvals) until about
+    budget_s seconds of CPU work; problems whose cutoff level is <= 0 are
+    skipped (the reference returns zeros there without computing anything).
+    Falls back to the C restatement when the reference build is absent."""
+    import fgt_amd
+    threads = int(os.environ.get("OMP_NUM_THREADS", host_cpus()))
+    from oracle import refrun
+    use_ref = refrun.available()
+    if use_ref:
         refrun.lib().refglue_silence()
-        kind = "reference"
-
-        class T:
-            pass
-        rt = T()
-        for k in ("ndim", "norder", "ipoly", "iperiod", "nboxes", "nlevels", "ltree", "itree", "iptr",
-                  "centers", "boxsize", "fvals"):
-            setattr(rt, k, getattr(tree, k))
-
-        def run():
-            return refrun.boxfgt(rt, cfg["delta"], cfg["eps"])
-    except Exception as e:  # pragma: no cover - fallback when the reference .so is absent
-        log(f"cpu baseline: reference unavailable ({e}); timing the C restatement")
+    else:
+        log("cpu baseline: reference build missing; timing the C restatement")
         from oracle import oracle
-        kind = "port"
+    kind = "reference" if use_ref else "port"
 
-        def run():
-            return oracle.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, tree.iperiod, tree.norder, tree.nboxes,
-                                 tree.nlevels, tree.itree, tree.iptr, tree.centers, tree.boxsize, tree.fvals)
-    # keep the Fortran runtime's stdout off our JSON line
+    class T:
+        pass
     saved = os.dup(1)
-    os.dup2(2, 1)
+    os.dup2(2, 1)   # keep the Fortran runtime's stdout off the JSON line
+    done_nodes, secs, used = 0, 0.0, 0
     try:
-        times = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            run()
-            times.append(time.perf_counter() - t0)
+        for tr, delta, eps in probs:
+            if fgt_amd.plan_summary(tr, delta, eps)["npwlevel"] <= 0:
+                continue
+            if use_ref:
+                rt = T()
+                for k in ("ndim", "norder", "ipoly", "iperiod", "nboxes", "nlevels", "ltree", "itree", "iptr",
+                          "centers", "boxsize", "fvals"):
+                    setattr(rt, k, getattr(tr, k))
+                t0 = time.perf_counter()
+                refrun.boxfgt(rt, delta, eps)
+            else:
+                t0 = time.perf_counter()
+                oracle.boxfgt(1, tr.ndim, delta, eps, 0, tr.iperiod, tr.norder, tr.nboxes, tr.nlevels, tr.itree,
+                              tr.iptr, tr.centers, tr.boxsize, tr.fvals)
+            secs += time.perf_counter() - t0
+            done_nodes += len(tr.leaves) * tr.npbox
+            used += 1
+            if secs >= budget_s:
+                break
     finally:
         os.dup2(saved, 1)
         os.close(saved)
-    return min(times), kind, threads
+    return {"value": round(done_nodes / secs, 1), "unit": "output nodes/s", "cores": threads, "kind": kind,
+            "host_cpus": host_cpus(),
+            "sample": f"the first {used} problems of the headline set with a cutoff level >= 1 "
+                      f"({done_nodes} output nodes), one boxfgt call each, {secs:.1f}s; "
+                      f"OpenMP threads = OMP_NUM_THREADS"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-drop-in", action="store_true", help="skip the end-to-end boxfgt_ call timing")
-    ap.add_argument("--batch", type=int, default=128,
-                    help="problems in the uneven-batch comparison (0 = skip)")
-    ap.add_argument("--serve-rates", default="1000,3000,5000", help="request arrivals per second (comma list)")
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
-    ap.add_argument("--no-mixed-sweep", action="store_true",
-                    help="skip the norder-12 eps x delta mixed-precision sweep")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="after the run, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              f"sharding of the box FGT is design-only in this build.", file=sys.stderr)
-        sys.exit(2)
-
+def c2_leg(args):
+    """Plain FP64 single problem (3-D periodic, norder 16, eps 1e-9,
+    delta 1e-4): device-resident step time, per-stage HIP-event times and
+    the roofline of its dominant kernel."""
     import torch
     import fgt_amd
-    assert torch.cuda.is_available(), "no GPU"
-    torch.cuda.init()
-
     tree, cfg = make_workload()
     n = cfg["n"]
     t0 = time.time()
@@ -322,8 +441,7 @@ def main():
                        tree.ltree, tree.itree, tree.iptr, tree.centers, tree.boxsize)
     s.set_fvals(tree.fvals)
     torch.cuda.synchronize()
-    log(f"plan+upload {time.time() - t0:.2f}s; stats {s.stats()}")
-
+    log(f"c2 plan+upload {time.time() - t0:.2f}s; stats {s.stats()}")
     for _ in range(args.warmup):
         s.execute()
     torch.cuda.synchronize()
@@ -331,19 +449,15 @@ def main():
     for _ in range(args.steps):
         s.execute()
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = elapsed / args.steps * 1e3
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
     st = s.stats()
     nout = st["nout"]
-    value = nout / (ms * 1e-3)
-
     # per-stage device time (HIP events on the launch stream), one more step
     s.execute(timed=True)
     st2 = s.stats()
-    stage_ms = st2["stage_ms"]
     names = ["p2pw", "merge", "m2l", "split", "pw2pot", "direct"]
-    stage = dict(zip(names, stage_ms[:6]))
-    log("stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage.items()))
+    stage = dict(zip(names, st2["stage_ms"][:6]))
+    log("c2 stage ms: " + ", ".join(f"{k}={v:.3f}" for k, v in stage.items()))
     model = stage_model(st2, n)
     dom = max(model, key=lambda k: stage[k])
     m = model[dom]
@@ -357,25 +471,17 @@ def main():
         t = stage[k] * 1e-3
         stages[k] = {"kernel": v["kernel"], "ms": round(stage[k], 4),
                      "tflops": round(v["flops"] / t / 1e12, 3), "gbs": round(v["bytes"] / t / 1e9, 1)}
-    roofline = {"kernel": m["kernel"], "bound": m["bound"], "achieved": round(achieved, 3), "peak": peak,
-                "unit": unit, "frac": round(achieved / peak, 4), "traffic": measured_traffic(m["kernel"]),
-                "algorithmic_bytes": m["bytes"], "algorithmic_flops": m["flops"],
-                "stage_ms": {k: round(v, 4) for k, v in stage.items()}, "stages": stages}
-
-    if args.copy_calibration:
-        # plain device copy of a known size: the scale for the PMC byte counters
-        import torch
-        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 3 x 1 GiB device copies done")
-
-    # the drop-in as a host caller sees it: one boxfgt_ call with host
-    # arrays = plan (host tables and lists) + H2D + STEPs 1-6 + D2H; best of 2
-    e2e = None
+    roofline = {"kernel": m["kernel"], "workload": "c2_fp64 leg", "bound": m["bound"],
+                "achieved": round(achieved, 3), "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
+                "traffic": measured_traffic(m["kernel"]), "algorithmic_bytes": m["bytes"],
+                "algorithmic_flops": m["flops"], "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+                "stages": stages}
+    leg = {"workload": "3-D periodic boxfgt, Legendre, norder=16, eps=1e-9, delta=1e-4, 20 anisotropic "
+                       "Gaussians (seed 1), plain FP64, device-resident",
+           "ms_per_step": round(ms, 4), "nodes_per_s": round(nout / ms * 1e3, 1), "steps": args.steps,
+           "nboxes": tree.nboxes, "nleaves": int(st["nleaves"]), "nlevels": tree.nlevels, "output_nodes": nout,
+           "npwlevel": st["npwlevel"], "npw": st["npw"], "pw_leaves": int(st["npw_leaves"]),
+           "direct_pairs": int(st["ndirect_pairs"])}
     if not args.no_drop_in:
         pot = np.zeros_like(tree.fvals)
         best, split = None, None
@@ -388,62 +494,115 @@ def main():
                            tree.fvals, 1, pot, timeinfo=ti)
             dt = time.perf_counter() - t0
             if best is None or dt < best:
-                # timeinfo: [0] plan + fvals upload, [1..6] device stages, [7] download and +=
                 best, split = dt, {"plan_and_upload_ms": round(ti[0] * 1e3, 2),
                                    "download_and_accumulate_ms": round(ti[7] * 1e3, 2)}
-        e2e = {"ms": round(best * 1e3, 2), "nodes_per_s": round(nout / best, 1), **split,
-               "includes": "plan + H2D + STEPs 1-6 + D2H, one boxfgt_ call with host arrays, best of 2"}
-        log(f"drop-in end to end: {best * 1e3:.1f} ms")
+        leg["drop_in_end_to_end"] = {"ms": round(best * 1e3, 2), "nodes_per_s": round(nout / best, 1), **split,
+                                     "includes": "plan + H2D + STEPs 1-6 + D2H, one boxfgt_ call, best of 2"}
+        log(f"c2 drop-in end to end: {best * 1e3:.1f} ms")
+    if args.copy_calibration:
+        # plain device copy of a known size: the scale for the PMC byte counters
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b2 = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b2.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device copies done")
+    s.close()
+    return leg, roofline
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", type=int, default=1, help="precision mode of the headline batch (1 FP32, 2 BF16 hi/lo)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true", help="skip the boxfgt_ drop-in timings")
+    ap.add_argument("--no-headline", action="store_true", help="skip the headline batch (profiling runs)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the single-problem FP64 leg")
+    ap.add_argument("--draws", type=int, default=512, help="generator draws the headline slice is taken from")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="problems of the whole uneven generator for a batched-vs-one-at-a-time leg (0 = skip)")
+    ap.add_argument("--serve-rates", default="", help="request arrivals per second (comma list)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
+    ap.add_argument("--no-mixed-sweep", action="store_true",
+                    help="skip the norder-12 eps x delta mixed-precision sweep")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the C2 leg, copy 1 GiB on the device 3 times (reference kernel for byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              f"sharding of the box FGT is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fgt_amd  # noqa: F401
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.init()
+
+    head = probs = None
+    if not args.no_headline:
+        probs = headline_problems(args.draws)
+        head = headline(probs, args.steps, args.warmup, args.mode, drop_in=not args.no_drop_in)
+        log(f"headline: {head}")
+    c2 = roofline = None
+    if not args.no_c2:
+        c2, roofline = c2_leg(args)
+        log(f"c2: {c2}")
     sweep = None
     if not args.no_mixed_sweep:
         sweep = mixed_sweep(max(1, min(args.steps, 5)))
         log(f"mixed sweep: {sweep}")
     batched = served = None
     if args.batch > 0:
-        probs = c4_problems(args.batch)
-        batched = batched_comparison(probs, 4, max(1, min(args.steps, 5)))
+        allp = c4_problems(args.batch)
+        batched = batched_comparison(allp, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
         rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
         if args.serve_seconds > 0 and rates:
-            served = [serving(probs, r, args.serve_seconds, 4) for r in rates]
+            served = [serving(allp, r, args.serve_seconds, 4) for r in rates]
             log(f"served: {served}")
-
     cpu = None
-    if not args.no_cpu_baseline:
-        secs, kind, threads = cpu_baseline(tree, cfg)
-        cpu = {"value": round(nout / secs, 1), "unit": "output nodes/s", "cores": threads, "kind": kind,
-               "sample": f"the full workload ({nout} output nodes), one boxfgt call, best of 2, "
-                         f"{secs:.2f}s"}
-        log(f"cpu baseline: {secs:.2f}s ({kind}, {threads} threads)")
+    if not args.no_cpu_baseline and probs is not None:
+        cpu = cpu_baseline(probs)
+        log(f"cpu baseline: {cpu}")
 
+    if head is not None:
+        value, ms, dtype = head["nodes_per_s_batched_mixed"], head["ms_batched_mixed"], "f32"
+        workload = (f"{head['problems']} problems: the eps=1e-6, 3-D slice of the first {args.draws} draws of the "
+                    f"uneven generator (norder 8/12/16, delta log-uniform in [1e-5,1e-1], 1-10 Gaussians, free "
+                    f"space), batched, mixed precision (FP32 contractions and expansions, FP64 phases, weights "
+                    f"and output accumulation)")
+        config = {"workload": workload, "problems": head["problems"], "output_nodes": head["output_nodes"],
+                  "precision_mode": args.mode}
+    else:
+        value, ms, dtype = c2["nodes_per_s"], c2["ms_per_step"], "f64"
+        config = {"workload": c2["workload"]}
     out = {
         "metric": "output nodes/sec at eps=1e-6 3D (batched vs plain FP64); p50/p99 latency",
-        "value": round(value, 1),
+        "value": value,
         "unit": "output nodes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (seeded Gaussian sources)",
-        "config": {"workload": "plain FP64 drop-in, one problem: 3-D periodic boxfgt, Legendre, norder=16, "
-                               "eps=1e-9, delta=1e-4, 20 anisotropic Gaussians (seed 1)",
-                   "nboxes": tree.nboxes, "nleaves": int(st["nleaves"]), "nlevels": tree.nlevels,
-                   "output_nodes": nout, "npwlevel": st["npwlevel"], "npw": st["npw"],
-                   "pw_leaves": int(st["npw_leaves"]), "direct_pairs": int(st["ndirect_pairs"])},
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "drop_in_end_to_end": e2e,
+        "headline": head,
+        "c2_fp64": c2,
         "mixed_precision_sweep": sweep,
         "batched": batched,
         "served": served,
     }
     print(json.dumps(out), flush=True)
-    s.close()
 
 
 if __name__ == "__main__":

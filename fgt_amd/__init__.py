@@ -179,7 +179,9 @@ class Solver:
         _check(lib().fgt_solver_execute(self._h, C.c_void_p(stream or 0), C.c_int(1 if timed else 0)))
 
     def set_precision(self, mode):
-        """0: FP64 expansions; 1: mixed (FP32 expansions, FP64 arithmetic)."""
+        """0 FP64; 1 FP32 matrix-core contractions and FP32 expansions (eps >=
+        1e-6); 2 BF16 hi/lo contractions (eps >= 1e-3).  Phases, weights and
+        the output accumulation stay FP64."""
         _check(lib().fgt_solver_set_precision(self._h, C.c_int(mode)))
 
     def set_ifpgh(self, ifpgh):
@@ -298,9 +300,11 @@ class Batch:
             self.shapes.append((t.nboxes, npbox) if nd == 1 else (t.nboxes, npbox, nd))
         _check(lib().fgt_batch_create(C.c_int(len(problems)), arr, C.c_int(nstreams), C.byref(self._h)))
 
-    def set_precision(self, mode, eps_min=1e-6):
-        """Mixed precision (mode 1) for the problems that support it and have
-        eps >= eps_min; returns how many switched."""
+    def set_precision(self, mode, eps_min=0.0):
+        """Per-problem precision: 0 FP64; 1 FP32 where eps >= 1e-6; 2 BF16
+        hi/lo where eps >= 1e-3; 3 the cheapest mode each eps allows.
+        Problems below eps_min or off the 3-D path stay FP64; returns how
+        many are not FP64."""
         n = C.c_int()
         _check(lib().fgt_batch_set_precision(self._h, C.c_int(mode), C.c_double(eps_min), C.byref(n)))
         return n.value

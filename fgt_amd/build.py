@@ -21,6 +21,9 @@ HIP_FLAGS = [
     # packed FP32 instructions stay off in all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument", "-fopenmp",
+    # a variable read before it is set (e.g. used in its own initialiser)
+    # fails the build
+    "-Werror=uninitialized", "-Werror=sometimes-uninitialized",
 ]
 
 

@@ -20,6 +20,11 @@
 
 namespace {
 
+// eps floors of the reduced-precision contraction modes (their error against
+// the FP64 result stays below eps above these)
+constexpr double kEpsFp32 = 1e-6 * (1.0 - 1e-12);
+constexpr double kEpsBf16 = 1e-3 * (1.0 - 1e-12);
+
 thread_local int g_status = 0;
 thread_local std::string g_msg;
 
@@ -558,6 +563,8 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
     const fgt::Plan& P = S.plan;
     const fgt::Geom& g = S.geom;
     const fgt::DevTables& t = S.tabs;
+    const double* fv = S.d_fvals.as<double>();
+    double2* ex = S.d_expn.as<double2>();
     hipEvent_t ev[8], evn[2];
     if (timed) {
         for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
@@ -572,7 +579,7 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
         for (const Comp& c : comps)
             fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
                                S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
-                               S.d_dir_flags.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, sn);
+                               S.d_dir_flags.as<int>(), fv, c.kd, c.od, sn);
     };
     const bool fork = fork_near && P.do_pw && !P.dir_tgt.empty();
     if (fork) {
@@ -595,10 +602,10 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
             // folded into its parent's expansion (STEP 1 + 2 fused), all levels
             // in one launch (they read only fvals)
             fgt::launch_p2pw(g, t, S.n_s1, S.d_s1_box.as<int>(), S.d_s1_lev.as<int>(), S.d_s1_slot.as<int>(),
-                             S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
+                             fv, ex, st);
             if (!fgt::p2pw_fused_3d(g, t, S.n_fz, S.d_fz_lev.as<int>(), S.d_fz_slot.as<int>(),
-                                    S.d_fz_kids.as<int>(), S.d_fz_ksg.as<int>(), S.d_fvals.as<double>(),
-                                    S.d_expn.as<double2>(), st))
+                                    S.d_fz_kids.as<int>(), S.d_fz_ksg.as<int>(), fv,
+                                    ex, st))
                 throw std::runtime_error("fused STEP 1+2 kernel not available for this shape");
             hcheck(hipGetLastError(), "k_p2pwF3 launch");
             mark(1);
@@ -607,27 +614,27 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
                 const int a = S.mp_level_off[l], b = S.mp_level_off[l + 1];
                 fgt::launch_merge(g, t, b - a, S.d_mp_slot.as<int>() + a, S.d_mp_first.as<int>() + a,
                                   S.d_mp_count.as<int>() + a, S.d_mp_child.as<int>(), S.d_mp_sign.as<int>(),
-                                  S.d_mp_acc.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+                                  S.d_mp_acc.as<int>() + a, P.nlevels - l, ex, st);
             }
             mark(2);
         } else {
             // STEP 1: every PW leaf, all levels in one launch
             fgt::launch_p2pw(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                             S.d_leaf_slot.as<int>(), S.d_fvals.as<double>(), S.d_expn.as<double2>(), st);
+                             S.d_leaf_slot.as<int>(), fv, ex, st);
             mark(1);
             // STEP 2: fine to coarse
             for (int l = P.nlevels - 1; l >= P.nlevstart; --l) {
                 const int a = P.par_level_off[l], b = P.par_level_off[l + 1];
                 fgt::launch_merge(g, t, b - a, S.d_par_slot.as<int>() + a, S.d_par_first.as<int>() + a,
                                   S.d_par_count.as<int>() + a, S.d_pc_child.as<int>(), S.d_pc_sign.as<int>(),
-                                  nullptr, P.nlevels - l, S.d_expn.as<double2>(), st);
+                                  nullptr, P.nlevels - l, ex, st);
             }
             mark(2);
         }
         // STEP 3
         fgt::launch_m2l27(g, t, S.n_m2l_groups, fgt::m2l_group(), 2LL * P.nd * P.nexp * P.nslots,
                           S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(),
-                          S.d_m2l_so.as<int>(), S.d_expn.as<double2>(), st);
+                          S.d_m2l_so.as<int>(), ex, st);
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)
@@ -635,11 +642,11 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
             if (S.fast3d) {
                 const int a = S.pcs_level_off[l], b = S.pcs_level_off[l + 1];
                 fgt::launch_split(g, t, b - a, S.d_pcs_parent.as<int>() + a, S.d_pcs_child.as<int>() + a,
-                                  S.d_pcs_sign.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+                                  S.d_pcs_sign.as<int>() + a, P.nlevels - l, ex, st);
             } else {
                 const int a = P.pc_level_off[l], b = P.pc_level_off[l + 1];
                 fgt::launch_split(g, t, b - a, S.d_pc_parent.as<int>() + a, S.d_pc_child.as<int>() + a,
-                                  S.d_pc_sign_split.as<int>() + a, P.nlevels - l, S.d_expn.as<double2>(), st);
+                                  S.d_pc_sign_split.as<int>() + a, P.nlevels - l, ex, st);
             }
         }
         mark(4);
@@ -651,10 +658,10 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) 
         for (const Comp& c : comps) {
             if (S.fast3d)
                 fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.n_s5_sib, S.d_s5_tasks.as<fgt::PotTask>(),
-                                         S.d_expn.as<double2>(), c.kd, c.od, st);
+                                         ex, c.kd, c.od, st);
             else
                 fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
-                                   S.d_leaf_slot.as<int>(), S.d_expn.as<double2>(), c.kd, c.od, st);
+                                   S.d_leaf_slot.as<int>(), ex, c.kd, c.od, st);
         }
         mark(5);
     } else {
@@ -818,6 +825,8 @@ int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly, int
 int fgt_solver_set_fvals(fgt_solver* s, const double* fvals) {
     return guarded([&] {
         if (!s || !fvals) throw std::invalid_argument("null argument");
+        // an execute queued on any stream may still read fvals
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hcheck(hipMemcpy(s->d_fvals.p, fvals, s->d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
     });
 }
@@ -832,6 +841,9 @@ int fgt_solver_execute(fgt_solver* s, void* stream, int timed) {
 int fgt_solver_get_pot(fgt_solver* s, double* pot, int accumulate) {
     return guarded([&] {
         if (!s || !pot) throw std::invalid_argument("null argument");
+        // wait for executes queued on any stream (a caller's non-blocking
+        // stream is not ordered with the null-stream copy below)
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const size_t cnt = s->d_pot.bytes / sizeof(double);
         if (!accumulate) {
             hcheck(hipMemcpy(pot, s->d_pot.p, s->d_pot.bytes, hipMemcpyDeviceToHost), "download pot");
@@ -969,15 +981,23 @@ extern "C" {
 int fgt_solver_set_precision(fgt_solver* s, int mode) {
     return guarded([&] {
         if (!s) throw std::invalid_argument("null solver");
-        if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
-        if (mode == 1 && !s->plan.do_pw)
+        if (mode < 0 || mode > 2)
+            throw std::invalid_argument("precision mode must be 0 (FP64), 1 (FP32) or 2 (BF16 hi/lo)");
+        if (mode == 1 && s->plan.eps < kEpsFp32)
+            throw std::invalid_argument("FP32 contractions need eps >= 1e-6 (this plan's eps is smaller)");
+        if (mode == 2 && s->plan.eps < kEpsBf16)
+            throw std::invalid_argument("BF16 hi/lo contractions need eps >= 1e-3 (this plan's eps is smaller)");
+        if (mode >= 1 && !s->plan.do_pw)
             throw std::invalid_argument("mixed precision: this plan forms no plane-wave expansions (near field only)");
-        if (mode == 1 && !s->fast3d)
+        if (mode >= 1 && !s->fast3d)
             throw std::invalid_argument("mixed precision needs the 3-D register-blocked path (ndim 3, norder 8/12/16)");
-        if (mode == s->tabs.expn_f32) return;
-        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        alloc_expansions(*s, mode ? sizeof(float2) : sizeof(double2));
-        s->tabs.expn_f32 = mode;
+        const int f32 = mode >= 1 ? 1 : 0;
+        if (f32 != s->tabs.expn_f32) {
+            hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            alloc_expansions(*s, f32 ? sizeof(float2) : sizeof(double2));
+            s->tabs.expn_f32 = f32;
+        }
+        s->tabs.prec = mode;
     });
 }
 
@@ -1005,6 +1025,9 @@ int fgt_solver_eval_targets(fgt_solver* s, int ntarg, const double* targs, int i
         if (ntarg < 0) throw std::invalid_argument("ntarg < 0");
         if (ntarg > 0 && (!targs || !pote || (ifpghtarg >= 2 && !grade) || (ifpghtarg >= 3 && !hesse)))
             throw std::invalid_argument("null target array");
+        // the values come from the last execute, which may be queued on
+        // another stream
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         eval_targets(*s, ntarg, targs, ifpghtarg, pote, grade, hesse, nullptr);
     });
 }
@@ -1056,12 +1079,19 @@ int fgt_batch_create(int nprob, const fgt_problem* probs, int nstreams, fgt_batc
 int fgt_batch_set_precision(fgt_batch* b, int mode, double eps_min, int* nmixed) {
     return guarded([&] {
         if (!b) throw std::invalid_argument("null batch");
-        if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
+        if (mode < 0 || mode > 3) throw std::invalid_argument("batch precision mode must be 0..3");
         int cnt = 0;
         for (auto& S : b->solvers) {
-            const bool ok = mode == 1 && S->plan.do_pw && S->fast3d && S->plan.eps >= eps_min;
-            if (fgt_solver_set_precision(S.get(), ok ? 1 : 0) != 0) throw std::runtime_error(g_msg);
-            cnt += ok;
+            const double eps = S->plan.eps;
+            const bool ok = mode >= 1 && S->plan.do_pw && S->fast3d && eps >= eps_min;
+            int m = 0;
+            if (ok) {
+                if (mode == 1) m = eps >= kEpsFp32 ? 1 : 0;
+                else if (mode == 2) m = eps >= kEpsBf16 ? 2 : 0;
+                else m = eps >= kEpsBf16 ? 2 : (eps >= kEpsFp32 ? 1 : 0);
+            }
+            if (fgt_solver_set_precision(S.get(), m) != 0) throw std::runtime_error(g_msg);
+            cnt += m > 0;
         }
         if (nmixed) *nmixed = cnt;
     });
@@ -1071,6 +1101,7 @@ int fgt_batch_set_fvals(fgt_batch* b, int i, const double* fvals) {
     return guarded([&] {
         if (!b || !fvals || i < 0 || i >= (int)b->solvers.size()) throw std::invalid_argument("bad argument");
         fgt_solver& S = *b->solvers[i];
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hcheck(hipMemcpy(S.d_fvals.p, fvals, S.d_fvals.bytes, hipMemcpyHostToDevice), "upload fvals");
     });
 }

@@ -100,6 +100,9 @@ struct DevTables {
     // 1: expansions are stored as FP32 complex (mixed precision; 3-D
     // register-blocked path only), 0: FP64
     int expn_f32;
+    // contraction precision of the mixed-precision forms: 0 FP64, 1 FP32
+    // matrix cores, 2 BF16 hi/lo matrix cores (prec >= 1 implies expn_f32)
+    int prec;
     size_t lds_limit;  // device LDS bytes per block
 };
 

@@ -19,6 +19,8 @@
 
 namespace fgt {
 
+int lowp_stages(int prec);
+
 namespace {
 
 __device__ __forceinline__ double2 zfma(double2 t, double2 f, double2 c) {
@@ -38,11 +40,156 @@ __device__ __forceinline__ void cmfma(double2 a, double2 b, dbl4& re, dbl4& im) 
     im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, im, 0, 0, 0);
 }
 
+// Reduced-precision matrix-core contractions (mixed-precision forms).  A
+// complex product C[row][kk] = sum_k A[row][k] U[kk][k] becomes a real GEMM
+// with columns (kk, part) -- part 0 the real, 1 the imaginary part of C:
+//   C = Ar * B1 + Ai * B2,  B1[k][(kk,0|1)] = (Ur | Ui),  B2 = (-Ui | Ur),
+// so a column tile of 16 holds 8 complex outputs (norder 8 fills one tile).
+// FP32: v_mfma_f32_16x16x4_f32, A lane l = A[l%16][l/16], B lane l =
+// B[l/16][l%16], result lane l element v = D[4(l/16)+v][l%16].
+// BF16 hi/lo: v_mfma_f32_16x16x32_bf16 with K = (k, part) interleaved, lane l
+// holding K entries 8(l/16)..8(l/16)+7 of row (A) or column (B) l%16; each
+// FP32 operand x is split as hi = bf16(x), lo = bf16(x - hi) and the product
+// is Ah Bh + Ah Bl + Al Bh (relative error ~1e-5), accumulated in FP32.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+// largest plane-wave count of the reduced-precision stages (row tiles of 16)
+constexpr int kLowpMaxPw = 80;
+
+__device__ __forceinline__ flt4 mfma32(float a, float b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma_bf(bf8 a, bf8 b, flt4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// hi/lo split of eight FP32 values
+__device__ __forceinline__ void split_bf(const float (&x)[8], bf8& hi, bf8& lo) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const __bf16 h = (__bf16)x[i];
+        hi[i] = h;
+        lo[i] = (__bf16)(x[i] - (float)h);
+    }
+}
+// B entry of column c = (kk, part) for K entry (k, kpart) of the expanded
+// real GEMM, from the complex table value u = U[kk][k]
+__device__ __forceinline__ float bcol(double2 u, int part, int kpart) {
+    return kpart == 0 ? (part == 0 ? (float)u.x : (float)u.y) : (part == 0 ? -(float)u.y : (float)u.x);
+}
+
+// x stage of STEP 1 in reduced precision on the matrix cores (one wave):
+//   E[k1][k2] = sum_{kk < KN} B(kk, k1) A(k2, kk)
+// rows k2 (tiles of 16), columns (k1, part) interleaved (a tile holds 8
+// frequencies k1), K = kk (KN complex terms: N values of m1, or 2N for the
+// fused form's two x signs).  fa(k2, kk) / fb(kk, k1) return the FP64
+// operands (stage buffer in LDS / 1-D table); fw(k2, k1, v) stores one
+// complex result.  The wave does column tiles ct0, ct0 + ctstep, ... in
+// pairs.  Each lane trades its part with the neighbouring lane (the other
+// part of the same output) so every result is written once as a complex
+// pair: part-0 lanes write rows v = 0, 1 of their four, part-1 lanes 2, 3.
+constexpr int kLowpColGroup = 2;
+
+template <int KN, int XS, class FA, class FB, class FW>
+__device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, FA fa, FB fb, FW fw) {
+    constexpr int CG = kLowpColGroup;
+    const int MT = (npw + 15) >> 4;
+    const int CT = (2 * npw + 15) >> 4;
+    const int li = lane & 15, lq = lane >> 4;
+    for (int c0 = ct0; c0 < CT; c0 += CG * ctstep) {
+        int k1c[CG], part[CG];
+        bool cval[CG];
+#pragma unroll
+        for (int gi = 0; gi < CG; ++gi) {
+            const int c = 16 * (c0 + gi * ctstep) + li;
+            part[gi] = c & 1;
+            cval[gi] = (c >> 1) < npw && c0 + gi * ctstep < CT;
+            k1c[gi] = cval[gi] ? (c >> 1) : 0;
+        }
+        // one row tile at a time (few live accumulators, high occupancy);
+        // the table operands are re-read per row tile from the cache
+        for (int mt = 0; mt < MT; ++mt) {
+            // two accumulator chains per tile (real / imaginary A parts, or
+            // hi*hi / the correction products) so dependent MFMAs interleave
+            flt4 acc[CG][2];
+#pragma unroll
+            for (int gi = 0; gi < CG; ++gi) acc[gi][0] = acc[gi][1] = flt4{0.f, 0.f, 0.f, 0.f};
+            const int k2r = min(16 * mt + li, npw - 1);
+            if constexpr (XS == 2) {
+#pragma unroll
+                for (int st = 0; st < (KN + 3) / 4; ++st) {
+                    const int kk = 4 * st + lq;
+                    const bool kv = kk < KN;
+                    const int kkc = kv ? kk : KN - 1;
+                    const double2 a = fa(k2r, kkc);
+#pragma unroll
+                    for (int gi = 0; gi < CG; ++gi) {
+                        const double2 u = fb(kkc, k1c[gi]);
+                        const bool ok = kv && cval[gi];
+                        const float b1 = ok ? bcol(u, part[gi], 0) : 0.f;
+                        const float b2 = ok ? bcol(u, part[gi], 1) : 0.f;
+                        acc[gi][0] = mfma32((float)a.x, b1, acc[gi][0]);
+                        acc[gi][1] = mfma32((float)a.y, b2, acc[gi][1]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int st = 0; st < (KN + 15) / 16; ++st) {
+                    float ax[8];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int kk = 16 * st + 4 * lq + j;
+                        const double2 a = fa(k2r, kk < KN ? kk : KN - 1);
+                        ax[2 * j] = (float)a.x;
+                        ax[2 * j + 1] = (float)a.y;
+                    }
+                    bf8 ah, al;
+                    split_bf(ax, ah, al);
+#pragma unroll
+                    for (int gi = 0; gi < CG; ++gi) {
+                        float bx[8];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int kk = 16 * st + 4 * lq + j;
+                            const bool ok = kk < KN && cval[gi];
+                            const double2 u = fb(kk < KN ? kk : KN - 1, k1c[gi]);
+                            bx[2 * j] = ok ? bcol(u, part[gi], 0) : 0.f;
+                            bx[2 * j + 1] = ok ? bcol(u, part[gi], 1) : 0.f;
+                        }
+                        bf8 bh, bl;
+                        split_bf(bx, bh, bl);
+                        acc[gi][0] = mfma_bf(ah, bh, acc[gi][0]);
+                        acc[gi][1] = mfma_bf(ah, bl, acc[gi][1]);
+                        acc[gi][1] = mfma_bf(al, bh, acc[gi][1]);
+                    }
+                }
+            }
+            // lane holds rows 16 mt + 4 lq + v of column (k1, part)
+#pragma unroll
+            for (int gi = 0; gi < CG; ++gi) {
+                float o[4], q[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    o[v] = acc[gi][0][v] + acc[gi][1][v];
+                    q[v] = __shfl_xor(o[v], 1);
+                }
+                if (cval[gi])
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int v = 2 * part[gi] + h;
+                        const int k2 = 16 * mt + 4 * lq + v;
+                        if (k2 < npw)
+                            fw(k2, k1c[gi], part[gi] ? make_float2(q[v], o[v]) : make_float2(o[v], q[v]));
+                    }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- STEP 1
 // LDS: sff [m2][s][m1] (N*4*N complex) and sff2 [s][m1][k2] (4*N*npw
 // complex) share one buffer: the y stage reads its sff column into registers
 // and passes a barrier before writing sff2 (more blocks fit on a CU).
-template <int N, typename ET>
+template <int N, int XS, typename ET>
 __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab, int npw,
                                               const int* __restrict__ box, const int* __restrict__ lev,
                                               const int* __restrict__ slot,
@@ -108,7 +255,17 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
             }
             __syncthreads();
             // x: E[k3][k1][k2] = sum_m1 T[k1][m1] ff2[s][m1][k2]; wave = slab, lane = k2
-            {
+            if constexpr (XS >= 2) {
+                const int k3 = r + wave;
+                if (k3 < nh) {
+                    ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    x1_lowp<N, XS>(
+                        npw, lane, 0, 1,
+                        [&](int k2, int m1) { return sff2[(wave * N + m1) * npw + k2]; },
+                        [&](int m1, int k1) { return T[k1 * N + m1]; },
+                        [&](int k2, int k1, float2 v) { Ek[(size_t)k1 * npw + k2] = v; });
+                }
+            } else {
                 const int k3 = r + wave;
                 if (k3 < nh) {
                     ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
@@ -155,7 +312,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // one buffer: sH is fully read into registers before sF is written.
 constexpr int kFusedParts = 4;
 
-template <int N, typename ET>
+template <int N, int XS, typename ET>
 __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS, int npw,
                                                const int* __restrict__ plev, const int* __restrict__ pslot,
                                                const int* __restrict__ kids, const int* __restrict__ ksg,
@@ -243,7 +400,24 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
             }
             __syncthreads();
             // x: E[k3][k1][k2] = sum_sx sum_m1 Tx^{sx}[k1][m1] F[sx][s][m1][k2]
-            {
+            if constexpr (XS >= 2) {
+                // K = (sx, m1); the two waves of a slab take alternate column tiles
+                const int s = wave & 1, hlf = wave >> 1;
+                const int k3 = r + s;
+                if (k3 < nh) {
+                    ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+                    const double2* __restrict__ Txp = T + base0 + 1 * per;
+                    const double2* __restrict__ Txm = T + base0 + 2 * per;
+                    x1_lowp<2 * N, XS>(
+                        npw, lane, hlf, 2,
+                        [&](int k2, int kk) {
+                            const int sx = kk >= N ? 1 : 0;
+                            return sF[((sx * 2 + s) * N + (kk - N * sx)) * npw + k2];
+                        },
+                        [&](int kk, int k1) { return kk >= N ? Txm[k1 * N + kk - N] : Txp[k1 * N + kk]; },
+                        [&](int k2, int k1, float2 v) { Ek[(size_t)k1 * npw + k2] = v; });
+                }
+            } else {
                 const int s = wave & 1, hlf = wave >> 1;
                 const int k3 = r + s;
                 if (k3 < nh) {
@@ -277,6 +451,100 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     }
 }
 
+// x stage of STEP 5 in reduced precision on the matrix cores (wave = slab k3):
+//   ff[k2][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2],
+// rows k2 (row tiles of 16, up to 80 frequencies), columns (kk1, part), K =
+// k1.  XS 2: FP32 operands; XS 3: BF16 hi/lo operands.  The result goes to
+// sff[(k2*4 + slab)*16 + (kk1 ^ (k2 & 15))] (FP64 complex, the layout of the
+// FP64 x stages), where the y and z stages read it.  Lanes past npw read a
+// valid duplicate row (discarded); K entries past npw get zero table values.
+template <int N, int XS>
+__device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const double2* __restrict__ UX,
+                                              int npw, int nh, int r, int wave, int lane, double2* sff) {
+    const int k3 = r + wave;
+    if (k3 >= nh) return;
+    const float2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+    constexpr int NT = (2 * N + 15) / 16;
+    const int MT = (npw + 15) >> 4;
+    const int li = lane & 15, lq = lane >> 4;
+    int ckk[NT], cpart[NT];
+    bool cval[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int c = 16 * t + li;
+        cval[t] = c < 2 * N;
+        cpart[t] = c >= N ? 1 : 0;
+        ckk[t] = cval[t] ? c - N * cpart[t] : 0;
+    }
+    double* __restrict__ sffd = reinterpret_cast<double*>(sff);
+    // one row tile (16 frequencies k2) at a time, two accumulator chains per
+    // column tile; table operands are re-read per row tile from the cache
+    for (int mt = 0; mt < MT; ++mt) {
+        const int k2r = min(16 * mt + li, npw - 1);
+        flt4 acc[NT][2];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = flt4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (XS == 2) {
+            const int ks = (npw + 3) >> 2;
+            for (int st = 0; st < ks; ++st) {
+                const int k1 = 4 * st + lq;
+                const bool kv = k1 < npw;
+                const int k1c = kv ? k1 : npw - 1;
+                const float2 e = Ek[(size_t)k1c * npw + k2r];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const double2 u = UX[k1c * N + ckk[t]];
+                    const bool ok = kv && cval[t];
+                    acc[t][0] = mfma32(e.x, ok ? bcol(u, cpart[t], 0) : 0.f, acc[t][0]);
+                    acc[t][1] = mfma32(e.y, ok ? bcol(u, cpart[t], 1) : 0.f, acc[t][1]);
+                }
+            }
+        } else {
+            // K = (k1, part): 16 frequencies per step; lane K entries 8 lq .. 8 lq + 7
+            const int ks = (npw + 15) >> 4;
+            for (int st = 0; st < ks; ++st) {
+                float ax[8];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k1c = min(16 * st + 4 * lq + j, npw - 1);
+                    const float2 e = Ek[(size_t)k1c * npw + k2r];
+                    ax[2 * j] = e.x;
+                    ax[2 * j + 1] = e.y;
+                }
+                bf8 ah, al;
+                split_bf(ax, ah, al);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    float bx[8];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int k1 = 16 * st + 4 * lq + j;
+                        const bool ok = k1 < npw && cval[t];
+                        const double2 u = UX[min(k1, npw - 1) * N + ckk[t]];
+                        bx[2 * j] = ok ? bcol(u, cpart[t], 0) : 0.f;
+                        bx[2 * j + 1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                    }
+                    bf8 bh, bl;
+                    split_bf(bx, bh, bl);
+                    acc[t][0] = mfma_bf(ah, bh, acc[t][0]);
+                    acc[t][1] = mfma_bf(ah, bl, acc[t][1]);
+                    acc[t][1] = mfma_bf(al, bh, acc[t][1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if (cval[t])
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k2 = 16 * mt + 4 * lq + v;
+                    if (k2 < npw)
+                        sffd[2 * ((k2 * 4 + wave) * 16 + (ckk[t] ^ (k2 & 15))) + cpart[t]] =
+                            acc[t][0][v] + acc[t][1][v];
+                }
+    }
+}
+
 // ---------------------------------------------------------------- STEP 5
 // Tables: tabS[kind][l][v][k][i] (frequency k, node i), v = 0 the plain
 // PW->value table, 1 / 2 the table times the + / conjugate parent->child
@@ -305,7 +573,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // C: most leaves per task (4 for sibling tasks, 1 for a leaf's own
 // expansion; with one leaf the freed registers carry the x stage's next
 // K-step loads while the current one runs).
-template <int N, bool MF, int C, typename ET>
+template <int N, int XS, int C, typename ET>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS, int npw,
                                                 const PotTask* __restrict__ tasks, const ET* __restrict__ expn,
                                                 size_t ks, Kinds kd, OutDesc od) {
@@ -357,7 +625,9 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             // (all loads issued before the FMAs that use them) so enough reads
             // are in flight to cover HBM latency; lanes past npw read a valid
             // duplicate and discard it.
-            if constexpr (MF) {
+            if constexpr (XS == 2 || XS == 3) {
+                pw2pot_x_lowp<N, XS>(E, UX, npw, nh, r, wave, lane, sff);
+            } else if constexpr (XS == 1) {
                 const int k3 = r + wave;
                 if (k3 < nh) {
                     const ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
@@ -527,12 +797,21 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
                const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
     const size_t lds = sizeof(double2) * std::max(4 * (size_t)N * N, 4 * (size_t)N * t.npw);
     if (lds > lds_limit) return false;
-    if (t.expn_f32)
-        hipLaunchKernelGGL((k_p2pw3<N, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
+    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 1)) {
+        float2* e32 = reinterpret_cast<float2*>(expn);
+        if (t.prec == 2)
+            hipLaunchKernelGGL((k_p2pw3<N, 3, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
+                               slot, fvals, e32);
+        else
+            hipLaunchKernelGGL((k_p2pw3<N, 2, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
+                               slot, fvals, e32);
+    } else if (t.expn_f32) {
+        hipLaunchKernelGGL((k_p2pw3<N, 0, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
                            fvals, reinterpret_cast<float2*>(expn));
-    else
-        hipLaunchKernelGGL((k_p2pw3<N, double2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
-                           fvals, expn);
+    } else {
+        hipLaunchKernelGGL((k_p2pw3<N, 0, double2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
+                           slot, fvals, expn);
+    }
     return true;
 }
 
@@ -540,24 +819,29 @@ size_t pw2pot3_lds(int n, int npw) {
     return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + 12 * (size_t)n);
 }
 
-template <int N, bool MF, int C>
+template <int N, int XS, int C>
 void launch_pw2pot3_c(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
                       Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
     if (ntask <= 0) return;
-    if (t.expn_f32)
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+    if constexpr (XS >= 2) {
+        hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
                            tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
-    else
-        hipLaunchKernelGGL((k_pw2pot3<N, MF, C, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
-                           tasks, expn, t.pw2potS_ks, kd, od);
+    } else {
+        if (t.expn_f32)
+            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+                               tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+        else
+            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS,
+                               t.npw, tasks, expn, t.pw2potS_ks, kd, od);
+    }
 }
 
 // tasks [0, nsib) hold up to four leaves, [nsib, ntask) one leaf each
-template <int N, bool MF>
+template <int N, int XS>
 void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                           const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
-    launch_pw2pot3_c<N, MF, 4>(g, t, nsib, tasks, expn, kd, od, s, lds);
-    launch_pw2pot3_c<N, MF, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds);
+    launch_pw2pot3_c<N, XS, 4>(g, t, nsib, tasks, expn, kd, od, s, lds);
+    launch_pw2pot3_c<N, XS, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds);
 }
 
 template <int N>
@@ -566,13 +850,19 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
+    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 2)) {
+        // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores)
+        if (t.prec == 2) launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+        else launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+        return true;
+    }
     if constexpr (N >= 12) {
         if (t.npw <= 64 && !pw2pot_valu_only()) {
-            launch_pw2pot3_typed<N, true>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+            launch_pw2pot3_typed<N, 1>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
             return true;
         }
     }
-    launch_pw2pot3_typed<N, false>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+    launch_pw2pot3_typed<N, 0>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
     return true;
 }
 
@@ -580,15 +870,39 @@ template <int N>
 void launch_p2pwF3_typed(const Geom& g, const DevTables& t, unsigned nblk, size_t lds, const int* plev,
                          const int* pslot, const int* kids, const int* ksg, const double* fvals, double2* expn,
                          hipStream_t s) {
-    if (t.expn_f32)
-        hipLaunchKernelGGL((k_p2pwF3<N, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
+    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 1)) {
+        float2* e32 = reinterpret_cast<float2*>(expn);
+        if (t.prec == 2)
+            hipLaunchKernelGGL((k_p2pwF3<N, 3, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
+                               pslot, kids, ksg, fvals, e32);
+        else
+            hipLaunchKernelGGL((k_p2pwF3<N, 2, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
+                               pslot, kids, ksg, fvals, e32);
+    } else if (t.expn_f32) {
+        hipLaunchKernelGGL((k_p2pwF3<N, 0, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
                            kids, ksg, fvals, reinterpret_cast<float2*>(expn));
-    else
-        hipLaunchKernelGGL((k_p2pwF3<N, double2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
-                           kids, ksg, fvals, expn);
+    } else {
+        hipLaunchKernelGGL((k_p2pwF3<N, 0, double2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
+                           pslot, kids, ksg, fvals, expn);
+    }
 }
 
 }  // namespace
+
+// FGT_LOWP_STAGES (bit 0 STEP 1, bit 1 STEP 5): which stages run their x
+// contraction on the reduced-precision matrix cores in the FP32 / BF16
+// modes; the others keep the FP64 VALU contraction on FP32 expansions
+// (tuning and comparison runs).
+int lowp_stages(int prec) {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_LOWP_STAGES");
+        return e ? (std::atoi(e) & 3) : -1;
+    }();
+    // defaults: FP32 keeps STEP 1 on the VALU (its matrix-core form measured
+    // slower: 50 x 50-class outputs pad to 64-row / 16-column tiles with K = 8
+    // or 16); BF16 hi/lo runs both stages on the matrix cores
+    return v >= 0 ? v : (prec == 2 ? 3 : 2);
+}
 
 // FGT_PW2POT_VALU=1 selects the FP64 VALU x stage of STEP 5 instead of the
 // matrix-core one (for comparison runs).

@@ -84,13 +84,14 @@ int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly,
                       int ltree, const int *itree, const int *iptr,
                       const double *centers, const double *boxsize,
                       fgt_solver **out);
-/* fvals(nd,npbox,nboxes) host -> device. */
+/* fvals(nd,npbox,nboxes) host -> device (after all queued device work). */
 int fgt_solver_set_fvals(fgt_solver *s, const double *fvals);
 /* Runs STEPs 1-6 on `stream` (a hipStream_t; NULL = default stream) into the
  * device potential (overwritten, not accumulated).  timed != 0 records
  * per-stage events (synchronises at the end). */
 int fgt_solver_execute(fgt_solver *s, void *stream, int timed);
-/* device potential -> host; accumulate != 0 adds into pot (reference +=). */
+/* device potential -> host; accumulate != 0 adds into pot (reference +=).
+ * Waits for all queued device work first (an execute on any stream). */
 int fgt_solver_get_pot(fgt_solver *s, double *pot, int accumulate);
 /* Output set of later executes, as boxfgt's ifpgh (src/bfgt/boxfgt.f:35-38):
  * 1 potential, 2 + gradient, 3 + Hessian.  Allocates the device gradient
@@ -98,12 +99,19 @@ int fgt_solver_get_pot(fgt_solver *s, double *pot, int accumulate);
  * arrays (Hessian order 1-D xx; 2-D xx,xy,yy; 3-D xx,yy,zz,xy,xz,yz, as
  * src/bfgt/bfgt_pwrouts.f:612-1065).  Default 1. */
 int fgt_solver_set_ifpgh(fgt_solver *s, int ifpgh);
-/* Expansion storage: mode 0 FP64 (default), 1 mixed precision -- the
- * plane-wave expansions are stored as FP32 complex while tables, phases,
- * arithmetic and outputs stay FP64 (for eps >= 1e-6; error against the
- * FP64 result stays below eps).  Mode 1 needs the 3-D register-blocked
- * path (ndim 3, norder 8/12/16); otherwise status 2.  No counterpart in the
- * reference (FP64 only). */
+/* Precision of later executes.  No counterpart in the reference (FP64 only).
+ *   0  FP64 (default).
+ *   1  FP32: plane-wave expansions stored as FP32 complex and the large
+ *      tensor-product contractions (STEP 5 x stage) on the FP32 matrix
+ *      cores with FP32 accumulation; table and phase values are formed in
+ *      FP64, translations (STEPs 2-4) multiply in FP64 and the output
+ *      accumulation is FP64.  Needs eps >= 1e-6.
+ *   2  BF16 hi/lo: as 1, with those contractions as three BF16 matrix-core
+ *      products per FP32 pair (hi*hi + hi*lo + lo*hi).  Needs eps >= 1e-3.
+ * Modes 1 and 2 need the 3-D register-blocked path (ndim 3, norder
+ * 8/12/16) and a plan that forms plane-wave expansions.  A mode the
+ * solver's eps or shape does not allow returns status 2 and changes
+ * nothing. */
 int fgt_solver_set_precision(fgt_solver *s, int mode);
 /* device gradient / Hessian -> host (accumulate != 0 adds, reference +=);
  * status 2 when not requested by fgt_solver_set_ifpgh. */
@@ -140,8 +148,12 @@ typedef struct fgt_batch fgt_batch;
  * (1..16) HIP streams; executes are spread over them, largest first. */
 int fgt_batch_create(int nprob, const fgt_problem *probs, int nstreams,
                      fgt_batch **out);
-/* mode 1: mixed precision for every problem that supports it and has
- * eps >= eps_min (the others stay FP64); *nmixed (nullable) = their count. */
+/* Per-problem precision (fgt_solver_set_precision modes): mode 0 FP64 for
+ * all; 1 FP32 where eps >= 1e-6; 2 BF16 hi/lo where eps >= 1e-3; 3 the
+ * cheapest mode each problem's eps allows (BF16 hi/lo for eps >= 1e-3, FP32
+ * for eps >= 1e-6).  Problems with eps < eps_min, or off the 3-D
+ * register-blocked path, stay FP64.  *nmixed (nullable) = problems not in
+ * FP64. */
 int fgt_batch_set_precision(fgt_batch *b, int mode, double eps_min,
                             int *nmixed);
 int fgt_batch_set_fvals(fgt_batch *b, int i, const double *fvals);

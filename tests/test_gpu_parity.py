@@ -133,3 +133,30 @@ def test_c2_workload_vs_reference():
     err = rel_l2(pot, ref)
     print(f"[gpu]   nboxes={tree.nboxes} rel l2 = {err:.3e}", flush=True)
     assert err < 1e-12
+
+
+def test_results_wait_for_execute_on_a_nonblocking_stream():
+    """execute queued on a caller's non-blocking stream, then get_pot /
+    eval_targets / set_fvals with no synchronisation by the caller: each
+    waits for the queued work (same values as a default-stream execute)."""
+    import torch
+    print("[gpu] execute on a non-blocking stream, then get_pot without a sync", flush=True)
+    c = load_case("c1_small")
+    s = fgt_amd.Solver.from_tree(c, c.delta, c.eps)
+    s.set_fvals(c.fvals)
+    s.execute()
+    ref = s.get_pot()
+    pe_ref, _, _ = s.eval_targets(c.centers[c.leaves[:7]] + 0.01)
+    st = torch.cuda.Stream()
+    for _ in range(3):
+        s.execute(stream=st.cuda_stream)
+        assert np.array_equal(s.get_pot(), ref)
+    s.execute(stream=st.cuda_stream)
+    pe, _, _ = s.eval_targets(c.centers[c.leaves[:7]] + 0.01)
+    assert np.array_equal(pe, pe_ref)
+    # set_fvals right after a queued execute: the execute still sees the old values
+    s.execute(stream=st.cuda_stream)
+    s.set_fvals(2.0 * c.fvals)
+    s.execute(stream=st.cuda_stream)
+    assert rel_l2(s.get_pot(), 2.0 * ref) < 1e-14
+    s.close()
