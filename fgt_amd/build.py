@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfgt_amd.so")
 
 SOURCES = ["plan.cpp", "host_api.cpp", "tree.cpp", "api.cpp", "kernels.hip", "kernels_pw3.hip", "kernels_direct3.hip",
-           "kernels_targets.hip"]
+           "kernels_targets.hip", "noalloc.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -37,18 +37,47 @@ void hcheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// A device buffer owned by the solver.
+// Bump allocator over a caller-provided device range (the no-allocation
+// entry point's scratch): while one is installed on this thread, device
+// buffers are carved from it instead of hipMalloc'd.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(size_t b) {
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off + b > cap)
+            throw std::invalid_argument("problem needs more device workspace than the " +
+                                        std::to_string(cap >> 20) + " MiB no-allocation scratch");
+        used = off + b;
+        return base + off;
+    }
+};
+thread_local Arena* g_arena = nullptr;
+
+// A device buffer owned by the solver (or carved from the installed arena).
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool from_arena = false;
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    ~DBuf() { release(); }
+    void release() {
+        if (p && !from_arena) (void)hipFree(p);
+        p = nullptr;
+        from_arena = false;
+    }
     void alloc(size_t b) {
-        if (p) { (void)hipFree(p); p = nullptr; }
+        release();
         bytes = b;
-        if (b) hcheck(hipMalloc(&p, b), "hipMalloc");
+        if (!b) return;
+        if (g_arena) {
+            p = g_arena->take(b);
+            from_arena = true;
+        } else {
+            hcheck(hipMalloc(&p, b), "hipMalloc");
+        }
     }
     template <class T>
     void upload(const std::vector<T>& v) {
@@ -158,11 +187,16 @@ void alloc_expansions(fgt_solver& S, size_t elem) {
     const fgt::Plan& P = S.plan;
     const size_t slot_elems = 2 * (size_t)P.nd * P.nexp;
     S.d_expn.alloc(elem * slot_elems * (P.nslots + 1));
-    hcheck(hipMemset(static_cast<char*>(S.d_expn.p) + elem * slot_elems * P.nslots, 0, elem * slot_elems),
-           "zero slot");
+    // an arena's contents persist between calls: its user clears the zero
+    // slot on its own stream before each execute
+    if (!S.d_expn.from_arena)
+        hcheck(hipMemset(static_cast<char*>(S.d_expn.p) + elem * slot_elems * P.nslots, 0, elem * slot_elems),
+               "zero slot");
 }
 
-void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
+// own_io = false: the caller brings device fvals / pot (no-allocation
+// entry point), so the solver keeps none of its own.
+void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
     std::string err;
     const int rc = fgt::make_plan(in, S.plan, &err);
     if (rc != 0) throw std::invalid_argument(err);
@@ -193,8 +227,10 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in) {
     if (P.do_pw && (lds_p2pw > lds_limit || lds_pw2pot > lds_limit))
         throw std::invalid_argument("plane-wave count too large for the LDS-staged kernels");
 
-    S.d_fvals.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
-    S.d_pot.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
+    if (own_io) {
+        S.d_fvals.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
+        S.d_pot.alloc(sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes);
+    }
 
     S.d_loctab.upload(P.loctab);
     S.d_locband.upload(P.locband);
@@ -531,7 +567,7 @@ struct Comp {
     fgt::OutDesc od;
 };
 
-std::vector<Comp> components(fgt_solver& S) {
+std::vector<Comp> components(fgt_solver& S, double* pot_override = nullptr) {
     const fgt::Plan& P = S.plan;
     const int ndim = P.ndim, nd = P.nd, nhess = ndim * (ndim + 1) / 2;
     auto axes = [&](int r0, int r1, int r2) {
@@ -540,6 +576,10 @@ std::vector<Comp> components(fgt_solver& S) {
         return fgt::Kinds{0, 0, r0};
     };
     std::vector<Comp> c;
+    if (pot_override) {
+        c.push_back({axes(0, 0, 0), {pot_override, nd, 0}});
+        return c;
+    }
     c.push_back({axes(0, 0, 0), {S.d_pot.as<double>(), nd, 0}});
     if (S.ifpgh >= 2)
         for (int a = 0; a < ndim; ++a)
@@ -559,22 +599,35 @@ std::vector<Comp> components(fgt_solver& S) {
 // 1-4 on `st` (single-problem executes; a batch already overlaps problems).
 // The outputs are the same bit for bit: each output node receives one STEP 5
 // and one STEP 6 sum onto zero, and two-term addition is commutative.
-void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false) {
+// Caller-provided buffers (no-allocation entry point): device fvals, a
+// device potential that is accumulated into (not zeroed) and the expansion
+// workspace; the potential only.
+struct ExecOverride {
+    const double* fvals;
+    double* pot;
+    double2* expn;
+};
+
+void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
+             const ExecOverride* ov = nullptr) {
     const fgt::Plan& P = S.plan;
     const fgt::Geom& g = S.geom;
     const fgt::DevTables& t = S.tabs;
-    const double* fv = S.d_fvals.as<double>();
-    double2* ex = S.d_expn.as<double2>();
+    const double* fv = ov ? ov->fvals : S.d_fvals.as<double>();
+    double2* ex = ov ? ov->expn : S.d_expn.as<double2>();
+    if (ov) timed = fork_near = false;
     hipEvent_t ev[8], evn[2];
     if (timed) {
         for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
         for (auto& e : evn) hcheck(hipEventCreate(&e), "hipEventCreate");
     }
     auto mark = [&](int i) { if (timed) hcheck(hipEventRecord(ev[i], st), "hipEventRecord"); };
-    hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
-    if (S.ifpgh >= 2) hcheck(hipMemsetAsync(S.d_grad.p, 0, S.d_grad.bytes, st), "hipMemsetAsync grad");
-    if (S.ifpgh >= 3) hcheck(hipMemsetAsync(S.d_hess.p, 0, S.d_hess.bytes, st), "hipMemsetAsync hess");
-    const std::vector<Comp> comps = components(S);
+    if (!ov) {
+        hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
+        if (S.ifpgh >= 2) hcheck(hipMemsetAsync(S.d_grad.p, 0, S.d_grad.bytes, st), "hipMemsetAsync grad");
+        if (S.ifpgh >= 3) hcheck(hipMemsetAsync(S.d_hess.p, 0, S.d_hess.bytes, st), "hipMemsetAsync hess");
+    }
+    const std::vector<Comp> comps = components(S, ov ? ov->pot : nullptr);
     auto near_field = [&](hipStream_t sn) {
         for (const Comp& c : comps)
             fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
@@ -1213,6 +1266,40 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
             timeinfo[7] = t3 - t2;
         }
         (void)t2;
+    });
+}
+
+int fgt_boxfgt_noalloc(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder, int nboxes,
+                       int nlevels, int ltree, const int* itree, const int* iptr, const double* centers,
+                       const double* boxsize, const double* fvals, double* pot, void* stream) {
+    return guarded([&] {
+        if (!itree || !iptr || !centers || !boxsize || !fvals || !pot) throw std::invalid_argument("null argument");
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        size_t cap = 0;
+        char* base = static_cast<char*>(fgt::scratch_region(&cap));
+        // every call on this device shares the scratch: the previous call
+        // ended with a synchronisation; wait for the caller's stream too
+        hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+        Arena arena{base, cap, 0};
+        struct Installed {
+            explicit Installed(Arena* a) { g_arena = a; }
+            ~Installed() { g_arena = nullptr; }
+        } installed(&arena);
+        int npbox = 1;
+        for (int i = 0; i < ndim && i < 3; ++i) npbox *= norder;
+        fgt::PlanInput in{nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels, ltree,
+                          itree, iptr, centers, boxsize};
+        fgt_solver S;   // every device buffer of S is carved from the scratch
+        build_solver(S, in, false);
+        ExecOverride ov{fvals, pot, S.d_expn.as<double2>()};
+        if (S.plan.do_pw) {
+            // the scratch keeps the last call's contents: clear the zero slot
+            const size_t slot = sizeof(double2) * 2 * (size_t)S.plan.nd * S.plan.nexp;
+            fgt::launch_zero(static_cast<char*>(S.d_expn.p) + slot * S.plan.nslots, slot, st);
+        }
+        execute(S, st, false, false, &ov);
+        hcheck(hipGetLastError(), "launch");
+        hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
     });
 }
 

@@ -167,6 +167,14 @@ bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* t
                       const int* off, const int* src, const int* code, const int* flags,
                       const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
 
+// Device workspace of the no-allocation entry point: a file-scope device
+// array of kScratchBytes per GPU.  scratch_region() returns its address on
+// the current device (looked up once per device); launch_zero() clears a
+// byte range (8-byte aligned) with a fill kernel on stream s.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+void* scratch_region(size_t* bytes);
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+
 // Extra targets (STEP 11): values (ncd, npbox, nboxes) on the listed leaves
 // -> coefficients coefs[b][q][i] (u = values->coefficients, u(k,j) at
 // u[k + j*n]); then per task (leaf tbox, sorted targets tfirst..+tcount,

@@ -54,6 +54,25 @@ int fgt_boxfgt(int *nd, int *ndim, double *delta, double *eps, int *ipoly,
                int *ntarg, double *targs, int *ifpghtarg, double *pote,
                double *grade, double *hesse, double *timeinfo);
 
+/*
+ * fgt_boxfgt_noalloc -- boxfgt with device-resident data that allocates no
+ * device memory.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same computation as boxfgt_ with
+ * ifpgh = 1, ifpghtarg = 0, ifnewtree = 0: fvals(nd,npbox,nboxes) and
+ * pot(nd,npbox,nboxes) are DEVICE pointers and pot is accumulated (+=) on
+ * `stream` (a hipStream_t; NULL = default stream); the tree arrays are host
+ * arrays, planned on the host.  Every other device buffer (tables, work
+ * lists, plane-wave expansions) is carved from a 128 MiB per-GPU static
+ * device workspace; a problem that needs more returns status 2.  Calls are
+ * synchronous (they return when the result is in pot).  All calls on one GPU
+ * share that workspace: calls that overlap in time on one GPU race.
+ */
+int fgt_boxfgt_noalloc(int nd, int ndim, double delta, double eps, int ipoly, int iperiod, int norder,
+                       int nboxes, int nlevels, int ltree, const int *itree, const int *iptr,
+                       const double *centers, const double *boxsize, const double *fvals, double *pot,
+                       void *stream);
+
 /* Status of the last call made on this thread: 0 = ok.  Copies the message
  * (NUL-terminated, truncated to len) into msg when msg != NULL.  Added error
  * channel; the reference has none (boxfgt.f:191 ignores its allocation
