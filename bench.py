@@ -107,18 +107,39 @@ def c4_problem(i, base=1000):
     return ndim, n, eps, delta, (c, s, a)
 
 
-def c4_problems(nprob):
-    """The first nprob draws of the uneven generator, every draw kept."""
+def c4_problems(nprob, max_leaves=8000, periodic=False, threads=4):
+    """nprob problems of the uneven generator within its stated leaf range:
+    draws whose tree would exceed max_leaves leaves are replaced by the next
+    draw (the generator's narrowest Gaussians at eps = 1e-9 give trees of up
+    to ~2e5 leaves).  periodic: every odd draw is periodic (the served
+    stream's mix).  Returns (problems, draws, replaced)."""
     import fgt_amd
     from fgt_amd.tree import gauss_source
-    t0 = time.time()
-    probs = []
-    for i in range(nprob):
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
         ndim, n, eps, delta, g = c4_problem(i)
-        tr = fgt_amd.vol_tree_build(ndim, n, 0, 0, eps, gauss_source(*g))
-        probs.append((tr, delta, eps))
-    log(f"batch: {nprob} problems built in {time.time() - t0:.1f}s")
-    return probs
+        iper = (i & 1) if periodic else 0
+        try:
+            tr = fgt_amd.vol_tree_build(ndim, n, 0, iper, eps, gauss_source(*g), max_leaves=max_leaves)
+        except fgt_amd.TreeTooLarge:
+            return None
+        return (tr, delta, eps)
+
+    t0 = time.time()
+    probs, i = [], 0
+    with ThreadPoolExecutor(threads) as ex:
+        while len(probs) < nprob:
+            chunk = list(ex.map(one, range(i, i + 2 * (nprob - len(probs)))))
+            for k, p in enumerate(chunk):
+                if p is not None and len(probs) < nprob:
+                    probs.append(p)
+                    last = i + k
+            i += len(chunk)
+    draws = last + 1
+    log(f"uneven batch: {nprob} problems from {draws} draws ({draws - nprob} over {max_leaves} leaves replaced), "
+        f"built in {time.time() - t0:.1f}s")
+    return probs, draws, draws - nprob
 
 
 def headline_problems(ndraw=512):
@@ -557,7 +578,7 @@ def main():
         log(f"mixed sweep: {sweep}")
     batched = served = None
     if args.batch > 0:
-        allp = c4_problems(args.batch)
+        allp, _, _ = c4_problems(args.batch)
         batched = batched_comparison(allp, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
         rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]

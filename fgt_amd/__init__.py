@@ -276,8 +276,10 @@ class _Problem(C.Structure):
 
 
 class Batch:
-    """Many independent box FGT problems planned together and executed
-    overlapped on ``nstreams`` HIP streams (largest problem first).
+    """Many independent box FGT problems planned and executed together: 3-D
+    problems of norder 8/12/16 in shape classes, one launch per stage and
+    class; the others on ``nstreams`` side streams; the whole transform
+    replayed from a captured HIP graph.
 
     ``problems``: sequence of (tree, delta, eps) with ``tree`` a
     :class:`fgt_amd.tree.Tree` (or any object with the same attributes).
@@ -318,6 +320,27 @@ class Batch:
     def execute(self):
         _check(lib().fgt_batch_execute(self._h))
 
+    def set_graph(self, on):
+        """Replay the batch from a captured HIP graph (default) or enqueue
+        every launch eagerly."""
+        _check(lib().fgt_batch_set_graph(self._h, C.c_int(1 if on else 0)))
+
+    def autotune(self, reps=3):
+        """Pick, per shape class, the STEP 3 targets per block and the STEP 1 /
+        STEP 5 x-stage variants by timing them on this batch's data."""
+        _check(lib().fgt_batch_autotune(self._h, C.c_int(reps)))
+
+    def classes(self):
+        """Shape classes: dicts of norder, nd, precision, plane-wave cap,
+        member count and the chosen group / x-stage variants."""
+        out = []
+        for k in range(lib().fgt_batch_nclasses(self._h)):
+            v = (C.c_int * 10)()
+            _check(lib().fgt_batch_class_info(self._h, C.c_int(k), v))
+            out.append(dict(zip(("norder", "nd", "prec", "npw_cap", "members", "group", "xs1", "xs5",
+                                 "step5_tasks", "near_targets"), list(v))))
+        return out
+
     def get_pot(self, i):
         out = np.zeros(self.shapes[i])
         _check(lib().fgt_batch_get_pot(self._h, C.c_int(i), _p(out), C.c_int(0)))
@@ -340,7 +363,7 @@ class Batch:
             pass
 
 
-from .tree import Tree, vol_tree_build  # noqa: E402  (host tree builder)
+from .tree import Tree, TreeTooLarge, vol_tree_build  # noqa: E402  (host tree builder)
 
-__all__ = ["boxfgt", "Solver", "Batch", "FgtError", "plan_summary", "Tree", "vol_tree_build", "lib",
+__all__ = ["boxfgt", "Solver", "Batch", "FgtError", "plan_summary", "Tree", "TreeTooLarge", "vol_tree_build", "lib",
            "last_status"]

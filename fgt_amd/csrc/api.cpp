@@ -7,10 +7,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -123,6 +125,17 @@ struct fgt_solver {
     DBuf d_pcs_parent, d_pcs_child, d_pcs_sign;   // splits into non-leaf children only
     std::vector<int> pcs_level_off;
     double stage_ms[8] = {0};
+    // host copies of the device work lists (the batch concatenates them
+    // per shape class)
+    struct HostLists {
+        std::vector<fgt::PotTask> s5;
+        std::vector<int> pcs_parent, pcs_child, pcs_sign;
+        std::vector<int> fz_lev, fz_slot, fz_kids, fz_ksg, s1_box, s1_lev, s1_slot;
+        std::vector<int> mp_slot, mp_first, mp_count, mp_child, mp_sign, mp_acc;
+        std::vector<long long> m2l_toff, m2l_soff;
+        std::vector<int> m2l_gent, m2l_code;
+        std::vector<int> dir_tgt, dir_lev, dir_off, dir_src, dir_code, dir_flags;
+    } h;
     // STEP 6 on a second stream, overlapping STEPs 1-4 (it reads only fvals;
     // STEP 5 waits for it, so pot gets each sum once): made on first use
     hipStream_t st_near = nullptr;
@@ -131,24 +144,6 @@ struct fgt_solver {
         if (st_near) (void)hipStreamDestroy(st_near);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
-    }
-};
-
-// Many independent problems (uneven sizes, orders, delta) solved together:
-// each has its own plan and device buffers; their executes are spread over
-// a few HIP streams (largest first, each onto the least-loaded stream) so
-// small problems overlap on the device instead of running one after another.
-struct fgt_batch {
-    std::vector<std::unique_ptr<fgt_solver>> solvers;
-    std::vector<hipStream_t> streams;
-    std::vector<int> stream_of;     // per problem
-    std::vector<int> order;         // launch order (largest first)
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    double last_ms = 0.0;
-    ~fgt_batch() {
-        for (auto s : streams) (void)hipStreamDestroy(s);
-        if (t0) (void)hipEventDestroy(t0);
-        if (t1) (void)hipEventDestroy(t1);
     }
 };
 
@@ -192,6 +187,75 @@ void alloc_expansions(fgt_solver& S, size_t elem) {
     if (!S.d_expn.from_arena)
         hcheck(hipMemset(static_cast<char*>(S.d_expn.p) + elem * slot_elems * P.nslots, 0, elem * slot_elems),
                "zero slot");
+}
+
+// STEP 3 entry streams with G consecutive targets per group: per target its
+// own expansion (unit phase, o = 13), then its list, the final entry
+// flagged; each group padded with zero-slot entries to a multiple of
+// kM2lBatch.  toff: G target element offsets per group; gent: group ranges
+// of soff / code.  Returns the group count.
+int m2l_streams(const fgt::Plan& P, int G, std::vector<long long>& toff, std::vector<int>& gent,
+                std::vector<long long>& soff, std::vector<int>& code) {
+    const long long se = 2LL * P.nd * P.nexp;
+    const int nt = (int)P.m2l_tgt.size();
+    const int ng = (nt + G - 1) / G;
+    toff.assign((size_t)ng * G, 0);
+    gent.assign(ng + 1, 0);
+    soff.clear();
+    code.clear();
+    for (int gi = 0; gi < ng; ++gi) {
+        gent[gi] = (int)soff.size();
+        for (int tk = gi * G; tk < std::min(nt, (gi + 1) * G); ++tk) {
+            toff[tk] = se * P.m2l_tgt[tk];
+            soff.push_back(toff[tk]);
+            code.push_back(13);
+            for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
+                soff.push_back(se * P.m2l_src[j]);
+                code.push_back(map_m2l(P.ndim, P.m2l_idx[j]));
+            }
+            code.back() |= 1 << 5;
+        }
+        while ((soff.size() - gent[gi]) % fgt::kM2lBatch) {
+            soff.push_back(se * P.nslots);   // the zero slot
+            code.push_back(13);
+        }
+        if ((int)soff.size() - gent[gi] > fgt::kM2lMaxEntries)
+            throw std::runtime_error("STEP 3 group entry stream longer than kM2lMaxEntries");
+    }
+    gent[ng] = (int)soff.size();
+    return ng;
+}
+
+// On the 3-D fast path a leaf below the first plane-wave level never holds
+// an expansion of its own: STEP 1 writes its parent's (fused with STEP 2)
+// and STEP 5 evaluates its parent's through phased tables.  Only non-leaf
+// boxes and the leaves of the first plane-wave level keep slots; every slot
+// list of the plan is renumbered (a dropped leaf's slot becomes -1, used by
+// the other path only).
+void compact_slots(fgt::Plan& P) {
+    const fgt::TreeView& T = P.tree;
+    std::vector<int> remap(P.nslots, -1);
+    int ns = 0;
+    for (int l = P.nlevstart; l <= P.nlevels; ++l)
+        for (int b = T.lfirst[l]; b <= T.llast[l]; ++b) {
+            const int sl = P.slot_of_box[b];
+            if (sl >= 0 && (T.nchild[b] > 0 || l == P.nlevstart)) remap[sl] = ns++;
+        }
+    auto rm = [&](std::vector<int>& v) {
+        for (int& x : v) x = x >= 0 ? remap[x] : -1;
+    };
+    rm(P.slot_of_box);
+    rm(P.leaf_slot);
+    rm(P.par_slot);
+    rm(P.pc_parent);
+    rm(P.pc_child);
+    rm(P.m2l_tgt);
+    rm(P.m2l_src);
+    for (int x : P.m2l_tgt)
+        if (x < 0) throw std::runtime_error("STEP 3 target lost its slot");
+    for (int x : P.m2l_src)
+        if (x < 0) throw std::runtime_error("STEP 3 source lost its slot");
+    P.nslots = ns;
 }
 
 // own_io = false: the caller brings device fvals / pot (no-allocation
@@ -247,7 +311,6 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
         S.d_pw2potT.upload(to_d2(tr));
         S.d_wms.upload(to_d2(P.wms));
         S.d_wm2l.upload(to_d2(P.wm2l));
-        alloc_expansions(S, sizeof(double2));
     }
     fgt::DevTables& t = S.tabs;
     t.p2pw = S.d_p2pw.as<double2>();
@@ -259,6 +322,8 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
     t.n = n;
     t.npw = P.npw;
     S.fast3d = P.do_pw && fgt::pw2pot_3d_supported(g, t);
+    if (S.fast3d) compact_slots(S.plan);
+    if (P.do_pw) alloc_expansions(S, sizeof(double2));
     if (S.fast3d) {
         const int nl = P.nlevels, np = P.npw;
         const size_t per = (size_t)n * np;
@@ -335,6 +400,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
         fam.insert(fam.end(), own.begin(), own.end());
         S.n_s5 = (int)fam.size();
         S.d_s5_tasks.upload(fam);
+        S.h.s5 = fam;
         // split entries, keeping non-leaf children only
         std::vector<int> pp, pcc, psg;
         S.pcs_level_off.assign(nl + 2, 0);
@@ -357,8 +423,11 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
         }
         S.pcs_level_off[nl + 1] = (int)pp.size();
         S.d_pcs_parent.upload(pp);
+        S.h.pcs_parent = pp;
         S.d_pcs_child.upload(pcc);
+        S.h.pcs_child = pcc;
         S.d_pcs_sign.upload(psg);
+        S.h.pcs_sign = psg;
 
         // fused STEP 1+2: value->PW tables times the child->parent merge phase
         std::vector<fgt::cplx> t1(3 * per * (nl + 1), fgt::cplx(0.0));
@@ -431,18 +500,31 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
         S.n_fz = (int)fzl.size();
         S.n_s1 = (int)s1b.size();
         S.d_fz_lev.upload(fzl);
+        S.h.fz_lev = fzl;
         S.d_fz_slot.upload(fzs);
+        S.h.fz_slot = fzs;
         S.d_fz_kids.upload(fzk);
+        S.h.fz_kids = fzk;
         S.d_fz_ksg.upload(fzg);
+        S.h.fz_ksg = fzg;
         S.d_s1_box.upload(s1b);
+        S.h.s1_box = s1b;
         S.d_s1_lev.upload(s1l);
+        S.h.s1_lev = s1l;
         S.d_s1_slot.upload(s1s);
+        S.h.s1_slot = s1s;
         S.d_mp_slot.upload(mps);
+        S.h.mp_slot = mps;
         S.d_mp_first.upload(mpf);
+        S.h.mp_first = mpf;
         S.d_mp_count.upload(mpc);
+        S.h.mp_count = mpc;
         S.d_mp_child.upload(mch);
+        S.h.mp_child = mch;
         S.d_mp_sign.upload(msg);
+        S.h.mp_sign = msg;
         S.d_mp_acc.upload(macc);
+        S.h.mp_acc = macc;
     }
     t.p2pwS = S.d_p2pwS.as<double2>();
     t.pw2potS = S.d_pw2potS.as<double2>();
@@ -477,40 +559,17 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
     for (size_t i = 0; i < mi.size(); ++i) mi[i] = map_m2l(ndim, P.m2l_idx[i]);
     S.d_m2l_idx.upload(mi);
     if (P.do_pw) {
-        // STEP 3 entry streams: kM2lGroup consecutive targets per group; per
-        // target its own expansion (unit phase, o = 13), then its list, the
-        // final entry flagged; each group padded with zero-slot entries
-        const long long se = 2LL * P.nd * P.nexp;
-        const int nt = (int)P.m2l_tgt.size();
-        const int G = fgt::m2l_group();
-        const int ng = (nt + G - 1) / G;
-        std::vector<long long> toff((size_t)ng * G, 0), soff;
-        std::vector<int> gent(ng + 1, 0), code;
-        for (int gi = 0; gi < ng; ++gi) {
-            gent[gi] = (int)soff.size();
-            for (int tk = gi * G; tk < std::min(nt, (gi + 1) * G); ++tk) {
-                toff[tk] = se * P.m2l_tgt[tk];
-                soff.push_back(toff[tk]);
-                code.push_back(13);
-                for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
-                    soff.push_back(se * P.m2l_src[j]);
-                    code.push_back(mi[j]);
-                }
-                code.back() |= 1 << 5;
-            }
-            while ((soff.size() - gent[gi]) % fgt::kM2lBatch) {
-                soff.push_back(se * P.nslots);   // the zero slot
-                code.push_back(13);
-            }
-            if ((int)soff.size() - gent[gi] > fgt::kM2lMaxEntries)
-                throw std::runtime_error("STEP 3 group entry stream longer than kM2lMaxEntries");
-        }
-        gent[ng] = (int)soff.size();
-        S.n_m2l_groups = ng;
+        std::vector<long long> toff, soff;
+        std::vector<int> gent, code;
+        S.n_m2l_groups = m2l_streams(P, fgt::m2l_group(), toff, gent, soff, code);
         S.d_m2l_toff.upload(toff);
+        S.h.m2l_toff = toff;
         S.d_m2l_poff.upload(gent);
+        S.h.m2l_gent = gent;
         S.d_m2l_soff.upload(soff);
+        S.h.m2l_soff = soff;
         S.d_m2l_so.upload(code);
+        S.h.m2l_code = code;
     }
 
     // near-field targets, longest list first (one block per target: the
@@ -549,11 +608,17 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
         }
     }
     S.d_dir_tgt.upload(dt);
+    S.h.dir_tgt = dt;
     S.d_dir_lev.upload(dl);
+    S.h.dir_lev = dl;
     S.d_dir_off.upload(doff);
+    S.h.dir_off = doff;
     S.d_dir_src.upload(ds);
+    S.h.dir_src = ds;
     S.d_dir_code.upload(dc);
+    S.h.dir_code = dc;
     S.d_dir_flags.upload(df);
+    S.h.dir_flags = df;
 }
 
 // Output components of ifpgh and the table kind each uses per axis
@@ -608,8 +673,11 @@ struct ExecOverride {
     double2* expn;
 };
 
+// kernel_zero: clear the outputs with a fill kernel instead of
+// hipMemsetAsync (work that may be captured into a graph records no memset
+// nodes).
 void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
-             const ExecOverride* ov = nullptr) {
+             const ExecOverride* ov = nullptr, bool kernel_zero = false) {
     const fgt::Plan& P = S.plan;
     const fgt::Geom& g = S.geom;
     const fgt::DevTables& t = S.tabs;
@@ -622,7 +690,11 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
         for (auto& e : evn) hcheck(hipEventCreate(&e), "hipEventCreate");
     }
     auto mark = [&](int i) { if (timed) hcheck(hipEventRecord(ev[i], st), "hipEventRecord"); };
-    if (!ov) {
+    if (!ov && kernel_zero) {
+        fgt::launch_zero(S.d_pot.p, S.d_pot.bytes, st);
+        if (S.ifpgh >= 2) fgt::launch_zero(S.d_grad.p, S.d_grad.bytes, st);
+        if (S.ifpgh >= 3) fgt::launch_zero(S.d_hess.p, S.d_hess.bytes, st);
+    } else if (!ov) {
         hcheck(hipMemsetAsync(S.d_pot.p, 0, S.d_pot.bytes, st), "hipMemsetAsync pot");
         if (S.ifpgh >= 2) hcheck(hipMemsetAsync(S.d_grad.p, 0, S.d_grad.bytes, st), "hipMemsetAsync grad");
         if (S.ifpgh >= 3) hcheck(hipMemsetAsync(S.d_hess.p, 0, S.d_hess.bytes, st), "hipMemsetAsync hess");
@@ -844,6 +916,470 @@ int guarded(F&& f) {
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------ batch
+// Many independent problems (uneven sizes, orders, delta) solved together.
+// Problems on the 3-D register-blocked path are grouped into shape classes
+// (norder, densities, precision, plane-wave count rounded up to 16); every
+// stage of a class is one launch over the tasks of all its problems, the
+// kernels reading each task's tables, values and expansions through a
+// per-problem descriptor.  STEP 2 and STEP 4 keep each problem's level
+// order: phase p of STEP 2 holds the parents p levels above each problem's
+// finest parent level, phase p of STEP 4 the parents p levels below each
+// problem's first plane-wave level.  Other problems (2-D, other orders)
+// run their own executes on side streams.  The whole sequence is captured
+// once into a HIP graph and replayed.
+
+// One shape class of a batch.
+struct BClass {
+    int n = 0, nd = 0, prec = 0, npw_cap = 0;
+    fgt::Geom g{};          // geometry with the class's largest plane-wave count
+    fgt::DevTables t{};     // class-wide fields; tables per problem come from pp
+    std::vector<int> members;
+    DBuf pp;
+    int group = fgt::kM2lGroup;
+    DBuf fz_lev, fz_slot, fz_kids, fz_ksg, fz_prob;
+    int n_fz = 0;
+    DBuf s1_box, s1_lev, s1_slot, s1_prob;
+    int n_s1 = 0;
+    DBuf mp_slot, mp_first, mp_count, mp_child, mp_sign, mp_acc, mp_prob, mp_klev;
+    std::vector<int> mp_phase;    // phase p: entries [mp_phase[p], mp_phase[p+1])
+    DBuf m2l_toff, m2l_gent, m2l_soff, m2l_code, m2l_prob;
+    int n_m2l = 0;
+    DBuf pcs_parent, pcs_child, pcs_sign, pcs_prob, pcs_klev;
+    std::vector<int> pcs_phase;
+    DBuf s5, s5_prob;
+    int n_s5 = 0, n_s5_sib = 0;
+    DBuf dir_tgt, dir_lev, dir_off, dir_src, dir_code, dir_flags, dir_prob;
+    int n_dir = 0;
+};
+
+// Tuned per-class choices, kept for the process: key (norder, densities,
+// precision, plane-wave bucket) -> (STEP 3 targets per block, STEP 1 and
+// STEP 5 x-stage variants).
+struct Tuned {
+    int group, xs1, xs5;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, Tuned> g_tuned;
+
+}  // namespace
+
+struct fgt_batch {
+    std::vector<std::unique_ptr<fgt_solver>> solvers;
+    std::vector<std::unique_ptr<BClass>> classes;
+    std::vector<int> loose;            // problems executed one by one
+    std::vector<hipStream_t> streams;  // [0] stages, [1] near field, [2..] loose problems
+    hipEvent_t ev_fork = nullptr, ev_near = nullptr;
+    std::vector<hipEvent_t> ev_loose;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = true;
+    double last_ms = 0.0;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+    ~fgt_batch() {
+        drop_graph();
+        for (auto s : streams) (void)hipStreamDestroy(s);
+        for (auto e : ev_loose) (void)hipEventDestroy(e);
+        for (auto e : {ev_fork, ev_near, t0, t1})
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+bool class_member(const fgt_solver& S) {
+    const fgt::Plan& P = S.plan;
+    return P.ndim == 3 && (P.n == 8 || P.n == 12 || P.n == 16) && (S.fast3d || !P.do_pw);
+}
+
+// STEP 3 lists of a class at targets-per-block G (rebuilt when G changes)
+void class_m2l(fgt_batch& B, BClass& C) {
+    std::vector<long long> toff, soff;
+    std::vector<int> gent(1, 0), code, prob;
+    for (int q = 0; q < (int)C.members.size(); ++q) {
+        const fgt_solver& S = *B.solvers[C.members[q]];
+        if (!S.plan.do_pw) continue;
+        std::vector<long long> to, so;
+        std::vector<int> ge, co;
+        const int ng = m2l_streams(S.plan, C.group, to, ge, so, co);
+        const int base = (int)soff.size();
+        toff.insert(toff.end(), to.begin(), to.end());
+        for (int gi = 0; gi < ng; ++gi) {
+            gent.back() = base + ge[gi];
+            gent.push_back(0);
+            prob.push_back(q);
+        }
+        soff.insert(soff.end(), so.begin(), so.end());
+        code.insert(code.end(), co.begin(), co.end());
+        gent.back() = (int)soff.size();
+    }
+    C.n_m2l = (int)prob.size();
+    C.m2l_toff.upload(toff);
+    C.m2l_gent.upload(gent);
+    C.m2l_soff.upload(soff);
+    C.m2l_code.upload(code);
+    C.m2l_prob.upload(prob);
+}
+
+void build_classes(fgt_batch& B) {
+    B.drop_graph();
+    B.classes.clear();
+    B.loose.clear();
+    std::map<std::tuple<int, int, int, int>, int> idx;
+    for (int i = 0; i < (int)B.solvers.size(); ++i) {
+        const fgt_solver& S = *B.solvers[i];
+        if (!class_member(S)) {
+            B.loose.push_back(i);
+            continue;
+        }
+        const int cap = S.plan.do_pw ? (S.plan.npw + 15) / 16 * 16 : 0;
+        const auto key = std::make_tuple(S.plan.n, S.plan.nd, S.tabs.prec, cap);
+        auto it = idx.find(key);
+        if (it == idx.end()) {
+            it = idx.emplace(key, (int)B.classes.size()).first;
+            B.classes.push_back(std::make_unique<BClass>());
+            BClass& C = *B.classes.back();
+            C.n = S.plan.n;
+            C.nd = S.plan.nd;
+            C.prec = S.tabs.prec;
+            C.npw_cap = cap;
+        }
+        B.classes[it->second]->members.push_back(i);
+    }
+    for (auto& cp : B.classes) {
+        BClass& C = *cp;
+        // geometry and class-wide table fields from the member with the most plane waves
+        int big = C.members[0];
+        for (int i : C.members)
+            if (B.solvers[i]->plan.npw > B.solvers[big]->plan.npw) big = i;
+        C.g = B.solvers[big]->geom;
+        C.t = B.solvers[big]->tabs;
+        C.t.p2pw = C.t.p2pwS = C.t.pw2pot = C.t.pw2potT = C.t.pw2potS = C.t.wms = C.t.wm2l = nullptr;
+        C.t.loctab = nullptr;
+        C.t.pw2potS_ks = C.t.loctab_ks = 0;   // the value tables (kind 0) only
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(std::make_tuple(C.n, C.nd, C.prec, C.npw_cap));
+            if (it != g_tuned.end()) {
+                C.group = it->second.group;
+                C.t.xs1 = it->second.xs1;
+                C.t.xs5 = it->second.xs5;
+            }
+        }
+        std::vector<fgt::ProbPtrs> pp;
+        std::vector<int> fzl, fzs, fzk, fzg, fzp, s1b, s1l, s1s, s1p;
+        std::vector<int> mch, msg, pcsp, pcsc, pcsg;
+        std::vector<fgt::PotTask> sib, own;
+        std::vector<int> sibp, ownp;
+        int maxph2 = 0, maxph4 = 0;
+        for (int q = 0; q < (int)C.members.size(); ++q) {
+            fgt_solver& S = *B.solvers[C.members[q]];
+            const fgt::Plan& P = S.plan;
+            fgt::ProbPtrs r{};
+            r.g = S.geom;
+            r.p2pw = S.d_p2pw.as<double2>();
+            r.p2pwS = S.d_p2pwS.as<double2>();
+            r.pw2potS = S.d_pw2potS.as<double2>();
+            r.wms = S.d_wms.as<double2>();
+            r.wm2l = S.d_wm2l.as<double2>();
+            r.loctab = S.d_loctab.as<double>();
+            r.fvals = S.d_fvals.as<double>();
+            r.pot = S.d_pot.as<double>();
+            r.expn = S.d_expn.p;
+            r.zero_off = 2LL * P.nd * P.nexp * P.nslots;
+            pp.push_back(r);
+            if (!P.do_pw) continue;
+            const auto& h = S.h;
+            fzl.insert(fzl.end(), h.fz_lev.begin(), h.fz_lev.end());
+            fzs.insert(fzs.end(), h.fz_slot.begin(), h.fz_slot.end());
+            fzk.insert(fzk.end(), h.fz_kids.begin(), h.fz_kids.end());
+            fzg.insert(fzg.end(), h.fz_ksg.begin(), h.fz_ksg.end());
+            fzp.insert(fzp.end(), h.fz_lev.size(), q);
+            s1b.insert(s1b.end(), h.s1_box.begin(), h.s1_box.end());
+            s1l.insert(s1l.end(), h.s1_lev.begin(), h.s1_lev.end());
+            s1s.insert(s1s.end(), h.s1_slot.begin(), h.s1_slot.end());
+            s1p.insert(s1p.end(), h.s1_box.size(), q);
+            maxph2 = std::max(maxph2, P.nlevels - P.nlevstart);
+            maxph4 = std::max(maxph4, P.nlevels - P.nlevstart);
+            sib.insert(sib.end(), h.s5.begin(), h.s5.begin() + S.n_s5_sib);
+            sibp.insert(sibp.end(), S.n_s5_sib, q);
+            own.insert(own.end(), h.s5.begin() + S.n_s5_sib, h.s5.end());
+            ownp.insert(ownp.end(), h.s5.size() - S.n_s5_sib, q);
+        }
+        C.pp.upload(pp);
+        C.n_fz = (int)fzp.size();
+        C.fz_lev.upload(fzl);
+        C.fz_slot.upload(fzs);
+        C.fz_kids.upload(fzk);
+        C.fz_ksg.upload(fzg);
+        C.fz_prob.upload(fzp);
+        C.n_s1 = (int)s1p.size();
+        C.s1_box.upload(s1b);
+        C.s1_lev.upload(s1l);
+        C.s1_slot.upload(s1s);
+        C.s1_prob.upload(s1p);
+        // STEP 2 phases (children arrays appended once per problem)
+        std::vector<int> ms, mf, mc, ma, mp, mk;
+        std::vector<int> chbase(C.members.size(), 0);
+        for (int q = 0; q < (int)C.members.size(); ++q) {
+            const fgt_solver& S = *B.solvers[C.members[q]];
+            chbase[q] = (int)mch.size();
+            mch.insert(mch.end(), S.h.mp_child.begin(), S.h.mp_child.end());
+            msg.insert(msg.end(), S.h.mp_sign.begin(), S.h.mp_sign.end());
+        }
+        C.mp_phase.assign(1, 0);
+        for (int ph = 0; ph < maxph2; ++ph) {
+            for (int q = 0; q < (int)C.members.size(); ++q) {
+                const fgt_solver& S = *B.solvers[C.members[q]];
+                const fgt::Plan& P = S.plan;
+                if (!P.do_pw) continue;
+                const int l = P.nlevels - 1 - ph;
+                if (l < P.nlevstart) continue;
+                for (int e = S.mp_level_off[l]; e < S.mp_level_off[l + 1]; ++e) {
+                    ms.push_back(S.h.mp_slot[e]);
+                    mf.push_back(chbase[q] + S.h.mp_first[e]);
+                    mc.push_back(S.h.mp_count[e]);
+                    ma.push_back(S.h.mp_acc[e]);
+                    mp.push_back(q);
+                    mk.push_back(P.nlevels - l);
+                }
+            }
+            C.mp_phase.push_back((int)ms.size());
+        }
+        C.mp_slot.upload(ms);
+        C.mp_first.upload(mf);
+        C.mp_count.upload(mc);
+        C.mp_acc.upload(ma);
+        C.mp_prob.upload(mp);
+        C.mp_klev.upload(mk);
+        C.mp_child.upload(mch);
+        C.mp_sign.upload(msg);
+        // STEP 4 phases
+        std::vector<int> pp4, pk4;
+        C.pcs_phase.assign(1, 0);
+        for (int ph = 0; ph < maxph4; ++ph) {
+            for (int q = 0; q < (int)C.members.size(); ++q) {
+                const fgt_solver& S = *B.solvers[C.members[q]];
+                const fgt::Plan& P = S.plan;
+                if (!P.do_pw) continue;
+                const int l = P.nlevstart + ph;
+                if (l > P.nlevels - 1) continue;
+                for (int e = S.pcs_level_off[l]; e < S.pcs_level_off[l + 1]; ++e) {
+                    pcsp.push_back(S.h.pcs_parent[e]);
+                    pcsc.push_back(S.h.pcs_child[e]);
+                    pcsg.push_back(S.h.pcs_sign[e]);
+                    pp4.push_back(q);
+                    pk4.push_back(P.nlevels - l);
+                }
+            }
+            C.pcs_phase.push_back((int)pcsp.size());
+        }
+        C.pcs_parent.upload(pcsp);
+        C.pcs_child.upload(pcsc);
+        C.pcs_sign.upload(pcsg);
+        C.pcs_prob.upload(pp4);
+        C.pcs_klev.upload(pk4);
+        // STEP 3
+        class_m2l(B, C);
+        // STEP 5: every sibling task first, then the one-leaf tasks
+        C.n_s5_sib = (int)sib.size();
+        sib.insert(sib.end(), own.begin(), own.end());
+        sibp.insert(sibp.end(), ownp.begin(), ownp.end());
+        C.n_s5 = (int)sib.size();
+        C.s5.upload(sib);
+        C.s5_prob.upload(sibp);
+        // STEP 6: every target of every problem, longest list first
+        struct DT {
+            int q, k, len;
+        };
+        std::vector<DT> order;
+        for (int q = 0; q < (int)C.members.size(); ++q) {
+            const fgt_solver& S = *B.solvers[C.members[q]];
+            for (int k = 0; k < (int)S.h.dir_tgt.size(); ++k)
+                order.push_back({q, k, S.h.dir_off[k + 1] - S.h.dir_off[k]});
+        }
+        std::stable_sort(order.begin(), order.end(), [](const DT& a, const DT& b) { return a.len > b.len; });
+        std::vector<int> dt, dl, doff(1, 0), ds, dc, df, dp;
+        for (const DT& o : order) {
+            const auto& h = B.solvers[C.members[o.q]]->h;
+            dt.push_back(h.dir_tgt[o.k]);
+            dl.push_back(h.dir_lev[o.k]);
+            dp.push_back(o.q);
+            for (int j = h.dir_off[o.k]; j < h.dir_off[o.k + 1]; ++j) {
+                ds.push_back(h.dir_src[j]);
+                dc.push_back(h.dir_code[j]);
+                df.push_back(h.dir_flags[j]);
+            }
+            doff.push_back((int)ds.size());
+        }
+        C.n_dir = (int)dt.size();
+        C.dir_tgt.upload(dt);
+        C.dir_lev.upload(dl);
+        C.dir_off.upload(doff);
+        C.dir_src.upload(ds);
+        C.dir_code.upload(dc);
+        C.dir_flags.upload(df);
+        C.dir_prob.upload(dp);
+    }
+}
+
+fgt::Multi multi(const BClass& C, const DBuf& prob, int off = 0, const DBuf* klev = nullptr) {
+    fgt::Multi m;
+    m.pp = C.pp.as<fgt::ProbPtrs>();
+    m.tprob = prob.as<int>() + off;
+    m.tklev = klev ? klev->as<int>() + off : nullptr;
+    return m;
+}
+
+// Stages of one class (timed: per-stage events on st, for the tuner)
+void class_step1(const BClass& C, hipStream_t st) {
+    if (C.n_fz > 0 && !fgt::p2pw_fused_3d(C.g, C.t, C.n_fz, C.fz_lev.as<int>(), C.fz_slot.as<int>(),
+                                          C.fz_kids.as<int>(), C.fz_ksg.as<int>(), nullptr, nullptr, st,
+                                          multi(C, C.fz_prob)))
+        throw std::runtime_error("fused STEP 1+2 kernel not available for a batch class");
+    if (C.n_s1 > 0)
+        fgt::launch_p2pw(C.g, C.t, C.n_s1, C.s1_box.as<int>(), C.s1_lev.as<int>(), C.s1_slot.as<int>(), nullptr,
+                         nullptr, st, multi(C, C.s1_prob));
+}
+void class_step2(const BClass& C, hipStream_t st) {
+    for (size_t ph = 0; ph + 1 < C.mp_phase.size(); ++ph) {
+        const int a = C.mp_phase[ph], b = C.mp_phase[ph + 1];
+        if (b > a)
+            fgt::launch_merge(C.g, C.t, b - a, C.mp_slot.as<int>() + a, C.mp_first.as<int>() + a,
+                              C.mp_count.as<int>() + a, C.mp_child.as<int>(), C.mp_sign.as<int>(),
+                              C.mp_acc.as<int>() + a, 1, nullptr, st, multi(C, C.mp_prob, a, &C.mp_klev));
+    }
+}
+void class_step3(const BClass& C, hipStream_t st) {
+    fgt::launch_m2l27(C.g, C.t, C.n_m2l, C.group, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
+                      C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob));
+}
+void class_step4(const BClass& C, hipStream_t st) {
+    for (size_t ph = 0; ph + 1 < C.pcs_phase.size(); ++ph) {
+        const int a = C.pcs_phase[ph], b = C.pcs_phase[ph + 1];
+        if (b > a)
+            fgt::launch_split(C.g, C.t, b - a, C.pcs_parent.as<int>() + a, C.pcs_child.as<int>() + a,
+                              C.pcs_sign.as<int>() + a, 1, nullptr, st, multi(C, C.pcs_prob, a, &C.pcs_klev));
+    }
+}
+void class_step5(const BClass& C, hipStream_t st) {
+    if (C.n_s5 > 0)
+        fgt::launch_pw2pot_tasks(C.g, C.t, C.n_s5, C.n_s5_sib, C.s5.as<fgt::PotTask>(), nullptr, fgt::Kinds{0, 0, 0},
+                                 fgt::OutDesc{nullptr, C.nd, 0}, st, multi(C, C.s5_prob));
+}
+void class_step6(const BClass& C, hipStream_t st) {
+    if (C.n_dir > 0)
+        fgt::launch_direct(C.g, C.t, C.n_dir, C.dir_tgt.as<int>(), C.dir_lev.as<int>(), C.dir_off.as<int>(),
+                           C.dir_src.as<int>(), C.dir_code.as<int>(), C.dir_flags.as<int>(), nullptr,
+                           fgt::Kinds{0, 0, 0}, fgt::OutDesc{nullptr, C.nd, 0}, st, multi(C, C.dir_prob));
+}
+
+// Enqueue one whole batch transform (graph capture records this sequence).
+void batch_enqueue(fgt_batch& B) {
+    hipStream_t st = B.streams[0], sn = B.streams[1];
+    for (auto& S : B.solvers) fgt::launch_zero(S->d_pot.p, S->d_pot.bytes, st);
+    hcheck(hipEventRecord(B.ev_fork, st), "hipEventRecord");
+    for (size_t k = 1; k < B.streams.size(); ++k)
+        hcheck(hipStreamWaitEvent(B.streams[k], B.ev_fork, 0), "hipStreamWaitEvent");
+    // near field of every class beside STEPs 1-4
+    for (auto& C : B.classes) class_step6(*C, sn);
+    hcheck(hipEventRecord(B.ev_near, sn), "hipEventRecord");
+    // problems off the class path, spread over the side streams (largest first)
+    const size_t nside = B.streams.size() - 2;
+    for (size_t i = 0; i < B.loose.size(); ++i) {
+        fgt_solver& S = *B.solvers[B.loose[i]];
+        execute(S, B.streams[2 + i % nside], false, false, nullptr, true);
+    }
+    for (auto& C : B.classes) {
+        if (C->n_fz + C->n_s1 + C->n_m2l + C->n_s5 == 0) continue;
+        class_step1(*C, st);
+        class_step2(*C, st);
+        class_step3(*C, st);
+        class_step4(*C, st);
+    }
+    hcheck(hipStreamWaitEvent(st, B.ev_near, 0), "hipStreamWaitEvent");
+    for (auto& C : B.classes) class_step5(*C, st);
+    for (size_t k = 2; k < B.streams.size(); ++k) {
+        hcheck(hipEventRecord(B.ev_loose[k - 2], B.streams[k]), "hipEventRecord");
+        hcheck(hipStreamWaitEvent(st, B.ev_loose[k - 2], 0), "hipStreamWaitEvent");
+    }
+}
+
+// Device time of f(st) over reps runs (ms per run)
+template <class F>
+float time_ms(hipStream_t st, int reps, F&& f) {
+    hipEvent_t a, b;
+    hcheck(hipEventCreate(&a), "hipEventCreate");
+    hcheck(hipEventCreate(&b), "hipEventCreate");
+    f(st);   // warm-up
+    hcheck(hipEventRecord(a, st), "hipEventRecord");
+    for (int r = 0; r < reps; ++r) f(st);
+    hcheck(hipEventRecord(b, st), "hipEventRecord");
+    hcheck(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0;
+    hcheck(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+// Picks, per class, the STEP 3 targets per block and the STEP 1 / STEP 5 x
+// stage variant with the shortest device time on the batch's own data.
+void batch_autotune(fgt_batch& B, int reps) {
+    hipStream_t st = B.streams[0];
+    hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    B.drop_graph();
+    batch_enqueue(B);   // expansions and potentials valid for the stage timings
+    hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (auto& cp : B.classes) {
+        BClass& C = *cp;
+        if (C.n_fz + C.n_s1 + C.n_m2l + C.n_s5 == 0) continue;
+        if (C.n_m2l > 0) {
+            int best = C.group;
+            float bt = 1e30f;
+            for (int G : {2, 3, 4, 6, 8}) {
+                C.group = G;
+                class_m2l(B, C);
+                const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
+                if (ms < bt) bt = ms, best = G;
+            }
+            C.group = best;
+            class_m2l(B, C);
+        }
+        {
+            std::vector<int> cand{0};
+            if (C.n >= 12 && C.t.npw <= 64) cand.push_back(1);
+            if (C.prec >= 1 && C.t.npw <= 80) cand.push_back(2);
+            int best = cand[0];
+            float bt = 1e30f;
+            for (int x : cand) {
+                C.t.xs5 = x;
+                const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step5(C, s); });
+                if (ms < bt) bt = ms, best = x;
+            }
+            C.t.xs5 = best;
+        }
+        if (C.prec >= 1) {
+            int best = 0;
+            float bt = 1e30f;
+            for (int x : {0, 2}) {
+                C.t.xs1 = x;
+                const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step1(C, s); });
+                if (ms < bt) bt = ms, best = x;
+            }
+            C.t.xs1 = best;
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[std::make_tuple(C.n, C.nd, C.prec, C.npw_cap)] = Tuned{C.group, C.t.xs1, C.t.xs5};
+    }
 }
 
 }  // namespace
@@ -1095,7 +1631,6 @@ int fgt_batch_create(int nprob, const fgt_problem* probs, int nstreams, fgt_batc
         if (nprob < 1 || !probs) throw std::invalid_argument("empty batch");
         if (nstreams < 1 || nstreams > 16) throw std::invalid_argument("nstreams must be 1..16");
         auto B = std::make_unique<fgt_batch>();
-        std::vector<double> work(nprob);
         for (int i = 0; i < nprob; ++i) {
             const fgt_problem& q = probs[i];
             if (!q.itree || !q.iptr || !q.centers || !q.boxsize) throw std::invalid_argument("null tree array");
@@ -1105,26 +1640,25 @@ int fgt_batch_create(int nprob, const fgt_problem* probs, int nstreams, fgt_batc
                               q.nlevels, q.ltree, q.itree, q.iptr, q.centers, q.boxsize};
             auto S = std::make_unique<fgt_solver>();
             build_solver(*S, in);
-            work[i] = (double)S->plan.nd * S->plan.npbox * S->plan.nleaves;
             B->solvers.push_back(std::move(S));
         }
-        for (int k = 0; k < nstreams; ++k) {
+        // stream 0: class stages; 1: near field; 2..: problems off the class path
+        for (int k = 0; k < 2 + nstreams; ++k) {
             hipStream_t st;
             hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
             B->streams.push_back(st);
         }
-        B->order.resize(nprob);
-        for (int i = 0; i < nprob; ++i) B->order[i] = i;
-        std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
-        std::vector<double> load(nstreams, 0.0);
-        B->stream_of.assign(nprob, 0);
-        for (int i : B->order) {
-            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            B->stream_of[i] = k;
-            load[k] += work[i];
-        }
+        hcheck(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming), "hipEventCreate");
+        hcheck(hipEventCreateWithFlags(&B->ev_near, hipEventDisableTiming), "hipEventCreate");
+        B->ev_loose.assign(nstreams, nullptr);
+        for (auto& e : B->ev_loose) hcheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         hcheck(hipEventCreate(&B->t0), "hipEventCreate");
         hcheck(hipEventCreate(&B->t1), "hipEventCreate");
+        // loose problems largest first (they go round-robin on the side streams)
+        build_classes(*B);
+        std::stable_sort(B->loose.begin(), B->loose.end(), [&](int x, int y) {
+            return B->solvers[x]->plan.nleaves > B->solvers[y]->plan.nleaves;
+        });
         *out = B.release();
     });
 }
@@ -1146,6 +1680,12 @@ int fgt_batch_set_precision(fgt_batch* b, int mode, double eps_min, int* nmixed)
             if (fgt_solver_set_precision(S.get(), m) != 0) throw std::runtime_error(g_msg);
             cnt += m > 0;
         }
+        // expansion buffers and class keys changed: regroup
+        const std::vector<int> loose_order = b->loose;
+        build_classes(*b);
+        std::stable_sort(b->loose.begin(), b->loose.end(), [&](int x, int y) {
+            return b->solvers[x]->plan.nleaves > b->solvers[y]->plan.nleaves;
+        });
         if (nmixed) *nmixed = cnt;
     });
 }
@@ -1159,24 +1699,58 @@ int fgt_batch_set_fvals(fgt_batch* b, int i, const double* fvals) {
     });
 }
 
+int fgt_batch_set_graph(fgt_batch* b, int on) {
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        b->use_graph = on != 0;
+        b->drop_graph();
+    });
+}
+
+int fgt_batch_autotune(fgt_batch* b, int reps) {
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        if (reps < 1 || reps > 100) throw std::invalid_argument("reps must be 1..100");
+        batch_autotune(*b, reps);
+    });
+}
+
+int fgt_batch_class_info(const fgt_batch* b, int k, int* info) {
+    return guarded([&] {
+        if (!b || !info) throw std::invalid_argument("null argument");
+        if (k < 0 || k >= (int)b->classes.size()) throw std::invalid_argument("no such class");
+        const BClass& C = *b->classes[k];
+        const int v[10] = {C.n, C.nd, C.prec, C.npw_cap, (int)C.members.size(), C.group, C.t.xs1, C.t.xs5,
+                           C.n_s5, C.n_dir};
+        std::memcpy(info, v, sizeof v);
+    });
+}
+
 int fgt_batch_execute(fgt_batch* b) {
     return guarded([&] {
         if (!b) throw std::invalid_argument("null batch");
-        // all streams start after t0 and t1 waits for all of them
+        hipStream_t st = b->streams[0];
+        // earlier work on any stream (uploads, other executes) comes first
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        hcheck(hipEventRecord(b->t0, b->streams[0]), "hipEventRecord");
-        for (size_t k = 1; k < b->streams.size(); ++k)
-            hcheck(hipStreamWaitEvent(b->streams[k], b->t0, 0), "hipStreamWaitEvent");
-        for (int i : b->order) execute(*b->solvers[i], b->streams[b->stream_of[i]], false);
-        std::vector<hipEvent_t> done(b->streams.size());
-        for (size_t k = 1; k < b->streams.size(); ++k) {
-            hcheck(hipEventCreate(&done[k]), "hipEventCreate");
-            hcheck(hipEventRecord(done[k], b->streams[k]), "hipEventRecord");
-            hcheck(hipStreamWaitEvent(b->streams[0], done[k], 0), "hipStreamWaitEvent");
+        if (b->use_graph && !b->gexec) {
+            hcheck(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                batch_enqueue(*b);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hcheck(hipStreamEndCapture(st, &b->graph), "hipStreamEndCapture");
+            hcheck(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         }
-        hcheck(hipEventRecord(b->t1, b->streams[0]), "hipEventRecord");
+        hcheck(hipEventRecord(b->t0, st), "hipEventRecord");
+        if (b->use_graph) hcheck(hipGraphLaunch(b->gexec, st), "hipGraphLaunch");
+        else batch_enqueue(*b);
+        hcheck(hipEventRecord(b->t1, st), "hipEventRecord");
         hcheck(hipEventSynchronize(b->t1), "hipEventSynchronize");
-        for (size_t k = 1; k < done.size(); ++k) (void)hipEventDestroy(done[k]);
         float ms = 0;
         hcheck(hipEventElapsedTime(&ms, b->t0, b->t1), "hipEventElapsedTime");
         b->last_ms = ms;
@@ -1198,6 +1772,8 @@ int fgt_batch_stats(const fgt_batch* b, long long* nout, double* last_ms) {
     if (last_ms) *last_ms = b->last_ms;
     return 0;
 }
+
+int fgt_batch_nclasses(const fgt_batch* b) { return b ? (int)b->classes.size() : -1; }
 
 void fgt_batch_destroy(fgt_batch* b) { delete b; }
 

@@ -71,6 +71,32 @@ struct PotTask {
     int code[4];
 };
 
+// Several problems of one shape class (3-D, same norder, density count and
+// precision; plane-wave counts may differ) in one launch.  Task i of a launch belongs
+// to problem tprob[i]; the kernel then takes that problem's tables, values,
+// outputs and expansion workspace from pp[tprob[i]] instead of its
+// single-problem arguments.  tklev[i]: the merge/split level code of task i
+// (STEPs 2 and 4, whose phase rows depend on the problem's depth).  All
+// pointers device pointers; pp == nullptr means a single-problem launch.
+struct ProbPtrs {
+    Geom g;                   // the problem's geometry (plane-wave count, nexp)
+    const double2* p2pw;      // value->PW tables [l][k][m]
+    const double2* p2pwS;     // the same times the merge phases [l][v][k][m]
+    const double2* pw2potS;   // PW->value tables [l][v][k][i] (value kind)
+    const double2* wms;       // merge/split 1-D phases [klev-1][k]
+    const double2* wm2l;      // M2L 1-D phases [o][k]
+    const double* loctab;     // near-field tables [l][slot][k][j] (value kind)
+    const double* fvals;      // (nd, npbox, nboxes)
+    double* pot;              // (nd, npbox, nboxes), accumulated into
+    void* expn;               // expansion slots (FP64 or FP32 complex)
+    long long zero_off;       // element offset of the all-zero slot
+};
+struct Multi {
+    const ProbPtrs* pp = nullptr;
+    const int* tprob = nullptr;
+    const int* tklev = nullptr;
+};
+
 // Device-resident tables (all pointers are device pointers).
 struct DevTables {
     // per level l: value->PW  T[l][k][m] (k < npw, m < n) complex (double2)
@@ -103,16 +129,22 @@ struct DevTables {
     // contraction precision of the mixed-precision forms: 0 FP64, 1 FP32
     // matrix cores, 2 BF16 hi/lo matrix cores (prec >= 1 implies expn_f32)
     int prec;
+    // x-stage variant of STEP 1 / STEP 5 (tuned per shape class; -1 = the
+    // default policy): 0 VALU, 1 FP64 matrix cores (STEP 5, norder >= 12,
+    // npw <= 64), 2 the reduced-precision matrix cores (prec >= 1)
+    int xs1 = -1, xs5 = -1;
     size_t lds_limit;  // device LDS bytes per block
 };
 
 // Register-blocked 3-D STEP 1 / STEP 5 kernels (norder 8, 12, 16); return
 // false (nothing launched) when the shape is not covered.
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit);
+                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit,
+                    const Multi& mu = Multi{});
 // tasks [0, nsib): sibling tasks (up to four leaves); [nsib, ntask): one leaf
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit);
+                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit,
+                      const Multi& mu = Multi{});
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 // true when FGT_PW2POT_VALU=1: STEP 5 keeps its x stage on the VALU
 bool pw2pot_valu_only();
@@ -120,16 +152,18 @@ bool pw2pot_valu_only();
 // plev/pslot per parent, kids/ksg 8 per parent (leaf child box or -1, merge
 // sign bits).  Writes (overwrites) the parent's mp; returns false if not covered.
 bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev, const int* pslot,
-                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s);
+                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s,
+                   const Multi& mu = Multi{});
 
 // Stage launchers (stream-ordered; all arguments device pointers).
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box,
                  const int* lev, const int* slot, const double* fvals,
-                 double2* expn, hipStream_t s);
+                 double2* expn, hipStream_t s, const Multi& mu = Multi{});
 // par_acc (nullable): per parent, nonzero = add to the parent's current mp
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child,
-                  const int* pc_sign, const int* par_acc, int klev, double2* expn, hipStream_t s);
+                  const int* pc_sign, const int* par_acc, int klev, double2* expn, hipStream_t s,
+                  const Multi& mu = Multi{});
 // STEP 3, kM2lGroup targets per block.  toff[i]: element offset of target
 // i's slot (slot * 2 * nd * nexp); gent: per group, the range of its entry
 // stream (each target's own expansion, then its list; padded to a multiple
@@ -148,24 +182,25 @@ constexpr int kM2lBatch = 4;
 constexpr int kM2lMaxEntries = kM2lGroupMax * 27 + kM2lBatch;
 void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
                   const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
-                  hipStream_t s);
+                  hipStream_t s, const Multi& mu = Multi{});
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
-                  hipStream_t s);
+                  hipStream_t s, const Multi& mu = Multi{});
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
                    const int* lev, const int* slot, const double2* expn,
                    Kinds kd, OutDesc od, hipStream_t s);
 // 3-D STEP 5 over PotTask lists (throws when the shape is not covered)
 void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s);
+                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu = Multi{});
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
                    const int* lev, const int* off, const int* src, const int* code,
-                   const int* flags, const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
+                   const int* flags, const double* fvals, Kinds kd, OutDesc od, hipStream_t s,
+                   const Multi& mu = Multi{});
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
-                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s);
+                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu = Multi{});
 
 // Device workspace of the no-allocation entry point: a file-scope device
 // array of kScratchBytes per GPU.  scratch_region() returns its address on

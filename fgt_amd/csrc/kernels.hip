@@ -181,14 +181,21 @@ __global__ void __launch_bounds__(kBlock) k_merge(Geom g, DevTables t, const int
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign,
                                                   const int* __restrict__ par_acc, int klev,
-                                                  ET* __restrict__ expn) {
+                                                  ET* __restrict__ expn_a, Multi mu) {
     const int task = blockIdx.y;
+    ET* __restrict__ expn = expn_a;
+    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[task]];
+        g = q.g;
+        expn = static_cast<ET*>(q.expn);
+        w1 = q.wms + (size_t)(mu.tklev[task] - 1) * q.g.npw;
+    }
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)g.nd * g.nexp;
     if (e >= tot) return;
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
     const int first = par_first[task], cnt = par_count[task];
     // par_acc[task] != 0: the parent already holds its leaf children's part
     double2 acc = (par_acc && par_acc[task]) ? ldE(expn[(size_t)par_slot[task] * 2 * tot + e])
@@ -230,9 +237,18 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
                                                   const int* __restrict__ gent,
                                                   const long long* __restrict__ soff,
                                                   const int* __restrict__ scode,
-                                                  ET* __restrict__ expn) {
+                                                  ET* __restrict__ expn_a, Multi mu) {
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int grp = lin % ngroup, chunk = lin / ngroup;
+    ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[grp]];
+        g = q.g;
+        t.npw = q.g.npw;
+        expn = static_cast<ET*>(q.expn);
+        t.wm2l = q.wm2l;
+        zero_off = q.zero_off;
+    }
     const long long tot = (long long)g.nd * g.nexp;
     const int npw = t.npw;
     long long e = (long long)chunk * blockDim.x + threadIdx.x;
@@ -316,14 +332,21 @@ template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign, int klev,
-                                                  ET* __restrict__ expn) {
+                                                  ET* __restrict__ expn_a, Multi mu) {
     const int entry = blockIdx.y;
+    ET* __restrict__ expn = expn_a;
+    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[entry]];
+        g = q.g;
+        expn = static_cast<ET*>(q.expn);
+        w1 = q.wms + (size_t)(mu.tklev[entry] - 1) * q.g.npw;
+    }
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)g.nd * g.nexp;
     if (e >= tot) return;
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
     const double2 v = ldE(expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e]);
     stE(expn + ((size_t)pc_child[entry] * 2 + 1) * tot + e, cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g)));
 }
@@ -591,9 +614,9 @@ unsigned blocks_for(long long tot) { return (unsigned)((tot + kBlock - 1) / kBlo
 }  // namespace
 
 void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                 const int* slot, const double* fvals, double2* expn, hipStream_t s) {
+                 const int* slot, const double* fvals, double2* expn, hipStream_t s, const Multi& mu) {
     if (ntask <= 0) return;
-    if (launch_p2pw_3d(g, t, ntask, box, lev, slot, fvals, expn, s, t.lds_limit)) {
+    if (launch_p2pw_3d(g, t, ntask, box, lev, slot, fvals, expn, s, t.lds_limit, mu)) {
         check(hipGetLastError(), "k_p2pw3 launch");
         return;
     }
@@ -613,19 +636,21 @@ void launch_p2pw(const Geom& g, const DevTables& t, int ntask, const int* box, c
 
 void launch_merge(const Geom& g, const DevTables& t, int ntask, const int* par_slot,
                   const int* par_first, const int* par_count, const int* pc_child, const int* pc_sign,
-                  const int* par_acc, int klev, double2* expn, hipStream_t s) {
+                  const int* par_acc, int klev, double2* expn, hipStream_t s, const Multi& mu) {
     // tasks go on grid y, which holds at most kMaxGridY rows: longer lists
     // are launched in chunks
     for (int a = 0; a < ntask; a += kMaxGridY) {
         const int cnt = std::min(kMaxGridY, ntask - a);
         const dim3 grid(blocks_for((long long)g.nd * g.nexp), cnt);
         const int* acc = par_acc ? par_acc + a : nullptr;
+        Multi ma = mu;
+        if (ma.pp) ma.tprob += a, ma.tklev += a;
         if (t.expn_f32)
             hipLaunchKernelGGL(k_merge<float2>, grid, dim3(kBlock), 0, s, g, t, par_slot + a, par_first + a,
-                               par_count + a, pc_child, pc_sign, acc, klev, reinterpret_cast<float2*>(expn));
+                               par_count + a, pc_child, pc_sign, acc, klev, reinterpret_cast<float2*>(expn), ma);
         else
             hipLaunchKernelGGL(k_merge<double2>, grid, dim3(kBlock), 0, s, g, t, par_slot + a, par_first + a,
-                               par_count + a, pc_child, pc_sign, acc, klev, expn);
+                               par_count + a, pc_child, pc_sign, acc, klev, expn, ma);
         check(hipGetLastError(), "k_merge launch");
     }
 }
@@ -641,38 +666,41 @@ int m2l_group() {
 
 void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
                   const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
-                  hipStream_t s) {
+                  hipStream_t s, const Multi& mu) {
     if (ngroup <= 0) return;
     const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
     if (t.expn_f32)
         hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
-                           soff, scode, reinterpret_cast<float2*>(expn));
+                           soff, scode, reinterpret_cast<float2*>(expn), mu);
     else
         hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
-                           soff, scode, expn);
+                           soff, scode, expn, mu);
     check(hipGetLastError(), "k_m2l27 launch");
 }
 
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
-                  const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s) {
+                  const int* pc_child, const int* pc_sign, int klev, double2* expn, hipStream_t s,
+                  const Multi& mu) {
     for (int a = 0; a < nentry; a += kMaxGridY) {
         const int cnt = std::min(kMaxGridY, nentry - a);
         const dim3 grid(blocks_for((long long)g.nd * g.nexp), cnt);
+        Multi ma = mu;
+        if (ma.pp) ma.tprob += a, ma.tklev += a;
         if (t.expn_f32)
             hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
-                               pc_sign + a, klev, reinterpret_cast<float2*>(expn));
+                               pc_sign + a, klev, reinterpret_cast<float2*>(expn), ma);
         else
             hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
-                               pc_sign + a, klev, expn);
+                               pc_sign + a, klev, expn, ma);
         check(hipGetLastError(), "k_split launch");
     }
 }
 
 void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
+                         const double2* expn, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu) {
     if (ntask <= 0) return;
-    if (!launch_pw2pot_3d(g, t, ntask, nsib, tasks, expn, kd, od, s, t.lds_limit))
+    if (!launch_pw2pot_3d(g, t, ntask, nsib, tasks, expn, kd, od, s, t.lds_limit, mu))
         throw std::runtime_error("3-D STEP 5 kernel not available for this shape");
     check(hipGetLastError(), "k_pw2pot3 launch");
 }
@@ -696,9 +724,9 @@ void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
 
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                    const int* off, const int* src, const int* code, const int* flags, const double* fvals,
-                   Kinds kd, OutDesc od, hipStream_t s) {
+                   Kinds kd, OutDesc od, hipStream_t s, const Multi& mu) {
     if (ntask <= 0) return;
-    if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s)) {
+    if (launch_direct_3d(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu)) {
         check(hipGetLastError(), "k_direct3 launch");
         return;
     }

@@ -21,15 +21,23 @@ namespace {
 
 // flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group
 template <int N>
-__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab,
+__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab_a,
                                                 const int* __restrict__ tgt, const int* __restrict__ lev,
                                                 const int* __restrict__ off, const int* __restrict__ src,
                                                 const int* __restrict__ code, const int* __restrict__ flags,
-                                                const double* __restrict__ fvals, size_t ks, Kinds kd,
-                                                OutDesc od) {
+                                                const double* __restrict__ fvals_a, size_t ks, Kinds kd,
+                                                OutDesc od, Multi mu) {
     extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
     constexpr int NN = N * N;
     const int task = blockIdx.x;
+    const double* __restrict__ loctab = loctab_a;
+    const double* __restrict__ fvals = fvals_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[task]];
+        loctab = q.loctab;
+        fvals = q.fvals;
+        od.p = q.pot;
+    }
     const int tb = tgt[task], l = lev[task];
     // table kind per axis (value / first / second derivative): stride ks
     const double* __restrict__ tabx = loctab + kd.x * ks + (size_t)l * 13 * NN;
@@ -135,11 +143,11 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
 template <int N>
 bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev, const int* off,
                  const int* src, const int* code, const int* flags, const double* fvals, Kinds kd, OutDesc od,
-                 hipStream_t s) {
+                 hipStream_t s, const Multi& mu) {
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
     hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, tgt, lev, off, src, code,
-                       flags, fvals, t.loctab_ks, kd, od);
+                       flags, fvals, t.loctab_ks, kd, od, mu);
     return true;
 }
 
@@ -147,12 +155,12 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
 
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
-                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s) {
+                      const double* fvals, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu) {
     if (g.ndim != 3 || ntask <= 0 || !flags) return false;
     switch (t.n) {
-        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
-        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
-        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s);
+        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
+        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
+        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
         default: return false;
     }
 }

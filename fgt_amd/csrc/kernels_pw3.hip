@@ -190,15 +190,26 @@ __device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, 
 // complex) share one buffer: the y stage reads its sff column into registers
 // and passes a barrier before writing sff2 (more blocks fit on a CU).
 template <int N, int XS, typename ET>
-__global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab, int npw,
+__global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict__ tab_a, int npw,
                                               const int* __restrict__ box, const int* __restrict__ lev,
                                               const int* __restrict__ slot,
-                                              const double* __restrict__ fvals,
-                                              ET* __restrict__ expn) {
+                                              const double* __restrict__ fvals_a,
+                                              ET* __restrict__ expn_a, Multi mu) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
     double2* sff2 = sm;
     const int task = blockIdx.x;
+    const double2* __restrict__ tab = tab_a;
+    const double* __restrict__ fvals = fvals_a;
+    ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[task]];
+        g = q.g;
+        npw = q.g.npw;
+        tab = q.p2pw;
+        fvals = q.fvals;
+        expn = static_cast<ET*>(q.expn);
+    }
     const int b = box[task], l = lev[task], sl = slot[task];
     const double2* __restrict__ T = tab + (size_t)l * npw * N;   // T[k][m]
     const int tid = threadIdx.x;
@@ -313,16 +324,27 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 constexpr int kFusedParts = 4;
 
 template <int N, int XS, typename ET>
-__global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS, int npw,
+__global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS_a, int npw,
                                                const int* __restrict__ plev, const int* __restrict__ pslot,
                                                const int* __restrict__ kids, const int* __restrict__ ksg,
-                                               const double* __restrict__ fvals,
-                                               ET* __restrict__ expn) {
+                                               const double* __restrict__ fvals_a,
+                                               ET* __restrict__ expn_a, Multi mu) {
     extern __shared__ double2 sm[];
     double2* sH = sm;                     // 4*2*N*N
     double2* sF = sm;                     // 2*2*N*npw (aliases sH)
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int task = lin / kFusedParts, part = lin % kFusedParts;
+    const double2* __restrict__ tabS = tabS_a;
+    const double* __restrict__ fvals = fvals_a;
+    ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[task]];
+        g = q.g;
+        npw = q.g.npw;
+        tabS = q.p2pwS;
+        fvals = q.fvals;
+        expn = static_cast<ET*>(q.expn);
+    }
     const int l = plev[task] + 1, sl = pslot[task];
     const int per = npw * N;
     const int base0 = __builtin_amdgcn_readfirstlane(3 * l * per);
@@ -574,16 +596,27 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
 // expansion; with one leaf the freed registers carry the x stage's next
 // K-step loads while the current one runs).
 template <int N, int XS, int C, typename ET>
-__global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS, int npw,
-                                                const PotTask* __restrict__ tasks, const ET* __restrict__ expn,
-                                                size_t ks, Kinds kd, OutDesc od) {
+__global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS_a, int npw,
+                                                const PotTask* __restrict__ tasks, const ET* __restrict__ expn_a,
+                                                size_t ks, Kinds kd, OutDesc od, Multi mu) {
     extern __shared__ double2 sm[];
     double2* sff = sm;
     double2* yres = sff + (size_t)npw * 64;
     double2* sUr = yres + 4 * 16 * 16;
     constexpr int NQ = N / 4;
     constexpr int PF = (N == 12) ? 6 : 8;
-    const PotTask* __restrict__ T = tasks + xcd_task(blockIdx.x, gridDim.x);
+    const int tix = xcd_task(blockIdx.x, gridDim.x);
+    const PotTask* __restrict__ T = tasks + tix;
+    const double2* __restrict__ tabS = tabS_a;
+    const ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[tix]];
+        g = q.g;
+        npw = q.g.npw;
+        tabS = q.pw2potS;
+        expn = static_cast<const ET*>(q.expn);
+        od.p = q.pot;
+    }
     const int sl = T->slot, l = T->lev, nvy = T->nvy;
     int cbox[C], ccode[C];
 #pragma unroll
@@ -792,25 +825,45 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     }
 }
 
+// STEP 1 x stage on the reduced-precision matrix cores?
+bool use_lowp1(const DevTables& t) {
+    if (t.prec < 1 || t.npw > kLowpMaxPw) return false;
+    return t.xs1 >= 0 ? t.xs1 == 2 : (lowp_stages(t.prec) & 1) != 0;
+}
+
+// STEP 5 x-stage variant (0 VALU, 1 FP64 matrix cores, 2 reduced precision)
+int x5_variant(int n, const DevTables& t) {
+    const bool lowp_ok = t.prec >= 1 && t.npw <= kLowpMaxPw;
+    const bool f64mf_ok = n >= 12 && t.npw <= 64;
+    if (t.xs5 >= 0) {
+        if (t.xs5 == 2 && lowp_ok) return 2;
+        if (t.xs5 == 1 && f64mf_ok) return 1;
+        return 0;
+    }
+    if (lowp_ok && (lowp_stages(t.prec) & 2)) return 2;
+    if (f64mf_ok && !pw2pot_valu_only()) return 1;
+    return 0;
+}
+
 template <int N>
 bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev, const int* slot,
-               const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
+               const double* fvals, double2* expn, hipStream_t s, size_t lds_limit, const Multi& mu) {
     const size_t lds = sizeof(double2) * std::max(4 * (size_t)N * N, 4 * (size_t)N * t.npw);
     if (lds > lds_limit) return false;
-    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 1)) {
+    if (use_lowp1(t)) {
         float2* e32 = reinterpret_cast<float2*>(expn);
         if (t.prec == 2)
             hipLaunchKernelGGL((k_p2pw3<N, 3, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
-                               slot, fvals, e32);
+                               slot, fvals, e32, mu);
         else
             hipLaunchKernelGGL((k_p2pw3<N, 2, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
-                               slot, fvals, e32);
+                               slot, fvals, e32, mu);
     } else if (t.expn_f32) {
         hipLaunchKernelGGL((k_p2pw3<N, 0, float2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev, slot,
-                           fvals, reinterpret_cast<float2*>(expn));
+                           fvals, reinterpret_cast<float2*>(expn), mu);
     } else {
         hipLaunchKernelGGL((k_p2pw3<N, 0, double2>), dim3(ntask), dim3(256), lds, s, g, t.p2pw, t.npw, box, lev,
-                           slot, fvals, expn);
+                           slot, fvals, expn, mu);
     }
     return true;
 }
@@ -821,69 +874,72 @@ size_t pw2pot3_lds(int n, int npw) {
 
 template <int N, int XS, int C>
 void launch_pw2pot3_c(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
-                      Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
+                      Kinds kd, OutDesc od, hipStream_t s, size_t lds, const Multi& mu) {
     if (ntask <= 0) return;
     if constexpr (XS >= 2) {
         hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
-                           tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+                           tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od, mu);
     } else {
         if (t.expn_f32)
             hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
-                               tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od);
+                               tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od, mu);
         else
             hipLaunchKernelGGL((k_pw2pot3<N, XS, C, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS,
-                               t.npw, tasks, expn, t.pw2potS_ks, kd, od);
+                               t.npw, tasks, expn, t.pw2potS_ks, kd, od, mu);
     }
 }
 
 // tasks [0, nsib) hold up to four leaves, [nsib, ntask) one leaf each
 template <int N, int XS>
 void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds) {
-    launch_pw2pot3_c<N, XS, 4>(g, t, nsib, tasks, expn, kd, od, s, lds);
-    launch_pw2pot3_c<N, XS, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds);
+                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds, const Multi& mu) {
+    launch_pw2pot3_c<N, XS, 4>(g, t, nsib, tasks, expn, kd, od, s, lds, mu);
+    Multi m1 = mu;
+    if (m1.tprob) m1.tprob += nsib;
+    launch_pw2pot3_c<N, XS, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds, m1);
 }
 
 template <int N>
 bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                 const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+                 const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit, const Multi& mu) {
     const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
-    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 2)) {
+    const int xs = x5_variant(N, t);
+    if (xs == 2) {
         // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores)
-        if (t.prec == 2) launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
-        else launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+        if (t.prec == 2) launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+        else launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
         return true;
     }
     if constexpr (N >= 12) {
-        if (t.npw <= 64 && !pw2pot_valu_only()) {
-            launch_pw2pot3_typed<N, 1>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+        if (xs == 1) {
+            launch_pw2pot3_typed<N, 1>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
             return true;
         }
     }
-    launch_pw2pot3_typed<N, 0>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds);
+    launch_pw2pot3_typed<N, 0>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
     return true;
 }
 
 template <int N>
 void launch_p2pwF3_typed(const Geom& g, const DevTables& t, unsigned nblk, size_t lds, const int* plev,
                          const int* pslot, const int* kids, const int* ksg, const double* fvals, double2* expn,
-                         hipStream_t s) {
-    if (t.prec >= 1 && t.npw <= kLowpMaxPw && (lowp_stages(t.prec) & 1)) {
+                         hipStream_t s, const Multi& mu) {
+    if (use_lowp1(t)) {
         float2* e32 = reinterpret_cast<float2*>(expn);
         if (t.prec == 2)
             hipLaunchKernelGGL((k_p2pwF3<N, 3, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
-                               pslot, kids, ksg, fvals, e32);
+                               pslot, kids, ksg, fvals, e32, mu);
         else
             hipLaunchKernelGGL((k_p2pwF3<N, 2, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
-                               pslot, kids, ksg, fvals, e32);
+                               pslot, kids, ksg, fvals, e32, mu);
     } else if (t.expn_f32) {
         hipLaunchKernelGGL((k_p2pwF3<N, 0, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev, pslot,
-                           kids, ksg, fvals, reinterpret_cast<float2*>(expn));
+                           kids, ksg, fvals, reinterpret_cast<float2*>(expn), mu);
     } else {
         hipLaunchKernelGGL((k_p2pwF3<N, 0, double2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
-                           pslot, kids, ksg, fvals, expn);
+                           pslot, kids, ksg, fvals, expn, mu);
     }
 }
 
@@ -915,18 +971,20 @@ bool pw2pot_valu_only() {
 }
 
 bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
-                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit) {
+                    const int* slot, const double* fvals, double2* expn, hipStream_t s, size_t lds_limit,
+                    const Multi& mu) {
     if (g.ndim != 3 || ntask <= 0) return false;
     switch (t.n) {
-        case 8: return try_p2pw3<8>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
-        case 12: return try_p2pw3<12>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
-        case 16: return try_p2pw3<16>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit);
+        case 8: return try_p2pw3<8>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit, mu);
+        case 12: return try_p2pw3<12>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit, mu);
+        case 16: return try_p2pw3<16>(g, t, ntask, box, lev, slot, fvals, expn, s, lds_limit, mu);
         default: return false;
     }
 }
 
 bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev, const int* pslot,
-                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s) {
+                   const int* kids, const int* ksg, const double* fvals, double2* expn, hipStream_t s,
+                   const Multi& mu) {
     if (g.ndim != 3 || ntask <= 0) return ntask <= 0;
     const int N = t.n;
     const size_t lds = sizeof(double2) * std::max(8 * (size_t)N * N, 4 * (size_t)N * t.npw);
@@ -934,9 +992,9 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
     if ((long long)ntask * kFusedParts > 0x7fffffffLL) return false;
     const unsigned nblk = (unsigned)(ntask * kFusedParts);
     switch (N) {
-        case 8: launch_p2pwF3_typed<8>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
-        case 12: launch_p2pwF3_typed<12>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
-        case 16: launch_p2pwF3_typed<16>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s); return true;
+        case 8: launch_p2pwF3_typed<8>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s, mu); return true;
+        case 12: launch_p2pwF3_typed<12>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s, mu); return true;
+        case 16: launch_p2pwF3_typed<16>(g, t, nblk, lds, plev, pslot, kids, ksg, fvals, expn, s, mu); return true;
         default: return false;
     }
 }
@@ -950,12 +1008,12 @@ bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
 }
 
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit) {
+                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit, const Multi& mu) {
     if (g.ndim != 3 || ntask <= 0) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
-        case 12: return try_pw2pot3<12>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
-        case 16: return try_pw2pot3<16>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit);
+        case 8: return try_pw2pot3<8>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
+        case 12: return try_pw2pot3<12>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
+        case 16: return try_pw2pot3<16>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
         default: return false;
     }
 }

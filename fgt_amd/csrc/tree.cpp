@@ -172,9 +172,8 @@ struct Builder {
         return s;
     }
 
-    // L2 norm of the tail coefficients of box b
-    double tail_error(int b) {
-        flush();
+    // L2 norm of the tail coefficients of box b (values already evaluated)
+    double tail_error_flushed(int b) const {
         const int n = norder;
         std::vector<double> a(fvals.begin() + static_cast<size_t>(b) * npbox,
                               fvals.begin() + static_cast<size_t>(b + 1) * npbox);
@@ -390,6 +389,11 @@ struct Builder {
         computecoll();
     }
 
+    // stop with TooLarge once a level leaves more than max_leaves leaves
+    // (0: no limit); the count before the level restriction is a lower bound
+    long long max_leaves = 0;
+    struct TooLarge {};
+
     void build() {
         mc = 1 << ndim;
         mnb = 1;
@@ -446,20 +450,19 @@ struct Builder {
             const double rsc = std::sqrt(1.0 / std::pow(boxsize[0], ndim)) * rint;
             const int f = lfirst[lev], l = llast[lev];
             std::vector<char> ref(l - f + 1, 0);
+            flush();
+            // boxes of one level are tested independently
+#pragma omp parallel for schedule(dynamic, 16)
+            for (int b = f; b <= l; ++b) ref[b - f] = tail_error_flushed(b) / rsum > eps * rsc ? 1 : 0;
             bool any = false;
-            for (int b = f; b <= l; ++b) {
-                const double err = tail_error(b) / rsum;
-                if (err > eps * rsc) {
-                    ref[b - f] = 1;
-                    any = true;
-                }
-            }
+            for (char c : ref) any = any || c;
             if (!any) break;
             boxsize.push_back(boxsize[lev] / 2);
             lfirst.push_back(nb());
             for (int b = f; b <= l; ++b)
                 if (ref[b - f]) refine(b, boxsize[lev + 1]);
             llast.push_back(nb() - 1);
+            if (max_leaves > 0 && nb() - (nb() - 1) / mc > max_leaves) throw TooLarge{};
             // running L2 norm: replace refined boxes by their children
             const double crs = std::pow(boxsize[lev + 1], ndim) / mc;
             double rsq = rint * rint;
@@ -478,6 +481,7 @@ struct Builder {
         nlevels = lev;
         computecoll();
         if (nlevels >= 2) fix_lr();
+        if (max_leaves > 0 && nb() - (nb() - 1) / mc > max_leaves) throw TooLarge{};
         flush();
     }
 };
@@ -492,12 +496,13 @@ struct fgt_tree_builder {
 
 extern "C" {
 
-int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder, double boxlen, int ng,
-                     const double* gauss, fgt_tree_builder** out, int* nboxes, int* nlevels, int* ltree) {
+int fgt_vol_tree_mem_capped(int ndim, int ipoly, int iperiod, double eps, int norder, double boxlen, int ng,
+                            const double* gauss, long long max_leaves, fgt_tree_builder** out, int* nboxes,
+                            int* nlevels, int* ltree) {
     if (!out || !nboxes || !nlevels || !ltree || !gauss) return 2;
     *out = nullptr;
     if (ndim < 1 || ndim > 3 || norder < 2 || norder > 16 || ng < 1 || (ipoly != 0 && ipoly != 1) ||
-        (iperiod != 0 && iperiod != 1) || !(eps > 0) || !(boxlen > 0))
+        (iperiod != 0 && iperiod != 1) || !(eps > 0) || !(boxlen > 0) || max_leaves < 0)
         return 2;
     try {
         auto tb = std::make_unique<fgt_tree_builder>();
@@ -509,6 +514,7 @@ int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder, d
         B.norder = norder;
         B.boxlen = boxlen;
         B.ng = ng;
+        B.max_leaves = max_leaves;
         B.gauss.assign(gauss, gauss + static_cast<size_t>(ng) * (2 * ndim + 1));
         B.build();
         *nboxes = B.nb();
@@ -516,10 +522,19 @@ int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder, d
         *ltree = (4 + B.mc + B.mnb) * B.nb() + 2 * (B.nlevels + 1);
         *out = tb.release();
         return 0;
+    } catch (const Builder::TooLarge&) {
+        g_tree_err = "tree exceeds " + std::to_string(max_leaves) + " leaves";
+        return 3;
     } catch (const std::exception& e) {
         g_tree_err = e.what();
         return 5;
     }
+}
+
+int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder, double boxlen, int ng,
+                     const double* gauss, fgt_tree_builder** out, int* nboxes, int* nlevels, int* ltree) {
+    return fgt_vol_tree_mem_capped(ndim, ipoly, iperiod, eps, norder, boxlen, ng, gauss, 0, out, nboxes, nlevels,
+                                   ltree);
 }
 
 int fgt_vol_tree_build(fgt_tree_builder* tb, int* itree, int* iptr, double* centers, double* boxsize,

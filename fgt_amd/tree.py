@@ -67,13 +67,18 @@ def gauss_source(centers, widths, amps) -> np.ndarray:
     return np.ascontiguousarray(np.hstack([centers, widths, amps]))
 
 
-def vol_tree_build(ndim, norder, ipoly, iperiod, eps, gauss, boxlen=1.0) -> Tree:
+class TreeTooLarge(RuntimeError):
+    """The tree would exceed the requested leaf limit."""
+
+
+def vol_tree_build(ndim, norder, ipoly, iperiod, eps, gauss, boxlen=1.0, max_leaves=0) -> Tree:
     """Adaptive level-restricted tree resolving ``gauss`` to ``eps``.
 
     Same refinement rule, level restriction and array layout as the
     reference's vol_tree_mem + vol_tree_build with iptype = 2 (L2), eta = 0
     (src/common/tree_vol_coeffs_nd.f:83-779,1242-1733).  With iperiod = 1 the
-    Gaussians are periodised over the 3^ndim nearest images.
+    Gaussians are periodised over the 3^ndim nearest images.  max_leaves > 0:
+    raise TreeTooLarge as soon as the tree would have more leaves.
     """
     from . import lib, _check, _p  # local import: avoids a cycle at import time
     L = lib()
@@ -81,9 +86,12 @@ def vol_tree_build(ndim, norder, ipoly, iperiod, eps, gauss, boxlen=1.0) -> Tree
     ng = g.shape[0]
     h = C.c_void_p()
     nb, nl, lt = C.c_int(), C.c_int(), C.c_int()
-    _check(L.fgt_vol_tree_mem(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod), C.c_double(eps),
-                              C.c_int(norder), C.c_double(boxlen), C.c_int(ng), _p(g),
-                              C.byref(h), C.byref(nb), C.byref(nl), C.byref(lt)))
+    rc = L.fgt_vol_tree_mem_capped(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod), C.c_double(eps),
+                                   C.c_int(norder), C.c_double(boxlen), C.c_int(ng), _p(g),
+                                   C.c_longlong(max_leaves), C.byref(h), C.byref(nb), C.byref(nl), C.byref(lt))
+    if rc == 3:
+        raise TreeTooLarge(f"tree exceeds {max_leaves} leaves")
+    _check(rc)
     nboxes, nlevels, ltree = nb.value, nl.value, lt.value
     itree = np.zeros(ltree, np.int32)
     iptr = np.zeros(8, np.int32)

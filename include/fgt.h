@@ -163,8 +163,13 @@ typedef struct {
     const double *centers, *boxsize;
 } fgt_problem;
 typedef struct fgt_batch fgt_batch;
-/* Plans every problem (arguments as fgt_solver_create) and creates nstreams
- * (1..16) HIP streams; executes are spread over them, largest first. */
+/* Plans every problem (arguments as fgt_solver_create).  3-D problems of
+ * norder 8, 12 or 16 are grouped into shape classes (norder, nd, precision,
+ * plane-wave count rounded up to 16) and every stage of a class runs as one
+ * launch over the tasks of all its problems (STEPs 2 and 4 in level phases
+ * aligned per problem); the other problems run their own executes on
+ * nstreams (1..16) side streams.  Replaces the reference's one problem per
+ * call with its level-synchronous box loops (src/bfgt/boxfgt.f:711-928). */
 int fgt_batch_create(int nprob, const fgt_problem *probs, int nstreams,
                      fgt_batch **out);
 /* Per-problem precision (fgt_solver_set_precision modes): mode 0 FP64 for
@@ -176,8 +181,23 @@ int fgt_batch_create(int nprob, const fgt_problem *probs, int nstreams,
 int fgt_batch_set_precision(fgt_batch *b, int mode, double eps_min,
                             int *nmixed);
 int fgt_batch_set_fvals(fgt_batch *b, int i, const double *fvals);
-/* Runs STEPs 1-6 of every problem, overlapped, and waits for all. */
+/* Runs STEPs 1-6 of every problem and waits for all.  The first execute
+ * (after create, a precision change, set_graph or autotune) captures the
+ * whole launch sequence into a HIP graph; later executes replay it. */
 int fgt_batch_execute(fgt_batch *b);
+/* on != 0: replay from a captured graph (default); 0: enqueue eagerly. */
+int fgt_batch_set_graph(fgt_batch *b, int on);
+/* Autotuner: per shape class, times each candidate STEP 3 targets-per-block
+ * (2, 3, 4, 6, 8) and STEP 1 / STEP 5 x-stage variant (VALU, FP64 or
+ * reduced-precision matrix cores) reps times on the batch's own data and
+ * keeps the fastest; the choices are remembered for the process per class
+ * key and applied to later batches. */
+int fgt_batch_autotune(fgt_batch *b, int reps);
+/* number of shape classes; info[10] of class k: norder, nd, precision,
+ * plane-wave cap, members, STEP 3 group, STEP 1 / STEP 5 variant (-1 =
+ * default policy), STEP 5 tasks, near-field targets */
+int fgt_batch_nclasses(const fgt_batch *b);
+int fgt_batch_class_info(const fgt_batch *b, int k, int *info);
 int fgt_batch_get_pot(fgt_batch *b, int i, double *pot, int accumulate);
 /* total output nodes of the batch; device time of the last execute (ms) */
 int fgt_batch_stats(const fgt_batch *b, long long *nout, double *last_ms);

@@ -24,6 +24,13 @@ int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder,
                      double boxlen, int ng, const double *gauss,
                      fgt_tree_builder **out, int *nboxes, int *nlevels,
                      int *ltree);
+/* fgt_vol_tree_mem with a size limit (no counterpart in the reference):
+ * returns status 3 without a builder as soon as the tree would have more
+ * than max_leaves leaves (0 = no limit). */
+int fgt_vol_tree_mem_capped(int ndim, int ipoly, int iperiod, double eps, int norder,
+                            double boxlen, int ng, const double *gauss,
+                            long long max_leaves, fgt_tree_builder **out,
+                            int *nboxes, int *nlevels, int *ltree);
 /* Second pass: copies the arrays out (vol_tree_build) and frees the builder.
  * itree(ltree), iptr(8), centers(ndim,nboxes), boxsize(0:nlevels),
  * fvals(npbox,nboxes). */
