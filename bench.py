@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), dense
 PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+# dense matrix peak of the contraction precision of each mode (TFLOP/s)
+PEAK_MODE_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 
 
 def log(*a):
@@ -167,6 +169,49 @@ def pot_error(pot, ref):
     return l2, mx
 
 
+def batch_roofline(probs, prof):
+    """Per class and stage: device ms of the profiled execute and the
+    algorithmic work of that launch group; roofline of the longest one among
+    STEP 1 (k_p2pwF3 / k_p2pw3), STEP 3 (k_m2l27) and STEP 5 (k_pw2pot3).
+    STEP 1 and 5 flops: the reference's per-leaf count 4 n^3 nh + 8 n^2 npw nh
+    + 8 n npw^2 nh (nh = (npw+1)/2) per plane-wave leaf of every member;
+    STEP 3 bytes: every cutoff-level expansion read once and every local
+    expansion written once (2 x nexp complex of the storage precision per
+    target)."""
+    import fgt_amd
+    rows, best = [], None
+    for c in prof:
+        n, prec = c["norder"], c["prec"]
+        elem = 16 if prec == 0 else 8
+        flops = byts = 0
+        for i in c["member_ids"]:
+            tr, delta, eps = probs[i]
+            ps = fgt_amd.plan_summary(tr, delta, eps)
+            npw = ps["npw"]
+            nh = (npw + 1) // 2
+            flops += ps["npw_leaves"] * (4 * n ** 3 * nh + 8 * n * n * npw * nh + 8 * n * npw * npw * nh)
+            byts += 2 * elem * ps["nexp"] * ps["n_m2l_targets"]
+        ms = c["stage_ms"]
+        rows.append({"norder": n, "prec": prec, "npw_cap": c["npw_cap"], "members": c["members"],
+                     "group": c["group"], "xs1": c["xs1"], "xs5": c["xs5"],
+                     "stage_ms": [round(v, 4) for v in ms], "step15_gflop": round(flops / 1e9, 3),
+                     "step3_gb": round(byts / 1e9, 3)})
+        for stg, kern, kind, work in ((4, "k_pw2pot3", "flops", flops), (0, "k_p2pwF3+k_p2pw3", "flops", flops),
+                                      (2, "k_m2l27", "bytes", byts)):
+            if ms[stg] > 0 and work > 0 and (best is None or ms[stg] > best[0]):
+                best = (ms[stg], kern, kind, work, n, prec, c["npw_cap"])
+    if best is None:
+        return None, rows
+    ms, kern, kind, work, n, prec, cap = best
+    if kind == "flops":
+        ach, peak, unit, bound = work / (ms * 1e-3) / 1e12, PEAK_MODE_TFLOPS[prec], "TFLOP/s", "mfma"
+    else:
+        ach, peak, unit, bound = work / (ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
+    return {"kernel": kern, "launch_group": f"norder {n}, precision mode {prec}, npw <= {cap}",
+            "bound": bound, "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 4),
+            "ms": round(ms, 4), "algorithmic": work, "traffic": None}, rows
+
+
 def headline(probs, steps, warmup, mode, drop_in=True):
     """North-star comparison on one problem set.
 
@@ -193,6 +238,9 @@ def headline(probs, steps, warmup, mode, drop_in=True):
     t_b64 = (time.perf_counter() - t0) / max(1, min(steps, 5))
     ref = [b.get_pot(k)[tr.leaves] for k, (tr, _, _) in enumerate(probs)]
     nmixed = b.set_precision(mode)
+    t0 = time.perf_counter()
+    b.autotune(3)
+    t_tune = time.perf_counter() - t0
     for _ in range(warmup):
         b.execute()
     torch.cuda.synchronize()
@@ -206,6 +254,8 @@ def headline(probs, steps, warmup, mode, drop_in=True):
         l2, mx = pot_error(b.get_pot(k)[tr.leaves], ref[k])
         worst_l2 = max(worst_l2, l2 / eps)
         worst_max = max(worst_max, mx / eps)
+    prof = b.profile()
+    roof, stage_tab = batch_roofline(probs, prof)
     b.close()
     # device-resident FP64 solver, one problem at a time
     solvers = []
@@ -246,7 +296,8 @@ def headline(probs, steps, warmup, mode, drop_in=True):
         "problems": nprob, "output_nodes": int(nout), "leaves_min": int(min(lv)), "leaves_max": int(max(lv)),
         "norders": sorted({int(t.norder) for t, _, _ in probs}),
         "ms_batched_mixed": round(t_mix * 1e3, 3), "nodes_per_s_batched_mixed": round(nout / t_mix, 1),
-        "problems_mixed": int(nmixed), "precision_mode": mode,
+        "problems_mixed": int(nmixed), "precision_mode": mode, "autotune_s": round(t_tune, 2),
+        "classes": stage_tab, "roofline": roof,
         "max_rel_l2_err_over_eps": float(f"{worst_l2:.3g}"), "max_err_over_eps": float(f"{worst_max:.3g}"),
         "error_reference": "FP64 batch on the same trees (equal to the reference boxfgt to ~1e-15)",
         "ms_batched_fp64": round(t_b64 * 1e3, 3), "nodes_per_s_batched_fp64": round(nout / t_b64, 1),
@@ -571,6 +622,7 @@ def main():
     c2 = roofline = None
     if not args.no_c2:
         c2, roofline = c2_leg(args)
+        c2["roofline"] = roofline
         log(f"c2: {c2}")
     sweep = None
     if not args.no_mixed_sweep:
@@ -592,6 +644,8 @@ def main():
 
     if head is not None:
         value, ms, dtype = head["nodes_per_s_batched_mixed"], head["ms_batched_mixed"], "f32"
+        if head.get("roofline"):
+            roofline = dict(head["roofline"], workload="headline batch")
         workload = (f"{head['problems']} problems: the eps=1e-6, 3-D slice of the first {args.draws} draws of the "
                     f"uneven generator (norder 8/12/16, delta log-uniform in [1e-5,1e-1], 1-10 Gaussians, free "
                     f"space), batched, mixed precision (FP32 contractions and expansions, FP64 phases, weights "

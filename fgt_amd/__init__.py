@@ -47,7 +47,7 @@ class _PlanInfo(C.Structure):
                 ("periodic_small", C.c_int), ("all_pw", C.c_int),
                 ("nleaves", C.c_longlong), ("npw_leaves", C.c_longlong),
                 ("ndirect_pairs", C.c_longlong), ("n_m2l_pairs", C.c_longlong),
-                ("n_parent_child", C.c_longlong)]
+                ("n_parent_child", C.c_longlong), ("n_m2l_targets", C.c_longlong)]
 
 
 def lib():
@@ -329,6 +329,19 @@ class Batch:
         """Pick, per shape class, the STEP 3 targets per block and the STEP 1 /
         STEP 5 x-stage variants by timing them on this batch's data."""
         _check(lib().fgt_batch_autotune(self._h, C.c_int(reps)))
+
+    def profile(self):
+        """One eager execute with per-stage events; per class: STEP 1..6 ms
+        and the member problem indices."""
+        _check(lib().fgt_batch_profile(self._h))
+        out = []
+        for k, c in enumerate(self.classes()):
+            ms = (C.c_double * 6)()
+            _check(lib().fgt_batch_class_stage_ms(self._h, C.c_int(k), ms))
+            mem = (C.c_int * max(1, c["members"]))()
+            _check(lib().fgt_batch_class_members(self._h, C.c_int(k), mem))
+            out.append(dict(c, stage_ms=list(ms), member_ids=list(mem)[:c["members"]]))
+        return out
 
     def classes(self):
         """Shape classes: dicts of norder, nd, precision, plane-wave cap,

@@ -953,6 +953,7 @@ struct BClass {
     int n_s5 = 0, n_s5_sib = 0;
     DBuf dir_tgt, dir_lev, dir_off, dir_src, dir_code, dir_flags, dir_prob;
     int n_dir = 0;
+    double stage_ms[6] = {0, 0, 0, 0, 0, 0};   // STEPs 1-6 of the last profiled execute
 };
 
 // Tuned per-class choices, kept for the process: key (norder, densities,
@@ -1282,15 +1283,36 @@ void class_step6(const BClass& C, hipStream_t st) {
                            fgt::Kinds{0, 0, 0}, fgt::OutDesc{nullptr, C.nd, 0}, st, multi(C, C.dir_prob));
 }
 
-// Enqueue one whole batch transform (graph capture records this sequence).
-void batch_enqueue(fgt_batch& B) {
+// Stage timing of a profiled execute: events around each class stage.
+struct StageEvents {
+    std::vector<hipEvent_t> ev;   // 2 per (class, stage)
+    ~StageEvents() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// Enqueue one whole batch transform (graph capture records this sequence);
+// prof != nullptr: record events around every stage of every class.
+void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     hipStream_t st = B.streams[0], sn = B.streams[1];
+    const size_t nc = B.classes.size();
+    if (prof) {
+        prof->ev.assign(12 * nc, nullptr);
+        for (auto& e : prof->ev) hcheck(hipEventCreate(&e), "hipEventCreate");
+    }
+    auto mark = [&](size_t c, int step, int end, hipStream_t s) {
+        if (prof) hcheck(hipEventRecord(prof->ev[12 * c + 2 * step + end], s), "hipEventRecord");
+    };
     for (auto& S : B.solvers) fgt::launch_zero(S->d_pot.p, S->d_pot.bytes, st);
     hcheck(hipEventRecord(B.ev_fork, st), "hipEventRecord");
     for (size_t k = 1; k < B.streams.size(); ++k)
         hcheck(hipStreamWaitEvent(B.streams[k], B.ev_fork, 0), "hipStreamWaitEvent");
     // near field of every class beside STEPs 1-4
-    for (auto& C : B.classes) class_step6(*C, sn);
+    for (size_t c = 0; c < nc; ++c) {
+        mark(c, 5, 0, sn);
+        class_step6(*B.classes[c], sn);
+        mark(c, 5, 1, sn);
+    }
     hcheck(hipEventRecord(B.ev_near, sn), "hipEventRecord");
     // problems off the class path, spread over the side streams (largest first)
     const size_t nside = B.streams.size() - 2;
@@ -1298,15 +1320,27 @@ void batch_enqueue(fgt_batch& B) {
         fgt_solver& S = *B.solvers[B.loose[i]];
         execute(S, B.streams[2 + i % nside], false, false, nullptr, true);
     }
-    for (auto& C : B.classes) {
-        if (C->n_fz + C->n_s1 + C->n_m2l + C->n_s5 == 0) continue;
-        class_step1(*C, st);
-        class_step2(*C, st);
-        class_step3(*C, st);
-        class_step4(*C, st);
+    for (size_t c = 0; c < nc; ++c) {
+        const BClass& C = *B.classes[c];
+        mark(c, 0, 0, st);
+        class_step1(C, st);
+        mark(c, 0, 1, st);
+        mark(c, 1, 0, st);
+        class_step2(C, st);
+        mark(c, 1, 1, st);
+        mark(c, 2, 0, st);
+        class_step3(C, st);
+        mark(c, 2, 1, st);
+        mark(c, 3, 0, st);
+        class_step4(C, st);
+        mark(c, 3, 1, st);
     }
     hcheck(hipStreamWaitEvent(st, B.ev_near, 0), "hipStreamWaitEvent");
-    for (auto& C : B.classes) class_step5(*C, st);
+    for (size_t c = 0; c < nc; ++c) {
+        mark(c, 4, 0, st);
+        class_step5(*B.classes[c], st);
+        mark(c, 4, 1, st);
+    }
     for (size_t k = 2; k < B.streams.size(); ++k) {
         hcheck(hipEventRecord(B.ev_loose[k - 2], B.streams[k]), "hipEventRecord");
         hcheck(hipStreamWaitEvent(st, B.ev_loose[k - 2], 0), "hipStreamWaitEvent");
@@ -1724,6 +1758,40 @@ int fgt_batch_class_info(const fgt_batch* b, int k, int* info) {
         const int v[10] = {C.n, C.nd, C.prec, C.npw_cap, (int)C.members.size(), C.group, C.t.xs1, C.t.xs5,
                            C.n_s5, C.n_dir};
         std::memcpy(info, v, sizeof v);
+    });
+}
+
+int fgt_batch_profile(fgt_batch* b) {
+    return guarded([&] {
+        if (!b) throw std::invalid_argument("null batch");
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        StageEvents prof;
+        batch_enqueue(*b, &prof);
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        for (size_t c = 0; c < b->classes.size(); ++c)
+            for (int k = 0; k < 6; ++k) {
+                float ms = 0;
+                hcheck(hipEventElapsedTime(&ms, prof.ev[12 * c + 2 * k], prof.ev[12 * c + 2 * k + 1]),
+                       "hipEventElapsedTime");
+                b->classes[c]->stage_ms[k] = ms;
+            }
+    });
+}
+
+int fgt_batch_class_stage_ms(const fgt_batch* b, int k, double* ms) {
+    return guarded([&] {
+        if (!b || !ms) throw std::invalid_argument("null argument");
+        if (k < 0 || k >= (int)b->classes.size()) throw std::invalid_argument("no such class");
+        for (int i = 0; i < 6; ++i) ms[i] = b->classes[k]->stage_ms[i];
+    });
+}
+
+int fgt_batch_class_members(const fgt_batch* b, int k, int* members) {
+    return guarded([&] {
+        if (!b || !members) throw std::invalid_argument("null argument");
+        if (k < 0 || k >= (int)b->classes.size()) throw std::invalid_argument("no such class");
+        const BClass& C = *b->classes[k];
+        std::memcpy(members, C.members.data(), sizeof(int) * C.members.size());
     });
 }
 

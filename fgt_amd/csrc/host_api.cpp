@@ -111,6 +111,7 @@ int fgt_plan_summary(int nd, int ndim, double delta, double eps, int ipoly, int 
     out->ndirect_pairs = P.ndirect_pairs;
     out->n_m2l_pairs = static_cast<long long>(P.m2l_src.size());
     out->n_parent_child = static_cast<long long>(P.pc_child.size());
+    out->n_m2l_targets = static_cast<long long>(P.m2l_tgt.size());
     return 0;
 }
 

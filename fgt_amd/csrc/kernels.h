@@ -32,6 +32,31 @@ __device__ __forceinline__ double2 ldE(const float2& v) { return make_double2(v.
 __device__ __forceinline__ void stE(double2* p, double2 v) { *p = v; }
 __device__ __forceinline__ void stE(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
+// Read-only 1-D tables viewed through the constant address space: loads
+// there are invariant for the kernel, so wave-uniform reads become scalar
+// loads into SGPRs whether the table pointer is a kernel argument or comes
+// from a per-problem descriptor in memory (where the compiler could not
+// otherwise rule out a clobbering store).  CTab2: complex (double2) entries,
+// CTab1: real.
+typedef const double __attribute__((address_space(4))) cdouble_t;
+typedef double dvec2_t __attribute__((ext_vector_type(2)));
+typedef const dvec2_t __attribute__((address_space(4))) cdvec2_t;
+struct CTab2 {
+    cdvec2_t* p;
+    __device__ __forceinline__ CTab2 operator+(long long o) const { return CTab2{p + o}; }
+    __device__ __forceinline__ double2 operator[](long long i) const {
+        const dvec2_t v = p[i];   // one 16-byte load
+        return make_double2(v.x, v.y);
+    }
+};
+struct CTab1 {
+    cdouble_t* p;
+    __device__ __forceinline__ CTab1 operator+(long long o) const { return CTab1{p + o}; }
+    __device__ __forceinline__ double operator[](long long i) const { return p[i]; }
+};
+__device__ __forceinline__ CTab2 ctab(const double2* q) { return CTab2{(cdvec2_t*)q}; }
+__device__ __forceinline__ CTab1 ctab(const double* q) { return CTab1{(cdouble_t*)q}; }
+
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
 // range of tasks so that tasks sharing data (children of one parent,
 // neighbouring targets, one element range of many targets) meet in the same L2.

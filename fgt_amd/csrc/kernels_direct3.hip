@@ -40,9 +40,9 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
     }
     const int tb = tgt[task], l = lev[task];
     // table kind per axis (value / first / second derivative): stride ks
-    const double* __restrict__ tabx = loctab + kd.x * ks + (size_t)l * 13 * NN;
-    const double* __restrict__ taby = loctab + kd.y * ks + (size_t)l * 13 * NN;
-    const double* __restrict__ tabz = loctab + kd.z * ks + (size_t)l * 13 * NN;
+    const CTab1 tabx = ctab(loctab) + (long long)(kd.x * ks + (size_t)l * 13 * NN);
+    const CTab1 taby = ctab(loctab) + (long long)(kd.y * ks + (size_t)l * 13 * NN);
+    const CTab1 tabz = ctab(loctab) + (long long)(kd.z * ks + (size_t)l * 13 * NN);
     const int tid = threadIdx.x;
     const bool act = tid < NN;
     // x stage: thread = (j2, j3); y stage: thread = (k1, j3); z stage: thread = (k1, k2)
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
         for (int i = 0; i < N; ++i) X[i] = Y[i] = acc[i] = 0.0;
         for (int j = j0; j < j1e; ++j) {
             const int sb = src[j], c = code[j], fl = flags[j];
-            const double* __restrict__ Ax = tabx + (size_t)(c & 15) * NN;
+            const CTab1 Ax = tabx + (long long)(c & 15) * NN;
             // ---- x: X[k1] += sum_j1 Ax[k1][j1] f[j1, j2=a, j3=bq]
             if (act) {
                 double f[N];
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
             }
             if (fl & 1) {
                 // ---- y: Y[k2] += sum_j2 Ay[k2][j2] X[k1, j2, j3]
-                const double* __restrict__ Ay = taby + (size_t)((c >> 4) & 15) * NN;
+                const CTab1 Ay = taby + (long long)((c >> 4) & 15) * NN;
                 __syncthreads();
                 if (act) {
 #pragma unroll
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
                 for (int i = 0; i < N; ++i) X[i] = 0.0;
                 if (fl & 2) {
                     // ---- z: acc[k3] += sum_j3 Az[k3][j3] Y[k1, k2, j3]
-                    const double* __restrict__ Az = tabz + (size_t)((c >> 8) & 15) * NN;
+                    const CTab1 Az = tabz + (long long)((c >> 8) & 15) * NN;
                     __syncthreads();
                     if (act) {
 #pragma unroll

@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
         expn = static_cast<ET*>(q.expn);
     }
     const int b = box[task], l = lev[task], sl = slot[task];
-    const double2* __restrict__ T = tab + (size_t)l * npw * N;   // T[k][m]
+    const CTab2 T = ctab(tab) + (long long)l * npw * N;   // T[k][m]
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     const int l = plev[task] + 1, sl = pslot[task];
     const int per = npw * N;
     const int base0 = __builtin_amdgcn_readfirstlane(3 * l * per);
-    const double2* __restrict__ T = tabS;
+    const CTab2 T = ctab(tabS);
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 for (int c = 0; c < nk; ++c) {
                     const int b = kids[8 * task + c], sg = ksg[8 * task + c];
                     const int vz = 1 + ((sg >> 2) & 1);
-                    const double2* __restrict__ Tz = T + base0 + vz * per;
+                    const CTab2 Tz = T + base0 + vz * per;
                     double fr[N];
 #pragma unroll
                     for (int m3 = 0; m3 < N; ++m3)
@@ -407,8 +407,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                     hp[m2] = act ? sH[(((sx + 0) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
                     hm[m2] = act ? sH[(((sx + 2) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
                 }
-                const double2* __restrict__ Typ = T + base0 + 1 * per;
-                const double2* __restrict__ Tym = T + base0 + 2 * per;
+                const CTab2 Typ = T + base0 + 1 * per;
+                const CTab2 Tym = T + base0 + 2 * per;
                 __syncthreads();   // sH consumed; sF (same memory) may be written
                 for (int k2 = wave; k2 < npw; k2 += 4) {
                     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
@@ -428,8 +428,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 const int k3 = r + s;
                 if (k3 < nh) {
                     ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
-                    const double2* __restrict__ Txp = T + base0 + 1 * per;
-                    const double2* __restrict__ Txm = T + base0 + 2 * per;
+                    const CTab2 Txp = T + base0 + 1 * per;
+                    const CTab2 Txm = T + base0 + 2 * per;
                     x1_lowp<2 * N, XS>(
                         npw, lane, hlf, 2,
                         [&](int k2, int kk) {
@@ -445,8 +445,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 if (k3 < nh) {
                     ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     const int k1a = hlf ? (npw + 1) / 2 : 0, k1b = hlf ? npw : (npw + 1) / 2;
-                    const double2* __restrict__ Txp = T + base0 + 1 * per;
-                    const double2* __restrict__ Txm = T + base0 + 2 * per;
+                    const CTab2 Txp = T + base0 + 1 * per;
+                    const CTab2 Txm = T + base0 + 2 * per;
                     for (int c = 0; c < npw; c += 64) {
                         const int k2 = c + lane;
                         const bool act = k2 < npw;
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // FP64 x stages), where the y and z stages read it.  Lanes past npw read a
 // valid duplicate row (discarded); K entries past npw get zero table values.
 template <int N, int XS>
-__device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const double2* __restrict__ UX,
+__device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const CTab2 UX,
                                               int npw, int nh, int r, int wave, int lane, double2* sff) {
     const int k3 = r + wave;
     if (k3 >= nh) return;
@@ -629,8 +629,8 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const int per = npw * N;
     const int ox = __builtin_amdgcn_readfirstlane((int)(kd.x * ks) + (3 * l + T->vx) * per);
     const int oz = __builtin_amdgcn_readfirstlane((int)(kd.z * ks) + 3 * l * per);
-    const double2* __restrict__ UX = tabS + ox;
-    const double2* __restrict__ UZ = tabS + oz;
+    const CTab2 UX = ctab(tabS) + ox;
+    const CTab2 UZ = ctab(tabS) + oz;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
@@ -769,7 +769,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                 if (j) __syncthreads();   // the z stage of the previous y variant is done with yres
                 // y: lane = (kk1, s), wave owns kk2 in [wave*NQ, wave*NQ+NQ)
                 const int oy = __builtin_amdgcn_readfirstlane((int)(kd.y * ks) + (3 * l + T->vy[j]) * per);
-                const double2* __restrict__ UY = tabS + oy;
+                const CTab2 UY = ctab(tabS) + oy;
                 const int k3 = r + s;
                 if (kk1 < N && k3 < nh) {
                     double2 y[NQ];

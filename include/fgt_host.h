@@ -56,6 +56,7 @@ typedef struct {
     int periodic_small, all_pw;
     long long nleaves, npw_leaves, ndirect_pairs;
     long long n_m2l_pairs, n_parent_child;
+    long long n_m2l_targets;   /* cutoff-level boxes receiving a local expansion */
 } fgt_plan_info;
 
 int fgt_plan_summary(int nd, int ndim, double delta, double eps, int ipoly,
