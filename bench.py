@@ -359,15 +359,17 @@ def mixed_sweep(steps):
 
 
 def batched_comparison(probs, nstreams, steps):
-    """All problems executed overlapped on nstreams streams (fgt_batch) vs
-    one at a time (device-resident solver per problem, synchronised after
-    each)."""
+    """The uneven generator's problems as one batch (shape classes, one launch
+    per stage and class, graph replay, autotuned) in FP64 and with the
+    cheapest precision per eps, vs the device-resident FP64 solver one
+    problem at a time (synchronised after each)."""
     import torch
     import fgt_amd
     nprob = len(probs)
     b = fgt_amd.Batch(probs, nstreams=nstreams)
     for k, (tr, _, _) in enumerate(probs):
         b.set_fvals(k, tr.fvals)
+    b.autotune(3)
     b.execute()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -376,10 +378,9 @@ def batched_comparison(probs, nstreams, steps):
     t_b = (time.perf_counter() - t0) / steps
     nout = b.stats()["nout"]
     ref = [b.get_pot(k) for k in range(nprob)]
-    # mixed precision (FP32 expansions) where eps >= 1e-6 and the 3-D path applies
-    nmixed = b.set_precision(1, 1e-6)
+    nmixed = b.set_precision(3)
+    b.autotune(3)
     b.execute()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         b.execute()
@@ -389,55 +390,60 @@ def batched_comparison(probs, nstreams, steps):
         den = np.linalg.norm(ref[k])
         err = np.linalg.norm(b.get_pot(k) - ref[k]) / (den if den > 0 else 1.0)
         worst = max(worst, err / eps)
+    ncls = len(b.classes())
     b.close()
-    solvers = []
+    t_p = 0.0
     for tr, delta, eps in probs:
         sv = fgt_amd.Solver.from_tree(tr, delta, eps)
         sv.set_fvals(tr.fvals)
-        solvers.append(sv)
-    for sv in solvers:
         sv.execute()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for sv in solvers:
-            sv.execute()
-            torch.cuda.synchronize()
-    t_p = (time.perf_counter() - t0) / steps
-    for sv in solvers:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sv.execute()
+        torch.cuda.synchronize()
+        t_p += time.perf_counter() - t0
         sv.close()
     dims = sum(1 for tr, _, _ in probs if tr.ndim == 2)
-    return {"problems": nprob, "two_d": dims, "nstreams": nstreams, "output_nodes": int(nout),
+    return {"problems": nprob, "two_d": dims, "classes": ncls, "output_nodes": int(nout),
             "leaves_min": int(min(len(t.leaves) for t, _, _ in probs)),
             "leaves_max": int(max(len(t.leaves) for t, _, _ in probs)),
             "ms_batched": round(t_b * 1e3, 3), "ms_one_at_a_time": round(t_p * 1e3, 3),
             "nodes_per_s_batched": round(nout / t_b, 1), "nodes_per_s_one_at_a_time": round(nout / t_p, 1),
             "speedup": round(t_p / t_b, 3), "dtype": "f64",
-            "mixed": {"problems_fp32_expansions": nmixed, "ms": round(t_m * 1e3, 3),
+            "mixed": {"problems_reduced_precision": nmixed, "ms": round(t_m * 1e3, 3),
                       "nodes_per_s": round(nout / t_m, 1), "speedup_vs_one_at_a_time_fp64": round(t_p / t_m, 3),
                       "max_rel_l2_err_over_eps": float(f"{worst:.3g}")}}
 
 
-def serving(probs, rate, seconds, nstreams):
-    """Poisson arrivals (seeded) at `rate` per second for `seconds`, each
-    naming one of the problems; served with continuous admission on
-    nstreams streams and, for comparison, one at a time (one stream).
-    Problems are made resident before the clock starts."""
-    from fgt_amd.serve import Server, poisson_arrivals
-    nout = [len(t.leaves) * t.norder ** t.ndim for t, _, _ in probs]
-    arr = poisson_arrivals(rate, seconds, len(probs), seed=7)
-    out = {"rate_per_s": rate, "seconds": seconds, "requests": len(arr), "problems": len(probs)}
-    for label, ns in (("served", nstreams), ("one_at_a_time", 1)):
-        srv = Server(nstreams=ns)
-        for k, (tr, delta, eps) in enumerate(probs):
-            srv.register(k, tr, delta, eps)
-            srv._solver(k)
-        lat, wall = srv.run(arr, seconds)
+def serving(pool, rate, seconds, nstreams, mode, budget=200 << 30):
+    """Seeded Poisson arrivals at `rate` per second for `seconds`, each
+    naming a problem of `pool` (the uneven generator, free space and
+    periodic), through the library's server: `nstreams` requests in flight
+    with the cheapest precision per eps (mode) vs the plain FP64 path one
+    request at a time (one stream, mode 0).  Each server first sees one
+    request per problem (planning, building, first upload) outside the
+    clock, so the measured stream is served from device-resident problems;
+    throughput = output nodes of the stream / (last finish - first arrival),
+    latency = finish - scheduled arrival."""
+    from fgt_amd.serve import Server, poisson_arrivals, run_stream
+    nodes = [len(t.leaves) * t.npbox for t, _, _ in pool]
+    arr = poisson_arrivals(rate, seconds, len(pool), seed=7)
+    out = {"rate_per_s": rate, "seconds": seconds, "requests": len(arr), "problems": len(pool)}
+    for label, ns, m in (("served", nstreams, mode), ("one_at_a_time_fp64", 1, 0)):
+        srv = Server(budget=budget, nstreams=ns, precision=m)
+        t0 = time.perf_counter()
+        run_stream(srv, pool, [(0.0, k, k) for k in range(len(pool))])
+        warm = time.perf_counter() - t0
+        lat, wall, _ = run_stream(srv, pool, arr)
+        st = srv.stats()
         srv.close()
-        v = np.array([lat[r] for _, r, _ in arr]) * 1e3
-        done = sum(nout[k] for _, _, k in arr)
-        out[label] = {"nstreams": ns, "nodes_per_s": round(done / wall, 1), "p50_ms": round(float(np.percentile(v, 50)), 3),
-                      "p99_ms": round(float(np.percentile(v, 99)), 3), "wall_s": round(wall, 3)}
+        v = np.array([lat[n] for _, n, _ in arr]) * 1e3
+        done = sum(nodes[k] for _, _, k in arr)
+        out[label] = {"nstreams": ns, "precision_mode": m, "nodes_per_s": round(done / wall, 1),
+                      "p50_ms": round(float(np.percentile(v, 50)), 3), "p99_ms": round(float(np.percentile(v, 99)), 3),
+                      "wall_s": round(wall, 3), "warm_up_s": round(warm, 2),
+                      "resident_gb": round(st["used_bytes"] / 1e9, 2), "plans": st["plans"],
+                      "evictions": st["evictions"]}
     return out
 
 
@@ -597,8 +603,9 @@ def main():
     ap.add_argument("--draws", type=int, default=512, help="generator draws the headline slice is taken from")
     ap.add_argument("--batch", type=int, default=0,
                     help="problems of the whole uneven generator for a batched-vs-one-at-a-time leg (0 = skip)")
-    ap.add_argument("--serve-rates", default="", help="request arrivals per second (comma list)")
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of each serving run (0 = skip)")
+    ap.add_argument("--serve-rates", default="100,1000,3000,5000", help="request arrivals per second (comma list)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of each serving run (0 = skip)")
+    ap.add_argument("--serve-pool", type=int, default=48, help="problems the served requests are drawn from")
     ap.add_argument("--no-mixed-sweep", action="store_true",
                     help="skip the norder-12 eps x delta mixed-precision sweep")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -633,10 +640,15 @@ def main():
         allp, _, _ = c4_problems(args.batch)
         batched = batched_comparison(allp, 4, max(1, min(args.steps, 5)))
         log(f"batched: {batched}")
-        rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
-        if args.serve_seconds > 0 and rates:
-            served = [serving(allp, r, args.serve_seconds, 4) for r in rates]
-            log(f"served: {served}")
+    rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+    if args.serve_seconds > 0 and rates and args.serve_pool > 0:
+        pool, draws, replaced = c4_problems(args.serve_pool, periodic=True)
+        served = {"pool": {"problems": len(pool), "periodic": int(sum(t.iperiod for t, _, _ in pool)),
+                           "output_nodes": int(sum(len(t.leaves) * t.npbox for t, _, _ in pool)),
+                           "draws": draws, "replaced_over_8000_leaves": replaced},
+                  "budget_gb": round((200 << 30) / 1e9, 1),
+                  "rates": [serving(pool, r, args.serve_seconds, 4, 3) for r in rates]}
+        log(f"served: {served}")
     cpu = None
     if not args.no_cpu_baseline and probs is not None:
         cpu = cpu_baseline(probs)

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -55,6 +56,9 @@ struct Arena {
     }
 };
 thread_local Arena* g_arena = nullptr;
+// device bytes allocated on this thread by DBuf::alloc (the server's
+// admission accounting reads it around a solver build)
+thread_local size_t g_tally = 0;
 
 // A device buffer owned by the solver (or carved from the installed arena).
 struct DBuf {
@@ -79,6 +83,7 @@ struct DBuf {
             from_arena = true;
         } else {
             hcheck(hipMalloc(&p, b), "hipMalloc");
+            g_tally += b;
         }
     }
     template <class T>
@@ -260,10 +265,26 @@ void compact_slots(fgt::Plan& P) {
 
 // own_io = false: the caller brings device fvals / pot (no-allocation
 // entry point), so the solver keeps none of its own.
-void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
-    std::string err;
-    const int rc = fgt::make_plan(in, S.plan, &err);
-    if (rc != 0) throw std::invalid_argument(err);
+// Precision a mode asks for on one problem (fgt_batch_set_precision modes):
+// 0 FP64; 1 FP32 where eps >= 1e-6; 2 BF16 hi/lo where eps >= 1e-3; 3 the
+// cheapest one eps allows.  ok: the problem is on the 3-D fast path and
+// forms plane-wave expansions.
+int mode_precision(int mode, double eps, bool ok) {
+    if (!ok || mode == 0) return 0;
+    if (mode == 1) return eps >= kEpsFp32 ? 1 : 0;
+    if (mode == 2) return eps >= kEpsBf16 ? 2 : 0;
+    return eps >= kEpsBf16 ? 2 : (eps >= kEpsFp32 ? 1 : 0);
+}
+
+void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true, fgt::Plan* planned = nullptr,
+                  int prec_mode = 0) {
+    if (planned) {
+        S.plan = std::move(*planned);
+    } else {
+        std::string err;
+        const int rc = fgt::make_plan(in, S.plan, &err);
+        if (rc != 0) throw std::invalid_argument(err);
+    }
     const fgt::Plan& P = S.plan;
     const int ndim = P.ndim, n = P.n;
     fgt::Geom& g = S.geom;
@@ -323,7 +344,9 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true) {
     t.npw = P.npw;
     S.fast3d = P.do_pw && fgt::pw2pot_3d_supported(g, t);
     if (S.fast3d) compact_slots(S.plan);
-    if (P.do_pw) alloc_expansions(S, sizeof(double2));
+    t.prec = mode_precision(prec_mode, P.eps, S.fast3d);
+    t.expn_f32 = t.prec > 0 ? 1 : 0;
+    if (P.do_pw) alloc_expansions(S, t.expn_f32 ? sizeof(float2) : sizeof(double2));
     if (S.fast3d) {
         const int nl = P.nlevels, np = P.npw;
         const size_t per = (size_t)n * np;
@@ -1597,6 +1620,287 @@ void upload_host(const DBuf& b, const double* in, const char* what) {
     }
     hcheck(hipStreamSynchronize(g.st), what);
 }
+
+// ----------------------------------------------------------------- server
+// Request serving on one GPU.  A request names a problem (tree arrays,
+// parameters, source values) by a caller key; resident problems (plan,
+// tables, lists, values, expansion workspace on the device) are cached by
+// key under a device-memory budget.  Admission: a request for a resident,
+// idle problem starts as soon as one of the server's streams is free; a
+// request for a problem that is not resident plans it on the host and, when
+// its estimated footprint fits the budget after evicting idle problems
+// least recently used first, builds it on the device -- otherwise it waits
+// (strictly in arrival order among requests that need memory) until running
+// requests finish.  Each finished request's potential is copied on the
+// device into a result buffer of its own, so the problem is free for the
+// next request at once; results stay on the device until fetched or
+// released.  All streams belong to the library.
+
+struct SrvProblem {
+    fgt_problem spec{};                 // the caller's arrays (kept alive by the caller)
+    std::unique_ptr<fgt_solver> S;      // null when not resident
+    std::unique_ptr<fgt::Plan> plan;    // host plan waiting for admission
+    size_t est = 0, bytes = 0;          // estimated / measured device bytes when resident
+    bool busy = false;
+    double last_use = 0;
+    const double* fv_ptr = nullptr;     // values resident on the device
+    long long fv_ver = -1;
+};
+
+struct SrvRequest {
+    long long key = 0;
+    const double* fvals = nullptr;
+    long long fver = 0;
+    double t_submit = 0, t_start = -1, t_done = -1;
+    int state = 0;            // 0 waiting, 1 running, 2 done, 3 failed
+    int stream = -1;
+    int flags = 0;            // bit0 plan-cache hit, bit1 planned on this request, bit2 waited for memory,
+                              // bit3 values uploaded
+    hipEvent_t ev = nullptr;
+    void* result = nullptr;   // device potential (nd, npbox, nboxes), from the result pool
+    size_t result_bytes = 0;
+    std::string err;
+};
+
+}  // namespace
+
+struct fgt_server {
+    double budget = 0;
+    size_t used = 0;
+    int mode = 0;
+    std::vector<hipStream_t> streams;
+    std::vector<int> stream_req;        // request running on each stream (-1 free)
+    std::unordered_map<long long, SrvProblem> probs;
+    std::vector<std::unique_ptr<SrvRequest>> reqs;
+    std::deque<int> waiting;
+    std::vector<int> finished;          // completed since the last poll
+    // released result buffers kept for reuse by size (hipFree would
+    // synchronise the device between requests); counted in `used`
+    std::multimap<size_t, void*> pool;
+    long long plans = 0, evictions = 0, mem_waits = 0, completed = 0;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    ~fgt_server() {
+        (void)hipDeviceSynchronize();
+        for (auto& r : reqs) {
+            if (r && r->ev) (void)hipEventDestroy(r->ev);
+            if (r && r->result) (void)hipFree(r->result);
+        }
+        for (auto& kv : pool) (void)hipFree(kv.second);
+        probs.clear();
+        for (auto st : streams) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+fgt::PlanInput plan_input(const fgt_problem& q) {
+    int npbox = 1;
+    for (int a = 0; a < q.ndim && a < 3; ++a) npbox *= q.norder;
+    return fgt::PlanInput{q.nd, q.ndim, q.delta, q.eps, q.ipoly, q.iperiod, q.norder, npbox, q.nboxes,
+                          q.nlevels, q.ltree, q.itree, q.iptr, q.centers, q.boxsize};
+}
+
+// Device bytes a problem will hold (values, potential, expansion slots as
+// the 3-D fast path compacts them, tables and lists), from its host plan.
+size_t estimate_bytes(const fgt::Plan& P, int mode) {
+    const fgt::TreeView& T = P.tree;
+    const size_t io = 2 * sizeof(double) * (size_t)P.nd * P.npbox * P.nboxes;
+    size_t slots = (size_t)P.nslots;
+    const bool fast = P.do_pw && P.ndim == 3 && (P.n == 8 || P.n == 12 || P.n == 16);
+    if (fast) {
+        slots = 0;
+        for (int l = P.nlevstart; l <= P.nlevels; ++l)
+            for (int b = T.lfirst[l]; b <= T.llast[l]; ++b)
+                if (P.slot_of_box[b] >= 0 && (T.nchild[b] > 0 || l == P.nlevstart)) ++slots;
+    }
+    const size_t elem = mode_precision(mode, P.eps, fast) > 0 ? sizeof(float2) : sizeof(double2);
+    const size_t expn = P.do_pw ? elem * 2 * (size_t)P.nd * P.nexp * (slots + 1) : 0;
+    const size_t tabs = sizeof(double) * P.loctab.size() + 10 * sizeof(double2) * (P.p2pw.size() + P.pw2pot.size()) +
+                        sizeof(int) * (P.dir_src.size() * 3 + P.locband.size() + 16 * (size_t)P.nboxes);
+    return io + expn + tabs + (size_t(1) << 20);
+}
+
+void srv_evict(fgt_server& V, long long key) {
+    SrvProblem& e = V.probs.at(key);
+    V.used -= std::min(V.used, e.bytes);
+    e.S.reset();
+    e.bytes = 0;
+    e.fv_ptr = nullptr;
+    e.fv_ver = -1;
+    ++V.evictions;
+}
+
+// A result buffer of `bytes` from the pool, or a new one.
+void* srv_take_result(fgt_server& V, size_t bytes) {
+    auto it = V.pool.find(bytes);
+    if (it != V.pool.end()) {
+        void* p = it->second;
+        V.pool.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    hcheck(hipMalloc(&p, bytes), "hipMalloc result");
+    V.used += bytes;
+    return p;
+}
+
+void srv_give_result(fgt_server& V, SrvRequest& r) {
+    if (!r.result) return;
+    V.pool.emplace(r.result_bytes, r.result);
+    r.result = nullptr;
+}
+
+// Make room for `need` bytes: first the pooled result buffers, then idle
+// resident problems least recently used first (never `keep`); false if
+// that cannot free enough.
+bool srv_make_room(fgt_server& V, size_t need, long long keep) {
+    while ((double)(V.used + need) > V.budget && !V.pool.empty()) {
+        auto it = V.pool.begin();
+        (void)hipFree(it->second);
+        V.used -= std::min(V.used, it->first);
+        V.pool.erase(it);
+    }
+    while ((double)(V.used + need) > V.budget) {
+        long long victim = 0;
+        double oldest = 1e300;
+        bool found = false;
+        for (auto& kv : V.probs) {
+            const SrvProblem& e = kv.second;
+            if (kv.first == keep || !e.S || e.busy) continue;
+            if (e.last_use < oldest) oldest = e.last_use, victim = kv.first, found = true;
+        }
+        if (!found) return false;
+        srv_evict(V, victim);
+    }
+    return true;
+}
+
+void srv_collect(fgt_server& V) {
+    for (size_t k = 0; k < V.streams.size(); ++k) {
+        const int id = V.stream_req[k];
+        if (id < 0) continue;
+        SrvRequest& r = *V.reqs[id];
+        const hipError_t q = hipEventQuery(r.ev);
+        if (q == hipErrorNotReady) continue;
+        hcheck(q, "request completion");
+        r.t_done = V.now();
+        r.state = 2;
+        V.stream_req[k] = -1;
+        SrvProblem& e = V.probs.at(r.key);
+        e.busy = false;
+        e.last_use = r.t_done;
+        V.finished.push_back(id);
+        ++V.completed;
+    }
+}
+
+// Starts one request on stream k (its problem is resident and idle).
+void srv_launch(fgt_server& V, int id, int k) {
+    SrvRequest& r = *V.reqs[id];
+    SrvProblem& e = V.probs.at(r.key);
+    fgt_solver& S = *e.S;
+    hipStream_t st = V.streams[k];
+    if (e.fv_ptr != r.fvals || e.fv_ver != r.fver) {
+        hcheck(hipMemcpyAsync(S.d_fvals.p, r.fvals, S.d_fvals.bytes, hipMemcpyHostToDevice, st), "upload fvals");
+        e.fv_ptr = r.fvals;
+        e.fv_ver = r.fver;
+        r.flags |= 8;
+    }
+    execute(S, st, false, false);
+    r.result = srv_take_result(V, S.d_pot.bytes);
+    r.result_bytes = S.d_pot.bytes;
+    hcheck(hipMemcpyAsync(r.result, S.d_pot.p, S.d_pot.bytes, hipMemcpyDeviceToDevice, st), "result copy");
+    hcheck(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming), "hipEventCreate");
+    hcheck(hipEventRecord(r.ev, st), "hipEventRecord");
+    r.state = 1;
+    r.stream = k;
+    r.t_start = V.now();
+    e.busy = true;
+    e.last_use = r.t_start;
+    V.stream_req[k] = id;
+}
+
+void srv_admit(fgt_server& V) {
+    bool mem_blocked = false;
+    for (auto it = V.waiting.begin(); it != V.waiting.end();) {
+        int k = 0;
+        while (k < (int)V.streams.size() && V.stream_req[k] >= 0) ++k;
+        if (k == (int)V.streams.size()) break;   // every stream busy
+        const int id = *it;
+        SrvRequest& r = *V.reqs[id];
+        SrvProblem& e = V.probs.at(r.key);
+        if (e.busy || mem_blocked) {
+            ++it;   // its problem runs another request, or an earlier request waits for memory
+            continue;
+        }
+        auto fail = [&](const std::string& m) {
+            r.state = 3;
+            r.err = m;
+            V.finished.push_back(id);
+            it = V.waiting.erase(it);
+        };
+        if (!e.S && !e.plan) {
+            auto pl = std::make_unique<fgt::Plan>();
+            std::string err;
+            if (fgt::make_plan(plan_input(e.spec), *pl, &err) != 0) {
+                fail(err);
+                continue;
+            }
+            if (e.est == 0) e.est = estimate_bytes(*pl, V.mode);
+            e.plan = std::move(pl);
+            r.flags |= 2;
+            ++V.plans;
+        }
+        // device memory this request needs: its problem (if not resident)
+        // and a result buffer (unless one of that size is pooled)
+        const size_t rb = sizeof(double) * (size_t)e.spec.nd * plan_input(e.spec).npbox * e.spec.nboxes;
+        const size_t need = (e.S ? 0 : e.est) + (V.pool.count(rb) ? 0 : rb);
+        if ((double)need > V.budget) {
+            fail("problem needs more device memory than the server's budget");
+            continue;
+        }
+        if (need > 0 && !srv_make_room(V, need, r.key)) {
+            bool running = false;
+            for (int q : V.stream_req) running = running || q >= 0;
+            if (!running) {
+                // nothing will free memory: the budget is held by results
+                // the caller has not fetched or released
+                fail("not enough device memory within the budget (held by unreleased results)");
+                continue;
+            }
+            // wait for running requests to finish; later requests wait
+            // behind this one
+            if (!(r.flags & 4)) ++V.mem_waits;
+            r.flags |= 4;
+            mem_blocked = true;
+            ++it;
+            continue;
+        }
+        if (!e.S) {
+            auto S = std::make_unique<fgt_solver>();
+            g_tally = 0;
+            build_solver(*S, plan_input(e.spec), true, e.plan.get(), V.mode);
+            e.plan.reset();
+            e.bytes = g_tally;
+            e.est = e.bytes;   // measured: exact for its next admission
+            V.used += e.bytes;
+            e.S = std::move(S);
+        } else {
+            r.flags |= 1;
+        }
+        srv_launch(V, id, k);
+        it = V.waiting.erase(it);
+    }
+}
+
+void srv_poll(fgt_server& V) {
+    srv_collect(V);
+    srv_admit(V);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1849,6 +2153,142 @@ void* fgt_solver_fvals_device(fgt_solver* s) { return s ? s->d_fvals.p : nullptr
 void* fgt_solver_pot_device(fgt_solver* s) { return s ? s->d_pot.p : nullptr; }
 
 void fgt_solver_destroy(fgt_solver* s) { delete s; }
+
+int fgt_server_create(double budget_bytes, int nstreams, int precision_mode, fgt_server** out) {
+    if (!out) return 2;
+    *out = nullptr;
+    return guarded([&] {
+        if (!(budget_bytes > 0)) throw std::invalid_argument("budget must be > 0");
+        if (nstreams < 1 || nstreams > 16) throw std::invalid_argument("nstreams must be 1..16");
+        if (precision_mode < 0 || precision_mode > 3) throw std::invalid_argument("precision mode must be 0..3");
+        auto V = std::make_unique<fgt_server>();
+        V->budget = budget_bytes;
+        V->mode = precision_mode;
+        for (int k = 0; k < nstreams; ++k) {
+            hipStream_t st;
+            hcheck(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+            V->streams.push_back(st);
+        }
+        V->stream_req.assign(nstreams, -1);
+        *out = V.release();
+    });
+}
+
+int fgt_server_submit(fgt_server* v, const fgt_problem* p, long long key, const double* fvals, long long fvals_version,
+                      int* req_id) {
+    return guarded([&] {
+        if (!v || !p || !fvals || !req_id) throw std::invalid_argument("null argument");
+        if (!p->itree || !p->iptr || !p->centers || !p->boxsize) throw std::invalid_argument("null tree array");
+        auto it = v->probs.find(key);
+        if (it == v->probs.end()) {
+            it = v->probs.emplace(key, SrvProblem{}).first;
+            it->second.spec = *p;
+        } else {
+            const fgt_problem& q = it->second.spec;
+            if (q.nd != p->nd || q.ndim != p->ndim || q.norder != p->norder || q.nboxes != p->nboxes ||
+                q.nlevels != p->nlevels || q.delta != p->delta || q.eps != p->eps || q.iperiod != p->iperiod ||
+                q.ipoly != p->ipoly)
+                throw std::invalid_argument("key already names a different problem");
+            it->second.spec = *p;   // the latest arrays (same tree)
+        }
+        auto r = std::make_unique<SrvRequest>();
+        r->key = key;
+        r->fvals = fvals;
+        r->fver = fvals_version;
+        r->t_submit = v->now();
+        const int id = (int)v->reqs.size();
+        v->reqs.push_back(std::move(r));
+        v->waiting.push_back(id);
+        *req_id = id;
+        srv_poll(*v);
+    });
+}
+
+int fgt_server_poll(fgt_server* v, int* done, int max_done, int* ndone) {
+    return guarded([&] {
+        if (!v || !ndone || (max_done > 0 && !done)) throw std::invalid_argument("null argument");
+        srv_poll(*v);
+        const int n = std::min<int>(max_done, (int)v->finished.size());
+        for (int i = 0; i < n; ++i) done[i] = v->finished[i];
+        v->finished.erase(v->finished.begin(), v->finished.begin() + n);
+        *ndone = n;
+    });
+}
+
+int fgt_server_drain(fgt_server* v, double timeout_s) {
+    return guarded([&] {
+        if (!v) throw std::invalid_argument("null server");
+        const double t_end = v->now() + timeout_s;
+        for (;;) {
+            srv_poll(*v);
+            bool busy = !v->waiting.empty();
+            for (int r : v->stream_req) busy = busy || r >= 0;
+            if (!busy) break;
+            if (v->now() > t_end) throw std::runtime_error("server did not drain in time");
+        }
+    });
+}
+
+int fgt_server_request_info(const fgt_server* v, int id, double* times, int* flags) {
+    if (!v || id < 0 || id >= (int)v->reqs.size()) return 2;
+    const SrvRequest& r = *v->reqs[id];
+    if (times) {
+        times[0] = r.t_submit;
+        times[1] = r.t_start;
+        times[2] = r.t_done;
+    }
+    if (flags) *flags = r.flags | (r.state << 8);
+    return 0;
+}
+
+int fgt_server_result(fgt_server* v, int id, double* pot) {
+    return guarded([&] {
+        if (!v || !pot || id < 0 || id >= (int)v->reqs.size()) throw std::invalid_argument("bad argument");
+        SrvRequest& r = *v->reqs[id];
+        if (r.state == 3) throw std::invalid_argument("request failed: " + r.err);
+        if (r.state != 2) throw std::invalid_argument("request not finished");
+        if (!r.result) throw std::invalid_argument("result already released");
+        DBuf view;   // borrowed: the pool keeps the buffer
+        view.p = r.result;
+        view.bytes = r.result_bytes;
+        view.from_arena = true;
+        download(view, pot, 0, "download result");
+        view.p = nullptr;
+        srv_give_result(*v, r);
+    });
+}
+
+int fgt_server_release(fgt_server* v, int id) {
+    return guarded([&] {
+        if (!v || id < 0 || id >= (int)v->reqs.size()) throw std::invalid_argument("bad argument");
+        SrvRequest& r = *v->reqs[id];
+        if (r.state == 1) throw std::invalid_argument("request still running");
+        srv_give_result(*v, r);
+        if (r.ev) {
+            (void)hipEventDestroy(r.ev);
+            r.ev = nullptr;
+        }
+    });
+}
+
+int fgt_server_stats(const fgt_server* v, fgt_server_info* o) {
+    if (!v || !o) return 2;
+    std::memset(o, 0, sizeof *o);
+    o->budget_bytes = v->budget;
+    o->used_bytes = (double)v->used;
+    for (const auto& kv : v->probs) o->resident += kv.second.S ? 1 : 0;
+    o->problems = (long long)v->probs.size();
+    o->requests = (long long)v->reqs.size();
+    o->completed = v->completed;
+    o->waiting = (long long)v->waiting.size();
+    for (int r : v->stream_req) o->running += r >= 0;
+    o->plans = v->plans;
+    o->evictions = v->evictions;
+    o->mem_waits = v->mem_waits;
+    return 0;
+}
+
+void fgt_server_destroy(fgt_server* v) { delete v; }
 
 int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* iperiod, int* norder,
                int* npbox, int* nboxes, int* nlevels, int* ltree, int* itree, int* iptr, double* centers,

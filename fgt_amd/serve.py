@@ -1,117 +1,161 @@
-"""Serving loop: continuous batching of box-FGT requests on one GPU.
+"""Serving: box-FGT requests on one GPU through the library's server
+(fgt_server_* in include/fgt.h; the request loop, admission, plan cache and
+streams are all in libfgt_amd.so).
 
-Requests name a problem (a tree with delta and eps, registered with its
-source values).  Problems stay resident on the device (plan, tables,
-lists, values and expansion workspace) while they fit an HBM budget; least
-recently used ones are evicted and planned again on their next request.  Each admitted request is launched on the next free HIP
-stream as soon as one frees up, so requests that arrive while others run
-overlap with them on the device.  Latency = completion - arrival.
+A request names its problem by a key and brings its source values.  The
+server keeps problems resident under a device-memory budget (plan cache by
+key), plans and builds a missing one on the request path when it fits
+(evicting idle problems least recently used first) and otherwise queues the
+request until running ones finish; up to ``nstreams`` requests run at once,
+overlapping on the device.  Results stay on the device until fetched.
+
+``run_stream`` is a client: it submits Poisson arrivals at their times,
+polls the server in between and records each request's latency (finish
+minus scheduled arrival).
 """
 from __future__ import annotations
 
-import collections
+import ctypes as C
 import time
 
 import numpy as np
 
-from . import Solver
+from . import _check, _Problem, _p, lib
+
+
+class _Info(C.Structure):
+    _fields_ = [("budget_bytes", C.c_double), ("used_bytes", C.c_double), ("problems", C.c_longlong),
+                ("resident", C.c_longlong), ("requests", C.c_longlong), ("completed", C.c_longlong),
+                ("waiting", C.c_longlong), ("running", C.c_longlong), ("plans", C.c_longlong),
+                ("evictions", C.c_longlong), ("mem_waits", C.c_longlong)]
+
+
+STATES = {0: "waiting", 1: "running", 2: "done", 3: "failed"}
 
 
 class Server:
-    def __init__(self, nstreams=4, hbm_budget=200 << 30):
-        import torch
-        self.torch = torch
-        self.streams = [torch.cuda.Stream() for _ in range(nstreams)]
-        self.busy = [None] * nstreams          # (request id, event) per stream
-        self.budget = hbm_budget
-        self.resident = collections.OrderedDict()   # problem key -> (solver, bytes)
-        self.used = 0
-        self.problems = {}                     # key -> (tree, delta, eps)
+    def __init__(self, budget=200 << 30, nstreams=4, precision=0):
+        self._h = C.c_void_p()
+        _check(lib().fgt_server_create(C.c_double(budget), C.c_int(nstreams), C.c_int(precision),
+                                       C.byref(self._h)))
+        self._keep = {}     # key -> arrays the server points at
+        self._fv = {}       # request id -> values (alive until finished)
+        self._shape = {}    # request id -> potential shape
 
-    def register(self, key, tree, delta, eps):
-        self.problems[key] = (tree, delta, eps)
+    def submit(self, key, tree, delta, eps, fvals=None, version=0, nd=1):
+        """Queue one request; returns its id.  fvals defaults to tree.fvals."""
+        if key not in self._keep:
+            arrs = [np.ascontiguousarray(tree.itree, np.int32), np.ascontiguousarray(tree.iptr, np.int32),
+                    np.ascontiguousarray(tree.centers, np.float64),
+                    np.ascontiguousarray(tree.boxsize, np.float64)]
+            prob = _Problem(nd, tree.ndim, delta, eps, tree.ipoly, tree.iperiod, tree.norder, tree.nboxes,
+                            tree.nlevels, tree.ltree, *(a.ctypes.data for a in arrs))
+            self._keep[key] = (arrs, prob)
+        _, prob = self._keep[key]
+        fv = np.ascontiguousarray(tree.fvals if fvals is None else fvals, np.float64)
+        rid = C.c_int()
+        _check(lib().fgt_server_submit(self._h, C.byref(prob), C.c_longlong(key), _p(fv), C.c_longlong(version),
+                                       C.byref(rid)))
+        self._fv[rid.value] = fv
+        self._shape[rid.value] = fv.shape
+        return rid.value
 
-    def _footprint(self, s):
-        st = s.stats()
-        return st["expansion_bytes"] + 2 * 8 * s.nd * s.npbox * s.nboxes
+    def poll(self, max_done=4096):
+        buf = (C.c_int * max_done)()
+        n = C.c_int()
+        _check(lib().fgt_server_poll(self._h, buf, C.c_int(max_done), C.byref(n)))
+        done = list(buf[:n.value])
+        for r in done:
+            self._fv.pop(r, None)
+        return done
 
-    def _solver(self, key):
-        if key in self.resident:
-            self.resident.move_to_end(key)
-            return self.resident[key][0]
-        tree, delta, eps = self.problems[key]
-        s = Solver.from_tree(tree, delta, eps)
-        s.set_fvals(tree.fvals)
-        nb = self._footprint(s)
-        while self.used + nb > self.budget and self.resident:
-            old, (os_, ob) = self.resident.popitem(last=False)
-            self.torch.cuda.synchronize()
-            os_.close()
-            self.used -= ob
-        self.resident[key] = (s, nb)
-        self.used += nb
-        return s
+    def drain(self, timeout=120.0):
+        _check(lib().fgt_server_drain(self._h, C.c_double(timeout)))
+        self._fv.clear()
 
-    def result(self, key):
-        """Potential of the last request served for a resident problem."""
-        self.torch.cuda.synchronize()
-        return self.resident[key][0].get_pot()
+    def info(self, rid):
+        t = (C.c_double * 3)()
+        f = C.c_int()
+        _check(lib().fgt_server_request_info(self._h, C.c_int(rid), t, C.byref(f)))
+        fl = f.value
+        return {"submit": t[0], "start": t[1], "done": t[2], "cache_hit": bool(fl & 1), "planned": bool(fl & 2),
+                "waited_for_memory": bool(fl & 4), "uploaded": bool(fl & 8), "state": STATES[(fl >> 8) & 3]}
 
-    def run(self, arrivals, horizon):
-        """arrivals: list of (time offset s, request id, problem key), sorted.
-        Returns per-request latency (s) and the wall time."""
-        torch = self.torch
-        pending = collections.deque()
-        lat = {}
-        t0 = time.perf_counter()
-        i = 0
-        inflight_key = {}
-        while i < len(arrivals) or pending or any(b is not None for b in self.busy):
-            now = time.perf_counter() - t0
-            while i < len(arrivals) and arrivals[i][0] <= now:
-                pending.append(arrivals[i])
-                i += 1
-            for k, b in enumerate(self.busy):
-                if b is not None and b[1].query():
-                    rid, _, arr = b
-                    lat[rid] = time.perf_counter() - t0 - arr
-                    inflight_key.pop(rid, None)
-                    self.busy[k] = None
-            for k in range(len(self.streams)):
-                if self.busy[k] is None and pending:
-                    arr, rid, key = pending.popleft()
-                    if key in inflight_key.values():
-                        pending.appendleft((arr, rid, key))   # one request per problem at a time
-                        break
-                    s = self._solver(key)
-                    st = self.streams[k]
-                    s.execute(stream=st.cuda_stream)
-                    ev = torch.cuda.Event()
-                    ev.record(st)
-                    self.busy[k] = (rid, ev, arr)
-                    inflight_key[rid] = key
-            if not pending and i < len(arrivals) and all(b is None for b in self.busy):
-                time.sleep(max(0.0, min(arrivals[i][0] - (time.perf_counter() - t0), 1e-3)))
-            if time.perf_counter() - t0 > horizon * 4 + 30:
-                raise RuntimeError("serving loop did not drain")
-        return lat, time.perf_counter() - t0
+    def result(self, rid):
+        out = np.zeros(self._shape[rid])
+        _check(lib().fgt_server_result(self._h, C.c_int(rid), _p(out)))
+        return out
+
+    def release(self, rid):
+        _check(lib().fgt_server_release(self._h, C.c_int(rid)))
+
+    def stats(self):
+        s = _Info()
+        _check(lib().fgt_server_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Info._fields_}
 
     def close(self):
-        self.torch.cuda.synchronize()
-        for s, _ in self.resident.values():
-            s.close()
-        self.resident.clear()
-        self.used = 0
+        if self._h:
+            lib().fgt_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def poisson_arrivals(rate, duration, nkeys, seed=0):
     """Seeded Poisson arrivals over `duration` seconds at `rate` per second,
-    each naming one of `nkeys` problems uniformly."""
+    each naming one of `nkeys` problems uniformly: [(t, request no, key)]."""
     rng = np.random.default_rng(seed)
-    t, out, rid = 0.0, [], 0
+    t, out, n = 0.0, [], 0
     while True:
         t += rng.exponential(1.0 / rate)
         if t >= duration:
             return out
-        out.append((t, rid, int(rng.integers(nkeys))))
-        rid += 1
+        out.append((t, n, int(rng.integers(nkeys))))
+        n += 1
+
+
+def run_stream(server, problems, arrivals, keep_results=(), release=True, timeout=None):
+    """Submit each arrival (t, n, key) at its time (problems[key] = (tree,
+    delta, eps)), polling in between; returns per-arrival latency (s, finish
+    on the server minus scheduled arrival), the wall time from the first
+    arrival to the last finish, and {n: request id} for arrivals whose
+    results are kept (others are released as they finish)."""
+    keep = set(keep_results)
+    rid_of, lat = {}, {}
+    n_of = {}
+    t0 = time.perf_counter()
+    off = None     # server clock minus client clock
+    horizon = arrivals[-1][0] if arrivals else 0.0
+    limit = timeout if timeout is not None else 4 * horizon + 120.0
+    i = 0
+    while i < len(arrivals) or len(lat) < len(arrivals):
+        now = time.perf_counter() - t0
+        if now > limit:
+            raise RuntimeError(f"stream did not finish within {limit:.0f}s ({len(lat)} of {len(arrivals)} done)")
+        while i < len(arrivals) and arrivals[i][0] <= now:
+            ta, n, key = arrivals[i]
+            tr, delta, eps = problems[key]
+            tc = time.perf_counter() - t0
+            rid = server.submit(key, tr, delta, eps)
+            if off is None:
+                off = server.info(rid)["submit"] - tc
+            rid_of[n] = rid
+            n_of[rid] = (n, ta)
+            i += 1
+        for rid in server.poll():
+            n, ta = n_of[rid]
+            inf = server.info(rid)
+            if inf["state"] != "done":
+                raise RuntimeError(f"request {n} {inf['state']}")
+            lat[n] = inf["done"] - off - ta
+            if release and n not in keep:
+                server.release(rid)
+        if i < len(arrivals) and len(lat) == i:
+            time.sleep(max(0.0, min(arrivals[i][0] - (time.perf_counter() - t0), 5e-4)))
+    wall = max(lat[n] + arrivals[k][0] for k, (_, n, _) in enumerate(arrivals)) - arrivals[0][0] if arrivals else 0.0
+    return lat, wall, {n: rid_of[n] for n in keep}

@@ -210,6 +210,47 @@ int fgt_batch_get_pot(fgt_batch *b, int i, double *pot, int accumulate);
 int fgt_batch_stats(const fgt_batch *b, long long *nout, double *last_ms);
 void fgt_batch_destroy(fgt_batch *b);
 
+/* ---- serving (no counterpart in the reference) ----
+ * A server keeps problems resident on one GPU under a device-memory budget
+ * and runs requests as they arrive on its own HIP streams (up to nstreams
+ * at once, overlapping on the device).  A request names its problem by a
+ * caller key (equal keys = the same tree and parameters; the arrays of a
+ * key's problem must stay valid while the server lives) and brings its
+ * source values (host fvals, valid until the request finishes; uploaded
+ * when the request starts unless the same pointer and version are already
+ * resident).  A problem that is not resident is planned on the host when
+ * its request comes up and built when its estimated footprint fits the
+ * budget after evicting idle problems least recently used first; otherwise
+ * the request waits for running ones to finish, and later requests wait
+ * behind it (a request whose problem is running another request may be
+ * overtaken).  Result buffers count against the budget too.  precision_mode: as fgt_batch_set_precision, per problem.  Each
+ * finished request's potential (nd,npbox,nboxes) stays on the device until
+ * fgt_server_result copies it out or fgt_server_release drops it. */
+typedef struct fgt_server fgt_server;
+typedef struct {
+    double budget_bytes, used_bytes;
+    long long problems, resident, requests, completed, waiting, running;
+    long long plans, evictions, mem_waits;
+} fgt_server_info;
+int fgt_server_create(double budget_bytes, int nstreams, int precision_mode, fgt_server **out);
+int fgt_server_submit(fgt_server *v, const fgt_problem *p, long long key, const double *fvals,
+                      long long fvals_version, int *req_id);
+/* advances the server (collects finished requests, admits and starts
+ * waiting ones); done[] receives up to max_done ids finished since the last
+ * poll */
+int fgt_server_poll(fgt_server *v, int *done, int max_done, int *ndone);
+/* polls until no request is waiting or running (status 6 after timeout_s) */
+int fgt_server_drain(fgt_server *v, double timeout_s);
+/* times[3]: submit, start, finish (seconds on the server's clock, -1 if
+ * not yet); flags: bit0 plan-cache hit, bit1 planned for this request, bit2
+ * waited for memory, bit3 values uploaded, bits 8-9 state (0 waiting, 1
+ * running, 2 done, 3 failed) */
+int fgt_server_request_info(const fgt_server *v, int id, double *times, int *flags);
+int fgt_server_result(fgt_server *v, int id, double *pot);
+int fgt_server_release(fgt_server *v, int id);
+int fgt_server_stats(const fgt_server *v, fgt_server_info *out);
+void fgt_server_destroy(fgt_server *v);
+
 #ifdef __cplusplus
 }
 #endif
