@@ -287,6 +287,7 @@ def headline(probs, steps, warmup, mode, drop_in=True):
             fgt_amd.boxfgt(1, 3, delta, eps, 0, 0, tr.norder, tr.npbox, tr.nboxes, tr.nlevels, tr.ltree,
                            tr.itree, tr.iptr, tr.centers, tr.boxsize, tr.fvals, 1, pots[k])
         t_drop = time.perf_counter() - t0
+        fgt_amd.lib().fgt_dropin_cache_clear()   # free the cached plans' device memory
         for k, (tr, _, _) in enumerate(probs):
             l2, _ = pot_error(pots[k][tr.leaves], ref[k])
             if l2 > 1e-12:
@@ -561,22 +562,30 @@ def c2_leg(args):
            "npwlevel": st["npwlevel"], "npw": st["npw"], "pw_leaves": int(st["npw_leaves"]),
            "direct_pairs": int(st["ndirect_pairs"])}
     if not args.no_drop_in:
+        # the first boxfgt_ call on this tree plans and uploads; later calls on
+        # the same tree arrays hit the library's plan cache and move only the
+        # values and the potential
+        fgt_amd.lib().fgt_dropin_cache_clear()
         pot = np.zeros_like(tree.fvals)
-        best, split = None, None
-        for _ in range(2):
+        calls = []
+        for _ in range(4):
             pot[...] = 0.0
             ti = np.zeros(20)
             t0 = time.perf_counter()
             fgt_amd.boxfgt(1, 3, cfg["delta"], cfg["eps"], 0, cfg["iperiod"], n, tree.npbox, tree.nboxes,
                            tree.nlevels, tree.ltree, tree.itree, tree.iptr, tree.centers, tree.boxsize,
                            tree.fvals, 1, pot, timeinfo=ti)
-            dt = time.perf_counter() - t0
-            if best is None or dt < best:
-                best, split = dt, {"plan_and_upload_ms": round(ti[0] * 1e3, 2),
-                                   "download_and_accumulate_ms": round(ti[7] * 1e3, 2)}
-        leg["drop_in_end_to_end"] = {"ms": round(best * 1e3, 2), "nodes_per_s": round(nout / best, 1), **split,
-                                     "includes": "plan + H2D + STEPs 1-6 + D2H, one boxfgt_ call, best of 2"}
-        log(f"c2 drop-in end to end: {best * 1e3:.1f} ms")
+            calls.append((time.perf_counter() - t0, ti[0], ti[7]))
+        first, rep = calls[0], min(calls[1:])
+        leg["drop_in_end_to_end"] = {
+            "first_call_ms": round(first[0] * 1e3, 2), "plan_and_upload_ms": round(first[1] * 1e3, 2),
+            "repeated_call_ms": round(rep[0] * 1e3, 2), "repeated_upload_ms": round(rep[1] * 1e3, 2),
+            "download_and_accumulate_ms": round(rep[2] * 1e3, 2),
+            "nodes_per_s_first": round(nout / first[0], 1), "nodes_per_s_repeated": round(nout / rep[0], 1),
+            "includes": "one boxfgt_ call: plan (first call only) + H2D + STEPs 1-6 + D2H; repeated = best of "
+                        "the 3 calls after the first on the same tree (plan cache hit)"}
+        fgt_amd.lib().fgt_dropin_cache_clear()
+        log(f"c2 drop-in end to end: first {first[0] * 1e3:.1f} ms, repeated {rep[0] * 1e3:.1f} ms")
     if args.copy_calibration:
         # plain device copy of a known size: the scale for the PMC byte counters
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB

@@ -1621,6 +1621,82 @@ void upload_host(const DBuf& b, const double* in, const char* what) {
     hcheck(hipStreamSynchronize(g.st), what);
 }
 
+// -------------------------------------------------------- drop-in cache
+// Repeated boxfgt_ calls on one tree keep their plan and device buffers:
+// the last few solvers stay resident, keyed by the call's parameters and
+// the tree arrays (a 64-bit hash finds the entry, a stored copy of the
+// arrays confirms it, so a collision can never return another tree's
+// plan).  Only fvals move and STEPs 1-6 run on a hit.  Entries beyond the
+// count or byte limit are dropped least recently used first.
+constexpr int kDropinCacheEntries = 4;
+constexpr size_t kDropinCacheBytes = size_t(32) << 30;
+
+struct DropinKey {
+    int nd, ndim, ipoly, iperiod, norder, nboxes, nlevels, ltree;
+    double delta, eps;
+    std::vector<int> itree, iptr;
+    std::vector<double> centers, boxsize;
+    unsigned long long hash = 0;
+};
+
+unsigned long long mix_hash(unsigned long long h, const void* p, size_t n) {
+    // FNV-1a over 8-byte words (tail bytes one at a time)
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        unsigned long long w;
+        std::memcpy(&w, c + i, 8);
+        h = (h ^ w) * 1099511628211ULL;
+    }
+    for (; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+    return h;
+}
+
+DropinKey dropin_key(const fgt::PlanInput& in) {
+    DropinKey k{in.nd, in.ndim, in.ipoly, in.iperiod, in.norder, in.nboxes, in.nlevels, in.ltree, in.delta, in.eps,
+                {}, {}, {}, {}, 0};
+    k.itree.assign(in.itree, in.itree + in.ltree);
+    k.iptr.assign(in.iptr, in.iptr + 8);
+    k.centers.assign(in.centers, in.centers + (size_t)in.ndim * in.nboxes);
+    k.boxsize.assign(in.boxsize, in.boxsize + in.nlevels + 1);
+    unsigned long long h = 1469598103934665603ULL;
+    const int sc[8] = {k.nd, k.ndim, k.ipoly, k.iperiod, k.norder, k.nboxes, k.nlevels, k.ltree};
+    h = mix_hash(h, sc, sizeof sc);
+    h = mix_hash(h, &k.delta, sizeof k.delta);
+    h = mix_hash(h, &k.eps, sizeof k.eps);
+    h = mix_hash(h, k.itree.data(), sizeof(int) * k.itree.size());
+    h = mix_hash(h, k.iptr.data(), sizeof(int) * k.iptr.size());
+    h = mix_hash(h, k.centers.data(), sizeof(double) * k.centers.size());
+    h = mix_hash(h, k.boxsize.data(), sizeof(double) * k.boxsize.size());
+    k.hash = h;
+    return k;
+}
+
+bool same_key(const DropinKey& a, const DropinKey& b) {
+    return a.hash == b.hash && a.nd == b.nd && a.ndim == b.ndim && a.ipoly == b.ipoly && a.iperiod == b.iperiod &&
+           a.norder == b.norder && a.nboxes == b.nboxes && a.nlevels == b.nlevels && a.ltree == b.ltree &&
+           a.delta == b.delta && a.eps == b.eps && a.itree == b.itree && a.iptr == b.iptr &&
+           a.centers == b.centers && a.boxsize == b.boxsize;
+}
+
+struct DropinEntry {
+    DropinKey key;
+    std::unique_ptr<fgt_solver> S;
+    size_t bytes = 0;
+    unsigned long long last = 0;
+};
+
+struct DropinCache {
+    std::mutex m;
+    std::vector<DropinEntry> e;
+    unsigned long long tick = 0;
+    long long hits = 0, misses = 0;
+};
+DropinCache& dropin_cache() {
+    static DropinCache c;
+    return c;
+}
+
 // ----------------------------------------------------------------- server
 // Request serving on one GPU.  A request names a problem (tree arrays,
 // parameters, source values) by a caller key; resident problems (plan,
@@ -2315,12 +2391,45 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         const double t0 = now_s();
         fgt::PlanInput in{*nd, *ndim, *delta, *eps, *ipoly, *iperiod, *norder, *npbox, *nboxes, *nlevels,
                           *ltree, itree, iptr, centers, boxsize};
-        fgt_solver S;
-        build_solver(S, in);
+        // the plan and device buffers of an earlier call on the same tree
+        // and parameters are reused (see DropinCache)
+        DropinCache& cache = dropin_cache();
+        std::unique_lock<std::mutex> lk(cache.m);
+        DropinKey key = dropin_key(in);
+        DropinEntry* hit = nullptr;
+        for (auto& en : cache.e)
+            if (same_key(en.key, key)) hit = &en;
+        if (hit) {
+            ++cache.hits;
+        } else {
+            ++cache.misses;
+            auto Sn = std::make_unique<fgt_solver>();
+            g_tally = 0;
+            build_solver(*Sn, in);
+            DropinEntry en;
+            en.key = std::move(key);
+            en.bytes = g_tally;
+            en.S = std::move(Sn);
+            // keep at most kDropinCacheEntries (and kDropinCacheBytes): drop
+            // the least recently used first
+            size_t total = en.bytes;
+            for (auto& o : cache.e) total += o.bytes;
+            while (!cache.e.empty() && ((int)cache.e.size() >= kDropinCacheEntries || total > kDropinCacheBytes)) {
+                auto old = std::min_element(cache.e.begin(), cache.e.end(),
+                                            [](const DropinEntry& a, const DropinEntry& b) { return a.last < b.last; });
+                total -= old->bytes;
+                cache.e.erase(old);
+            }
+            cache.e.push_back(std::move(en));
+            hit = &cache.e.back();
+        }
+        hit->last = ++cache.tick;
+        fgt_solver& S = *hit->S;
         // gradients / Hessians are formed when the caller or the targets need
         // them (boxfgt.f:202-272 routes ifpgh < ifpghtarg the same way)
         const int ifpgh_dev = std::max(*ifpgh, nt > 0 ? *ifpghtarg : 0);
-        if (ifpgh_dev > 1) {
+        {
+            // a cached solver keeps the previous call's output set
             const int rc = fgt_solver_set_ifpgh(&S, ifpgh_dev);
             if (rc) throw std::runtime_error(g_msg);
         }
@@ -2384,6 +2493,24 @@ int fgt_boxfgt_noalloc(int nd, int ndim, double delta, double eps, int ipoly, in
         execute(S, st, false, false, &ov);
         hcheck(hipGetLastError(), "launch");
         hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    });
+}
+
+int fgt_dropin_cache_stats(long long* hits, long long* misses, int* entries) {
+    DropinCache& c = dropin_cache();
+    std::lock_guard<std::mutex> lk(c.m);
+    if (hits) *hits = c.hits;
+    if (misses) *misses = c.misses;
+    if (entries) *entries = (int)c.e.size();
+    return 0;
+}
+
+int fgt_dropin_cache_clear(void) {
+    return guarded([&] {
+        DropinCache& c = dropin_cache();
+        std::lock_guard<std::mutex> lk(c.m);
+        hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        c.e.clear();
     });
 }
 

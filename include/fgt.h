@@ -73,6 +73,15 @@ int fgt_boxfgt_noalloc(int nd, int ndim, double delta, double eps, int ipoly, in
                        const double *centers, const double *boxsize, const double *fvals, double *pot,
                        void *stream);
 
+/* boxfgt_ / fgt_boxfgt keep the plan and device buffers of their last few
+ * trees (up to 4 entries, 32 GiB): a repeated call on the same tree arrays
+ * and parameters skips planning and allocation and moves only fvals and the
+ * outputs.  Entries are matched by comparing the stored arrays, not only a
+ * hash.  Counters (nullable outputs) and a way to free the device memory
+ * they hold.  No counterpart in the reference, which replans every call. */
+int fgt_dropin_cache_stats(long long *hits, long long *misses, int *entries);
+int fgt_dropin_cache_clear(void);
+
 /* Status of the last call made on this thread: 0 = ok.  Copies the message
  * (NUL-terminated, truncated to len) into msg when msg != NULL.  Added error
  * channel; the reference has none (boxfgt.f:191 ignores its allocation

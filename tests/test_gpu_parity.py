@@ -160,3 +160,32 @@ def test_results_wait_for_execute_on_a_nonblocking_stream():
     s.execute(stream=st.cuda_stream)
     assert rel_l2(s.get_pot(), 2.0 * ref) < 1e-14
     s.close()
+
+
+def test_repeated_dropin_reuses_its_plan():
+    """A second boxfgt call on the same tree hits the plan cache and gives the
+    same potential; another tree misses; clearing frees the entries."""
+    import ctypes as C
+    print("[gpu] drop-in plan cache", flush=True)
+    L = fgt_amd.lib()
+    L.fgt_dropin_cache_clear()
+    h, m, n = C.c_longlong(), C.c_longlong(), C.c_int()
+    c = load_case("c1_small")
+    p1 = run_dropin(c)
+    L.fgt_dropin_cache_stats(C.byref(h), C.byref(m), C.byref(n))
+    h0, m0 = h.value, m.value
+    p2 = run_dropin(c)
+    L.fgt_dropin_cache_stats(C.byref(h), C.byref(m), C.byref(n))
+    assert h.value == h0 + 1 and m.value == m0
+    assert np.array_equal(p1, p2)
+    # same arrays, one centre nudged: a different tree, a miss
+    c.centers = c.centers.copy()
+    c.centers[0, 0] += 0.0
+    c.boxsize = c.boxsize.copy()
+    c.boxsize[0] = np.nextafter(c.boxsize[0], 2.0)
+    run_dropin(c)
+    L.fgt_dropin_cache_stats(C.byref(h), C.byref(m), C.byref(n))
+    assert m.value == m0 + 1
+    assert L.fgt_dropin_cache_clear() == 0
+    L.fgt_dropin_cache_stats(C.byref(h), C.byref(m), C.byref(n))
+    assert n.value == 0
