@@ -167,11 +167,13 @@ int map_m2l(int ndim, int idx) {
     if (ndim == 2) return (idx % 3) + 3 * 1 + 9 * (idx / 3);
     return 1 + 3 * 1 + 9 * idx;
 }
-int map_code(int ndim, int c) {
-    const int a0 = c & 15, a1 = (c >> 4) & 15, a2 = (c >> 8) & 15;
-    if (ndim == 3) return a0 | (a1 << 4) | (a2 << 8);
-    if (ndim == 2) return a0 | (6 << 4) | (a1 << 8);
-    return 6 | (6 << 4) | (a0 << 8);
+// table slot codes, 8 bits per axis; unit axes take the centre slot
+// (same-box table), which the kernels read but do not apply
+int map_code(int ndim, int c, int centre = 6) {
+    const int a0 = c & 255, a1 = (c >> 8) & 255, a2 = (c >> 16) & 255;
+    if (ndim == 3) return a0 | (a1 << 8) | (a2 << 16);
+    if (ndim == 2) return a0 | (centre << 8) | (a1 << 16);
+    return centre | (centre << 8) | (a0 << 16);
 }
 
 std::vector<double2> to_d2(const std::vector<fgt::cplx>& v) {
@@ -618,7 +620,7 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true, f
         for (int i = a0; i < b0; ++i) idx[i - a0] = i;
         std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
             const int cx = map_code(ndim, P.dir_code[x]), cy = map_code(ndim, P.dir_code[y]);
-            return (cx >> 4) < (cy >> 4);
+            return (cx >> 8) < (cy >> 8);
         });
         for (int i = a; i < b; ++i) {
             ds[i] = P.dir_src[idx[i - a]];
@@ -626,8 +628,8 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true, f
         }
         for (int i = a; i < b; ++i) {
             const bool last = (i + 1 == b);
-            if (last || (dc[i] >> 4) != (dc[i + 1] >> 4)) df[i] |= 1;
-            if (last || (dc[i] >> 8) != (dc[i + 1] >> 8)) df[i] |= 2;
+            if (last || (dc[i] >> 8) != (dc[i + 1] >> 8)) df[i] |= 1;
+            if (last || (dc[i] >> 16) != (dc[i + 1] >> 16)) df[i] |= 2;
         }
     }
     S.d_dir_tgt.upload(dt);

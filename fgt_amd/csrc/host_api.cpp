@@ -33,12 +33,15 @@ void fgt_host_pw2pot(int ipoly, int n, int npw, const double* ts, double delta, 
     std::memcpy(tab, t.data(), sizeof(double) * 2 * t.size());
 }
 
-void fgt_host_loctab(double eps, int ipoly, int n, double delta, double boxdim, double* tab, int* ind) {
+void fgt_host_loctab_mref(double eps, int ipoly, int n, double delta, double boxdim, int mref, double* tab,
+                          int* ind) {
     const size_t nn = static_cast<size_t>(n) * n;
-    std::vector<double> t(nn * fgt::kLocSlots);
-    std::vector<int> band(static_cast<size_t>(2) * n * fgt::kLocSlots), cols(2 * fgt::kLocSlots);
-    fgt::make_loctab_level(eps, ipoly, n, delta, boxdim, t.data(), band.data(), cols.data());
-    for (int s = 0; s < fgt::kLocSlots; ++s) {
+    const int ns = 2 * fgt::loc_nloctab(mref) + 1;
+    std::vector<double> t(nn * ns);
+    std::vector<int> band(static_cast<size_t>(2) * n * ns), cols(2 * ns);
+    fgt::make_loctab_level(eps, ipoly, n, delta, boxdim, t.data(), band.data(), cols.data(), nullptr, nullptr,
+                           mref);
+    for (int s = 0; s < ns; ++s) {
         for (int k = 0; k < n; ++k)
             for (int j = 0; j < n; ++j) tab[s * nn + j + static_cast<size_t>(k) * n] = t[(s * n + k) * n + j];
         int* o = ind + static_cast<size_t>(s) * 2 * (n + 1);
@@ -50,6 +53,10 @@ void fgt_host_loctab(double eps, int ipoly, int n, double delta, double boxdim, 
         o[2 * n] = cols[2 * s] + 1;
         o[2 * n + 1] = cols[2 * s + 1] + 1;
     }
+}
+
+void fgt_host_loctab(double eps, int ipoly, int n, double delta, double boxdim, double* tab, int* ind) {
+    fgt_host_loctab_mref(eps, ipoly, n, delta, boxdim, 0, tab, ind);
 }
 
 void fgt_host_pw2pgh(int ipoly, int n, int npw, const double* ts, double delta, double boxdim, double* tab,
@@ -130,7 +137,7 @@ long long fgt_plan_direct_pairs(int nd, int ndim, double delta, double eps, int 
             if (p >= cap) continue;
             tgt[p] = P.dir_tgt[t] + 1;
             src[p] = P.dir_src[j] + 1;
-            for (int a = 0; a < 3; ++a) ixyz[3 * p + a] = a < ndim ? ((P.dir_code[j] >> (4 * a)) & 15) - 6 : 0;
+            for (int a = 0; a < 3; ++a) ixyz[3 * p + a] = a < ndim ? ((P.dir_code[j] >> (8 * a)) & 255) - 6 : 0;
         }
     return total;
 }

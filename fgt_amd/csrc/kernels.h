@@ -141,9 +141,13 @@ struct DevTables {
     const double2* wm2l;
     // identity table for unit axes (one complex 1.0)
     const double2* one;
-    // near-field masked tables loc[kind][l][slot][k][j], bands [l][slot][k][2]
+    // near-field masked tables loc[kind][l][slot][k][j], bands [l][slot][k][2];
+    // loc_slots table slots per level (13 for the solve, 193 for the
+    // refinement of a new tree); a pair's table slots are packed 8 bits per
+    // kernel axis: x | y << 8 | z << 16
     const double* loctab;
     const int* locband;
+    int loc_slots = 13;
     // kind strides (elements): pw2pot / pw2potT, pw2potS, loctab
     size_t pw2pot_ks, pw2potS_ks, loctab_ks;
     int n;     // norder

@@ -491,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
     int* sB = (int*)(sA + 3 * n * n);  // 3 bands [k][2]
     const size_t nn = (size_t)n * n;
     const int kax[3] = {kd.x, kd.y, kd.z};
-    const int* bandl = t.locband + (size_t)l * 13 * 2 * n;
+    const int* bandl = t.locband + (size_t)l * t.loc_slots * 2 * n;
     const int k1 = threadIdx.x % g.nx, k2 = threadIdx.x / g.nx;
     for (int ind = 0; ind < g.nd; ++ind) {
         double acc[16];
@@ -502,8 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_direct(Geom g, DevTables t, const in
             for (int i = threadIdx.x; i < g.npbox; i += blockDim.x)
                 sf[i] = fvals[((size_t)sb * g.npbox + i) * g.nd + ind];
             for (int a = 0; a < 3; ++a) {
-                const int slot = (c >> (4 * a)) & 15;
-                const double* A = t.loctab + kax[a] * t.loctab_ks + ((size_t)l * 13 + slot) * nn;
+                const int slot = (c >> (8 * a)) & 255;
+                const double* A = t.loctab + kax[a] * t.loctab_ks + ((size_t)l * t.loc_slots + slot) * nn;
                 const int* B = bandl + (size_t)slot * 2 * n;
                 for (int i = threadIdx.x; i < n * n; i += blockDim.x) sA[a * nn + i] = A[i];
                 for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) sB[a * 2 * n + i] = B[i];
@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(kBlock) k_direct2(Geom g, DevTables t, const i
     double* Ax = h + nn;
     double* Az = Ax + nn;
     int* band = (int*)(Az + nn);
-    const int* bandl = t.locband + (size_t)l * 13 * 2 * n;
+    const int* bandl = t.locband + (size_t)l * t.loc_slots * 2 * n;
     const int tid = threadIdx.x;
     const bool act = tid < (int)nn;
     const int r = tid % n, c = tid / n;   // (k1, j3) in the first stage, (k1, k3) in the second
@@ -573,10 +573,10 @@ __global__ void __launch_bounds__(kBlock) k_direct2(Geom g, DevTables t, const i
         double acc = 0.0;
         for (int j = off[task]; j < off[task + 1]; ++j) {
             const int sb = src[j], cd = code[j];
-            const int slx = cd & 15, slz = (cd >> 8) & 15;
+            const int slx = cd & 255, slz = (cd >> 16) & 255;
             __syncthreads();
-            const double* __restrict__ A = t.loctab + kd.x * t.loctab_ks + ((size_t)l * 13 + slx) * nn;
-            const double* __restrict__ Z = t.loctab + kd.z * t.loctab_ks + ((size_t)l * 13 + slz) * nn;
+            const double* __restrict__ A = t.loctab + kd.x * t.loctab_ks + ((size_t)l * t.loc_slots + slx) * nn;
+            const double* __restrict__ Z = t.loctab + kd.z * t.loctab_ks + ((size_t)l * t.loc_slots + slz) * nn;
             if (act) {
                 sf[tid] = fvals[((size_t)sb * g.npbox + tid) * g.nd + ind];
                 Ax[tid] = A[tid];

@@ -19,9 +19,10 @@ namespace fgt {
 
 namespace {
 
-// flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group
+// flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group;
+// nls table slots per level, slot codes 8 bits per axis
 template <int N>
-__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab_a,
+__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab_a, int nls,
                                                 const int* __restrict__ tgt, const int* __restrict__ lev,
                                                 const int* __restrict__ off, const int* __restrict__ src,
                                                 const int* __restrict__ code, const int* __restrict__ flags,
@@ -40,9 +41,9 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
     }
     const int tb = tgt[task], l = lev[task];
     // table kind per axis (value / first / second derivative): stride ks
-    const CTab1 tabx = ctab(loctab) + (long long)(kd.x * ks + (size_t)l * 13 * NN);
-    const CTab1 taby = ctab(loctab) + (long long)(kd.y * ks + (size_t)l * 13 * NN);
-    const CTab1 tabz = ctab(loctab) + (long long)(kd.z * ks + (size_t)l * 13 * NN);
+    const CTab1 tabx = ctab(loctab) + (long long)(kd.x * ks + (size_t)l * nls * NN);
+    const CTab1 taby = ctab(loctab) + (long long)(kd.y * ks + (size_t)l * nls * NN);
+    const CTab1 tabz = ctab(loctab) + (long long)(kd.z * ks + (size_t)l * nls * NN);
     const int tid = threadIdx.x;
     const bool act = tid < NN;
     // x stage: thread = (j2, j3); y stage: thread = (k1, j3); z stage: thread = (k1, k2)
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
         for (int i = 0; i < N; ++i) X[i] = Y[i] = acc[i] = 0.0;
         for (int j = j0; j < j1e; ++j) {
             const int sb = src[j], c = code[j], fl = flags[j];
-            const CTab1 Ax = tabx + (long long)(c & 15) * NN;
+            const CTab1 Ax = tabx + (long long)(c & 255) * NN;
             // ---- x: X[k1] += sum_j1 Ax[k1][j1] f[j1, j2=a, j3=bq]
             if (act) {
                 double f[N];
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
             }
             if (fl & 1) {
                 // ---- y: Y[k2] += sum_j2 Ay[k2][j2] X[k1, j2, j3]
-                const CTab1 Ay = taby + (long long)((c >> 4) & 15) * NN;
+                const CTab1 Ay = taby + (long long)((c >> 8) & 255) * NN;
                 __syncthreads();
                 if (act) {
 #pragma unroll
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __rest
                 for (int i = 0; i < N; ++i) X[i] = 0.0;
                 if (fl & 2) {
                     // ---- z: acc[k3] += sum_j3 Az[k3][j3] Y[k1, k2, j3]
-                    const CTab1 Az = tabz + (long long)((c >> 8) & 15) * NN;
+                    const CTab1 Az = tabz + (long long)((c >> 16) & 255) * NN;
                     __syncthreads();
                     if (act) {
 #pragma unroll
@@ -146,7 +147,7 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
                  hipStream_t s, const Multi& mu) {
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
-    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, tgt, lev, off, src, code,
+    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src, code,
                        flags, fvals, t.loctab_ks, kd, od, mu);
     return true;
 }

@@ -387,9 +387,12 @@ void band_pattern(double eps, int n, const double* a, int* ind) {
 // and mk_loctab, src/bfgt/bfgt_volrouts.f:407-660,920-1060).  Output:
 //   tab[slot][k][j] masked to the band, band[slot][k] = (lo, hi) 0-based,
 //   cols[slot] = (first, last) 0-based target columns (first > last: empty).
+int loc_nloctab(int mref) { return (1 << (mref + 1)) * (mref + 3); }
+
 void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim,
-                       double* tab, int* band, int* cols, double* tabx, double* tabxx) {
+                       double* tab, int* band, int* cols, double* tabx, double* tabxx, int mref) {
     constexpr int nq = 50;
+    const int nlt = loc_nloctab(mref), nslot = 2 * nlt + 1;
     std::vector<double> tn(n), tw(n), tu(n * n), xq(nq), wq(nq), pv(nq * n), p(n + 1);
     poly_exps(ipoly, n, tn.data(), tw.data(), tu.data());
     gauss_legendre(nq, xq.data(), wq.data());
@@ -400,17 +403,33 @@ void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim
     }
     const size_t nn = static_cast<size_t>(n) * n;
     // full (unmasked) tables for pot/grad/hess kinds, Fortran layout (m=row, j=col)
-    std::vector<double> full(nn * kLocSlots * 3, 0.0);
-    auto F = [&](int kind, int k) { return &full[(static_cast<size_t>(kind) * kLocSlots + (k + 6)) * nn]; };
+    std::vector<double> full(nn * nslot * 3, 0.0);
+    auto F = [&](int kind, int k) { return &full[(static_cast<size_t>(kind) * nslot + (k + nlt)) * nn]; };
     std::vector<double> raw(nn * 3), wexp(static_cast<size_t>(nq) * n);
     double fi[100], fx[100], fxx[100];
-    // geometric cases: k = -1,-3 (source 2x target), 0,-4 (same size),
-    // -2,-6 (source half the target); +k by mirror symmetry
-    const int ks[6] = {-1, -3, 0, -4, -2, -6};
-    const double scales[6] = {2.0, 2.0, 1.0, 1.0, 0.5, 0.5};
-    for (int c = 0; c < 6; ++c) {
+    // geometric cases (mk_loctab_all, bfgt_volrouts.f:565-580): source /
+    // target size ratio 2^i0 for i0 = m0 .. -1 (m0 = mref + 1), target
+    // centres k / 2^m0 source half-widths apart, k <= 0 up to -nloctab; +k
+    // by mirror symmetry.  mref = 0 gives k = -1, -3 (ratio 2), 0, -4 (1),
+    // -2, -6 (1/2).
+    const int m0 = mref + 1;
+    std::vector<int> ks;
+    std::vector<double> scales;
+    for (int i0 = m0; i0 >= -1; --i0)
+        for (int j0 = 1; j0 <= (1 << m0); ++j0) {
+            int k;
+            if (i0 > 0) k = -(1 << (m0 - i0)) - (1 << (m0 - i0 + 1)) * (j0 - 1);
+            else if (i0 == 0) k = -(1 << (m0 + 1)) * (j0 - 1);
+            else k = -(1 << m0) - (1 << (m0 + 1)) * (j0 - 1);
+            if (k < -nlt) break;
+            ks.push_back(k);
+            scales.push_back(std::ldexp(1.0, i0));
+        }
+    std::vector<char> made(nslot, 0);
+    for (size_t c = 0; c < ks.size(); ++c) {
         const int k = ks[c];
-        const double scale = scales[c], cen = k / 2.0;
+        made[k + nlt] = made[-k + nlt] = 1;
+        const double scale = scales[c], cen = std::ldexp(static_cast<double>(k), -m0);
         double tg[64];
         for (int j = 0; j < n; ++j) tg[j] = tn[j] / scale + cen;
         const double bs = boxdim * scale;
@@ -486,10 +505,11 @@ void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim
         }
     }
     // bands: union of the pot/grad/hess patterns, as mk_loctab_all forms it
-    for (int k = -6; k <= 6; ++k) {
-        const int slot = k + 6;
+    // (offsets no geometric case produces stay empty)
+    for (int k = -nlt; k <= nlt; ++k) {
+        const int slot = k + nlt;
         int a[2 * 65], b[2 * 65], c[2 * 65], u[2 * 65];
-        if (k == 5 || k == -5) {
+        if (!made[slot]) {
             for (int j = 0; j <= n; ++j) { u[2 * j] = 0; u[2 * j + 1] = -1; }
         } else {
             band_pattern(eps, n, F(0, k), a);
@@ -770,7 +790,7 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
                 const int* cr = &P.loccols[2 * (kLocSlots * tl + slot)];
                 if (cr[0] > cr[1]) empty = true;
                 code |= slot << sh;
-                sh += 4;
+                sh += 8;
             }
             if (empty) return;  // the reference returns early for an empty band
             srcs.push_back(src);

@@ -128,8 +128,12 @@ void make_pw2pgh(int n, int npw, const double* ts, const double* xs, double delt
 // One target level's near-field tables: tab[slot][k][j] (masked to the band),
 // band[slot][k] = (lo, hi) 0-based rows, cols[slot] = (first, last) columns;
 // tabx/tabxx (nullable): the first/second derivative tables, same mask.
+// mref: refinement depth the tables cover (mrefinelev of boxfgt.f:601-607):
+// 0 for the solve (13 slots, offsets -6..6), 3 when the tree is refined
+// afterwards (193 slots, offsets -96..96); slot = offset + loc_nloctab(mref).
 void make_loctab_level(double eps, int ipoly, int n, double delta, double boxdim,
                        double* tab, int* band, int* cols, double* tabx = nullptr,
-                       double* tabxx = nullptr);
+                       double* tabxx = nullptr, int mref = 0);
+int loc_nloctab(int mref);
 
 }  // namespace fgt

@@ -38,6 +38,13 @@ void fgt_host_pw2pot(int ipoly, int n, int npw, const double *ts,
 void fgt_host_loctab(double eps, int ipoly, int n, double delta,
                      double boxdim, double *tab, int *ind);
 
+/* mk_loctab_all with mrefinelev = mref (0 or 3; boxfgt.f:601-607 uses 3
+ * when the tree is refined after the solve): tab(n,n,-m:m), ind(2,n+1,-m:m),
+ * m = 2^(mref+1) (mref+3); offsets no geometric case produces have an empty
+ * band (zero rows, column range 0..-1). */
+void fgt_host_loctab_mref(double eps, int ipoly, int n, double delta,
+                          double boxdim, int mref, double *tab, int *ind);
+
 /* mk_pw2pgh_tables (src/bfgt/bfgt_volrouts.f:339-401): tab_pw2pot,
  * tab_pw2der, tab_pw2dxx, each (npw,n) complex interleaved. */
 void fgt_host_pw2pgh(int ipoly, int n, int npw, const double *ts,

@@ -71,19 +71,29 @@ void refglue_silence(void)
     prini_(&z, &z);
 }
 
-/* Sizes of the tree that vol_tree_build will produce (vol_tree_mem). */
-void refglue_tree_mem(int ndim, int ipoly, int iperiod, double eps,
-                      int norder, int ng, double *dpars, int *nboxes,
-                      int *nlevels, int *ltree)
+/* Sizes of the tree that vol_tree_build will produce (vol_tree_mem); with
+ * ifnewtree = 1 the box capacity is three times the tree's box count
+ * (tree_vol_coeffs_nd.f:432), room for boxfgt's refinement. */
+void refglue_tree_mem_nt(int ndim, int ipoly, int iperiod, double eps,
+                         int norder, int ng, double *dpars, int ifnewtree,
+                         int *nboxes, int *nlevels, int *ltree)
 {
     int ipars[3] = {ndim, ng, iperiod};
     double _Complex zk = 0.0, zpars[1] = {0.0};
     double boxlen = 1.0, eta = 0.0;
-    int iptype = 2, nd = 1, ifnewtree = 0;
+    int iptype = 2, nd = 1;
     refglue_silence();
     vol_tree_mem_(&ndim, &ipoly, &iperiod, &eps, &zk, &boxlen, &norder,
                   &iptype, &eta, (void (*)())refglue_gauss_rhs_, &nd, dpars,
                   zpars, ipars, &ifnewtree, nboxes, nlevels, ltree, g_rintl);
+}
+
+void refglue_tree_mem(int ndim, int ipoly, int iperiod, double eps,
+                      int norder, int ng, double *dpars, int *nboxes,
+                      int *nlevels, int *ltree)
+{
+    refglue_tree_mem_nt(ndim, ipoly, iperiod, eps, norder, ng, dpars, 0,
+                        nboxes, nlevels, ltree);
 }
 
 /* Build the tree whose sizes the previous refglue_tree_mem call returned. */

@@ -65,13 +65,15 @@ def gauss_params(centers, widths, amps) -> np.ndarray:
     return np.ascontiguousarray(np.hstack([centers, widths, amps]).ravel())
 
 
-def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng) -> Tree:
+def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng, ifnewtree=0) -> Tree:
+    """ifnewtree=1: arrays sized for three times the boxes (vol_tree_mem's
+    room for boxfgt's refinement); boxes past the tree's own stay zero."""
     L = lib()
     dp = np.ascontiguousarray(dpars, np.float64)
     nb, nl, lt = C.c_int(), C.c_int(), C.c_int()
-    L.refglue_tree_mem(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod),
-                       C.c_double(eps), C.c_int(norder), C.c_int(ng), _p(dp),
-                       C.byref(nb), C.byref(nl), C.byref(lt))
+    L.refglue_tree_mem_nt(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod),
+                          C.c_double(eps), C.c_int(norder), C.c_int(ng), _p(dp), C.c_int(ifnewtree),
+                          C.byref(nb), C.byref(nl), C.byref(lt))
     nboxes, nlevels, ltree = nb.value, nl.value, lt.value
     npbox = norder ** ndim
     itree = np.zeros(ltree, np.int32)
@@ -88,8 +90,11 @@ def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng) -> Tree:
 
 
 def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh: int = 1, targs=None,
-           ifpghtarg: int = 0):
-    """Run the reference boxfgt (ifnewtree=0, no extra targets).
+           ifpghtarg: int = 0, ifnewtree: int = 0, out_tree=None):
+    """Run the reference boxfgt.
+
+    ifnewtree=1 rewrites the tree (refinement, coarsening, 2:1 rebalance);
+    out_tree (a dict) then receives the rewritten itree, centers and nlevels.
 
     Returns (pot, timeinfo[20]) for ifpgh=1, (pot, grad, hess, timeinfo) for
     ifpgh=2/3 (hess None for ifpgh=2).  pot is shaped like fvals; grad is
@@ -116,16 +121,20 @@ def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh:
     dum = np.zeros(1, np.float64)
     timeinfo = np.zeros(100, np.float64)
     itree = tree.itree.copy()   # the reference may write colleagues back (periodic, npwlevel<=1)
+    centers = tree.centers.copy()
+    nlev = C.c_int(tree.nlevels)
     ints = [C.c_int(v) for v in (nd, tree.ndim)]
     L.boxfgt_(C.byref(ints[0]), C.byref(ints[1]), C.byref(C.c_double(delta)),
               C.byref(C.c_double(eps)), C.byref(C.c_int(tree.ipoly)),
               C.byref(C.c_int(tree.iperiod)), C.byref(C.c_int(tree.norder)),
               C.byref(C.c_int(npbox)), C.byref(C.c_int(tree.nboxes)),
-              C.byref(C.c_int(tree.nlevels)), C.byref(C.c_int(tree.ltree)),
-              _p(itree), _p(tree.iptr), _p(tree.centers), _p(tree.boxsize),
+              C.byref(nlev), C.byref(C.c_int(tree.ltree)),
+              _p(itree), _p(tree.iptr), _p(centers), _p(tree.boxsize),
               _p(f), C.byref(C.c_int(ifpgh)), _p(pot), _p(grad), _p(hess),
-              C.byref(C.c_int(0)), C.byref(C.c_int(nt)), _p(targs),
+              C.byref(C.c_int(ifnewtree)), C.byref(C.c_int(nt)), _p(targs),
               C.byref(C.c_int(ifpghtarg if nt else 0)), _p(pote), _p(grade), _p(hesse), _p(timeinfo))
+    if out_tree is not None:
+        out_tree.update(itree=itree, centers=centers, nlevels=nlev.value)
     if nt and ifpghtarg >= 1:
         return (pot, grad if ifpgh >= 2 else None, hess if ifpgh >= 3 else None, pote,
                 grade if ifpghtarg >= 2 else None, hesse if ifpghtarg >= 3 else None, timeinfo[:20].copy())

@@ -255,11 +255,35 @@ def tables():
     print(f"tables: {len(out)} arrays, {os.path.getsize(path)/1e6:.2f} MB")
 
 
+def tables_mref3():
+    """mk_loctab_all with mrefinelev = 3 (193 offsets per level), the tables
+    boxfgt uses when it refines the tree after the solve."""
+    L = refrun.lib()
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    out = {}
+    for ipoly, n, delta, eps, boxdim in ((0, 8, 1e-3, 1e-6, 0.25), (1, 6, 1e-4, 1e-9, 0.0625),
+                                         (0, 12, 1e-4, 1e-8, 0.125)):
+        key = f"{ipoly}_{n}_{delta:g}_{eps:g}_{boxdim:g}"
+        nt = 96
+        ns = 2 * nt + 1
+        tl = np.zeros(n * n * ns); tlx = np.zeros(n * n * ns); tlxx = np.zeros(n * n * ns)
+        ind = np.zeros(2 * (n + 1) * ns, np.int32)
+        L.mk_loctab_all_(_dp(eps), _ip(ipoly), _ip(n), _ip(50), _dp(delta), _dp(boxdim), _ip(3),
+                         _ip(nt), P(tl), P(tlx), P(tlxx), P(ind))
+        out[f"loctab_{key}"] = tl
+        out[f"locind_{key}"] = ind
+    path = os.path.join(HERE, "tables_mref3.npz")
+    np.savez_compressed(path, **out)
+    print(f"tables_mref3: {len(out)} arrays, {os.path.getsize(path)/1e6:.2f} MB")
+
+
 if __name__ == "__main__":
     refrun.lib().refglue_silence()
     only = sys.argv[1:]
     if not only or "tables" in only:
         tables()
+    if not only or "tables_mref3" in only:
+        tables_mref3()
     for name, spec in CASES.items():
         if not only or name in only:
             boxfgt_case(name, spec)

@@ -245,3 +245,47 @@ def test_corrupt_parent_or_level_is_rejected(what):
     assert rc == 2
     msg = fgt_amd.last_status()[1]
     assert "iparent" in msg or "level" in msg
+
+
+MREF3 = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tables_mref3.npz"))
+
+
+@pytest.mark.parametrize("key", sorted(k[len("loctab_"):] for k in MREF3.files if k.startswith("loctab_")))
+def test_loctab_mrefinelev3_match_reference(key):
+    """The 193-offset near-field tables of a refined tree (mk_loctab_all with
+    mrefinelev = 3, reference values) - bands bit for bit, values to 1e-13."""
+    ipoly, n, delta, eps, boxdim = key.split("_")
+    ipoly, n, delta, eps, boxdim = int(ipoly), int(n), float(delta), float(eps), float(boxdim)
+    ns = 193
+    tab = np.zeros(n * n * ns)
+    ind = np.zeros(2 * (n + 1) * ns, np.int32)
+    fgt_amd.lib().fgt_host_loctab_mref(C.c_double(eps), C.c_int(ipoly), C.c_int(n), C.c_double(delta),
+                                       C.c_double(boxdim), C.c_int(3), _p(tab), _p(ind))
+    tab, ind = tab.reshape(ns, n, n), ind.reshape(ns, n + 1, 2)
+    rtab, rind = MREF3[f"loctab_{key}"].reshape(ns, n, n), MREF3[f"locind_{key}"].reshape(ns, n + 1, 2)
+    made = 0
+    for s in range(ns):
+        if not rind[s].any() and not rtab[s].any():
+            continue   # an offset no geometric case produces (left unset by the reference)
+        made += 1
+        np.testing.assert_array_equal(ind[s], rind[s], err_msg=f"band k={s - 96}")
+        mask = np.zeros((n, n), bool)
+        lo_c, hi_c = rind[s, n]
+        for col in range(n):
+            lo, hi = rind[s, col]
+            if lo_c <= col + 1 <= hi_c and lo >= 1:
+                mask[col, lo - 1:hi] = True
+        scale = max(np.abs(rtab[s]).max(), 1e-300)
+        np.testing.assert_allclose(tab[s][mask], rtab[s][mask], rtol=0, atol=1e-13 * scale)
+        assert np.all(tab[s][~mask] == 0)
+    assert made > 100
+    # the mrefinelev = 0 offsets are the same tables at 8x the slot index
+    t0 = np.zeros(n * n * 13)
+    i0 = np.zeros(2 * (n + 1) * 13, np.int32)
+    fgt_amd.lib().fgt_host_loctab(C.c_double(eps), C.c_int(ipoly), C.c_int(n), C.c_double(delta),
+                                  C.c_double(boxdim), _p(t0), _p(i0))
+    t0, i0 = t0.reshape(13, n, n), i0.reshape(13, n + 1, 2)
+    for k in range(-6, 7):
+        if abs(k) != 5:
+            np.testing.assert_array_equal(t0[k + 6], tab[8 * k + 96])
+            np.testing.assert_array_equal(i0[k + 6], ind[8 * k + 96])
