@@ -94,7 +94,7 @@ def _d(v):
 def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
            ltree, itree, iptr, centers, boxsize, fvals, ifpgh, pot, grad=None,
            hess=None, ifnewtree=0, ntarg=0, targs=None, ifpghtarg=0, pote=None,
-           grade=None, hesse=None, timeinfo=None):
+           grade=None, hesse=None, timeinfo=None, newtree=None):
     """Box Gauss transform, argument for argument as the reference ``boxfgt``.
 
     Arrays use the reference's Fortran memory order; with C-ordered numpy
@@ -106,9 +106,21 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
     xz, yz).  With ifpghtarg >= 1, ``targs[ntarg, ndim]`` and
     ``pote[ntarg(, nd)]``, ``grade[ntarg, ndim(, nd)]``,
     ``hesse[ntarg, nhess(, nd)]`` receive the values at the targets
-    (overwritten).  Raises :class:`FgtError` for unsupported options
-    (ifnewtree = 1) instead of silently returning.
+    (overwritten).  Raises :class:`FgtError` for invalid input instead of
+    silently returning.
+
+    ifnewtree = 1 refines, coarsens and 2:1-rebalances the tree after the
+    solve, as the reference does: ``nboxes`` is then the capacity of the
+    arrays (vol_tree_mem's three times the tree's boxes), ``itree``,
+    ``centers``, ``pot`` (and ``grad``/``hess``) are rewritten in place (so
+    they must be C-contiguous int32 / float64 arrays) and the new level count
+    is stored in ``newtree["nlevels"]`` when a dict is passed.
     """
+    if ifnewtree == 1:
+        for arr, dt, nm in ((itree, np.int32, "itree"), (centers, np.float64, "centers")):
+            if not (isinstance(arr, np.ndarray) and arr.dtype == dt and arr.flags.c_contiguous):
+                raise TypeError(f"with ifnewtree = 1, {nm} must be a C-contiguous {dt.__name__} array "
+                                "(it is rewritten in place)")
     itree = np.ascontiguousarray(itree, np.int32)
     iptr = np.ascontiguousarray(iptr, np.int32)
     centers = np.ascontiguousarray(centers, np.float64)
@@ -135,8 +147,9 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
                 raise TypeError(f"{nm} must be a C-contiguous float64 array of nd*{ncomp}*ntarg values")
     dummy = np.zeros(1, np.float64)
     ti = np.zeros(20, np.float64) if timeinfo is None else timeinfo
+    nl = C.c_int(nlevels)
     rc = lib().fgt_boxfgt(_i(nd), _i(ndim), _d(delta), _d(eps), _i(ipoly), _i(iperiod),
-                          _i(norder), _i(npbox), _i(nboxes), _i(nlevels), _i(ltree),
+                          _i(norder), _i(npbox), _i(nboxes), C.byref(nl), _i(ltree),
                           _p(itree), _p(iptr), _p(centers), _p(boxsize), _p(fvals),
                           _i(ifpgh), _p(pot), _p(dummy if grad is None else grad),
                           _p(dummy if hess is None else hess), _i(ifnewtree), _i(ntarg),
@@ -144,6 +157,8 @@ def boxfgt(nd, ndim, delta, eps, ipoly, iperiod, norder, npbox, nboxes, nlevels,
                           _p(dummy if pote is None else pote), _p(dummy if grade is None else grade),
                           _p(dummy if hesse is None else hesse), _p(ti))
     _check(rc)
+    if newtree is not None:
+        newtree["nlevels"] = nl.value
     return pot
 
 

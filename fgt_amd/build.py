@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfgt_amd.so")
 
-SOURCES = ["plan.cpp", "host_api.cpp", "tree.cpp", "api.cpp", "kernels.hip", "kernels_pw3.hip", "kernels_direct3.hip",
+SOURCES = ["plan.cpp", "host_api.cpp", "tree.cpp", "retree.cpp", "api.cpp", "kernels.hip", "kernels_pw3.hip", "kernels_direct3.hip",
            "kernels_targets.hip", "noalloc.hip"]
 
 HIP_FLAGS = [

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +22,7 @@
 #include "../../include/fgt.h"
 #include "kernels.h"
 #include "plan.h"
+#include "retree.h"
 
 namespace {
 
@@ -68,6 +71,18 @@ struct DBuf {
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            from_arena = o.from_arena;
+            o.p = nullptr;
+            o.bytes = 0;
+            o.from_arena = false;
+        }
+        return *this;
+    }
     ~DBuf() { release(); }
     void release() {
         if (p && !from_arena) (void)hipFree(p);
@@ -657,32 +672,36 @@ struct Comp {
     fgt::OutDesc od;
 };
 
-std::vector<Comp> components(fgt_solver& S, double* pot_override = nullptr) {
-    const fgt::Plan& P = S.plan;
-    const int ndim = P.ndim, nd = P.nd, nhess = ndim * (ndim + 1) / 2;
+// the components of outputs pot (nd, npbox, *), grad (nd, ndim, npbox, *)
+// and hess (nd, nhess, npbox, *) that ifpgh asks for
+std::vector<Comp> components_on(int ndim, int nd, int ifpgh, double* pot, double* grad, double* hess) {
+    const int nhess = ndim * (ndim + 1) / 2;
     auto axes = [&](int r0, int r1, int r2) {
         if (ndim == 3) return fgt::Kinds{r0, r1, r2};
         if (ndim == 2) return fgt::Kinds{r0, 0, r1};
         return fgt::Kinds{0, 0, r0};
     };
     std::vector<Comp> c;
-    if (pot_override) {
-        c.push_back({axes(0, 0, 0), {pot_override, nd, 0}});
-        return c;
-    }
-    c.push_back({axes(0, 0, 0), {S.d_pot.as<double>(), nd, 0}});
-    if (S.ifpgh >= 2)
+    c.push_back({axes(0, 0, 0), {pot, nd, 0}});
+    if (ifpgh >= 2)
         for (int a = 0; a < ndim; ++a)
-            c.push_back({axes(a == 0, a == 1, a == 2), {S.d_grad.as<double>(), nd * ndim, a * nd}});
-    if (S.ifpgh >= 3) {
+            c.push_back({axes(a == 0, a == 1, a == 2), {grad, nd * ndim, a * nd}});
+    if (ifpgh >= 3) {
         static const int h1[1][3] = {{2, 0, 0}};
         static const int h2[3][3] = {{2, 0, 0}, {1, 1, 0}, {0, 2, 0}};
         static const int h3[6][3] = {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
         const int(*h)[3] = ndim == 1 ? h1 : (ndim == 2 ? h2 : h3);
         for (int k = 0; k < nhess; ++k)
-            c.push_back({axes(h[k][0], h[k][1], h[k][2]), {S.d_hess.as<double>(), nd * nhess, k * nd}});
+            c.push_back({axes(h[k][0], h[k][1], h[k][2]), {hess, nd * nhess, k * nd}});
     }
     return c;
+}
+
+std::vector<Comp> components(fgt_solver& S, double* pot_override = nullptr) {
+    const fgt::Plan& P = S.plan;
+    if (pot_override) return components_on(P.ndim, P.nd, 1, pot_override, nullptr, nullptr);
+    return components_on(P.ndim, P.nd, S.ifpgh, S.d_pot.as<double>(), S.d_grad.as<double>(),
+                         S.d_hess.as<double>());
 }
 
 // fork_near: run STEP 6 on the solver's second stream, overlapping STEPs
@@ -1630,6 +1649,344 @@ void upload_host(const DBuf& b, const double* in, const char* what) {
 // arrays confirms it, so a collision can never return another tree's
 // plan).  Only fvals move and STEPs 1-6 run on a hit.  Entries beyond the
 // count or byte limit are dropped least recently used first.
+// ------------------------------------------------------- tree refinement
+// boxfgt with ifnewtree = 1 evaluates the potential on every box its
+// refinement creates (boxfgt.f:1130-1227): the plane-wave part from the
+// parent's local expansion shifted to the new box (gnd_shiftpw_loc +
+// gnd_pw2pgh), the near field from the original leaf's modified list 1 with
+// the mrefinelev = 3 tables (gnd_find_loctab_ind + gnd_tens_prod_to_pghloc).
+// Here each generation of new boxes is one batch on the device: local
+// expansions of the refined boxes are gathered into a workspace (copied
+// from the solve's slots, or split from the parent's slot for leaves below
+// the cutoff level), split into the new boxes with the STEP 4 kernel and
+// evaluated with the STEP 5 kernel; the near field runs through the STEP 6
+// kernels with the 193-offset tables.
+struct Refiner {
+    fgt_solver& S;
+    DBuf d_loctab, d_locband;
+    std::vector<int> loccols;   // [l][slot][2]
+    fgt::DevTables tr;          // the solve's tables with the refinement near-field tables
+    int nlt = 0, nslot = 0;
+    DBuf prev;                  // the previous generation's workspace
+    std::unordered_map<int, int> prev_slot;
+    std::vector<std::vector<int>> list1;   // per original leaf, filled on demand
+    std::vector<char> have_list1;
+
+    explicit Refiner(fgt_solver& s) : S(s) {
+        const fgt::Plan& P = S.plan;
+        const int n = P.n, nl = P.nlevels;
+        nlt = fgt::loc_nloctab(3);
+        nslot = 2 * nlt + 1;
+        const size_t nn = (size_t)n * n, per = nn * nslot;
+        const size_t ks = per * (nl + 1);
+        std::vector<double> tab(3 * ks, 0.0);
+        std::vector<int> band((size_t)2 * n * nslot * (nl + 1), 0);
+        loccols.assign((size_t)2 * nslot * (nl + 1), 0);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int l = 0; l <= nl; ++l)
+            fgt::make_loctab_level(P.eps, P.ipoly, n, P.delta, P.tree.boxsize[l], &tab[per * l],
+                                   &band[(size_t)2 * n * nslot * l], &loccols[(size_t)2 * nslot * l],
+                                   &tab[ks + per * l], &tab[2 * ks + per * l], 3);
+        d_loctab.upload(tab);
+        d_locband.upload(band);
+        tr = S.tabs;
+        tr.loctab = d_loctab.as<double>();
+        tr.locband = d_locband.as<int>();
+        tr.loctab_ks = ks;
+        tr.loc_slots = nslot;
+        list1.resize(P.nboxes);
+        have_list1.assign(P.nboxes, 0);
+    }
+
+    const std::vector<int>& list1_of(int b) {
+        if (!have_list1[b]) {
+            fgt::modified_list1(S.plan, b, list1[b]);
+            have_list1[b] = 1;
+        }
+        return list1[b];
+    }
+
+    // split sign of child c relative to parent p (kernel axes), STEP 4's convention
+    int split_sign(const std::vector<double>& cen, int ndim, int c, int p) const {
+        int k = 0;
+        for (int a = 0; a < ndim; ++a)
+            if (cen[(size_t)ndim * p + a] - cen[(size_t)ndim * c + a] < 0) k |= 1 << a;
+        return map_sign(ndim, k) ^ real_mask(ndim);
+    }
+
+    void eval(const fgt::NewTree& T, const std::vector<fgt::NewBox>& kids, std::vector<double>& np,
+              std::vector<double>& ng, std::vector<double>& nh) {
+        const fgt::Plan& P = S.plan;
+        const fgt::Geom& g = S.geom;
+        const int ndim = P.ndim, nd = P.nd, mc = 1 << ndim;
+        const int nk = (int)kids.size();
+        const int nb0 = P.nboxes;
+        const hipStream_t st = nullptr;
+        // ---- plane waves: local expansions of the refined boxes, then of the new boxes
+        const size_t slot_el = 2 * (size_t)nd * P.nexp;   // double2 per slot
+        std::vector<int> parents;
+        std::unordered_map<int, int> aslot;   // refined box -> workspace slot
+        for (const auto& k : kids)
+            if (P.do_pw && P.ifpwexp[k.origin] && !aslot.count(k.parent)) {
+                aslot[k.parent] = -1;
+                parents.push_back(k.parent);
+            }
+        std::unordered_map<int, int> tmp;     // solve slot of a grandparent -> workspace slot
+        int ns = 0;
+        for (int p : parents)
+            if (p < nb0 && T.level[p] > P.nlevstart && !tmp.count(T.parent[p])) tmp[T.parent[p]] = ns++;
+        for (int p : parents) aslot[p] = ns++;
+        std::vector<int> bslot(nk, -1);
+        for (int i = 0; i < nk; ++i)
+            if (aslot.count(kids[i].parent)) bslot[i] = ns++;
+        DBuf ws;
+        if (ns > 0) ws.alloc(sizeof(double2) * slot_el * ns);
+        double2* W = ws.as<double2>();
+        const double2* X = S.d_expn.as<double2>();
+        const size_t sb = sizeof(double2) * slot_el;
+        for (const auto& kv : tmp) {
+            const int sl = P.slot_of_box[kv.first];
+            if (sl < 0) throw std::logic_error("refinement: parent without a plane-wave slot");
+            hcheck(hipMemcpyAsync(W + slot_el * kv.second, X + slot_el * sl, sb, hipMemcpyDeviceToDevice, st), "copy loc");
+        }
+        // entries per split level code: (source slot, target slot, sign)
+        std::map<int, std::array<std::vector<int>, 3>> splits;
+        for (int p : parents) {
+            const int a = aslot[p];
+            if (p < nb0) {
+                if (T.level[p] < P.nlevstart) throw std::logic_error("refinement: plane-wave leaf above the cutoff level");
+                if (T.level[p] == P.nlevstart) {
+                    const int sl = P.slot_of_box[p];
+                    if (sl < 0) throw std::logic_error("refinement: leaf without a plane-wave slot");
+                    hcheck(hipMemcpyAsync(W + slot_el * a, X + slot_el * sl, sb, hipMemcpyDeviceToDevice, st),
+                           "copy loc");
+                } else {
+                    const int q = T.parent[p];
+                    auto& e = splits[P.nlevels - T.level[q]];
+                    e[0].push_back(tmp[q]);
+                    e[1].push_back(a);
+                    e[2].push_back(split_sign(T.centers, ndim, p, q));
+                }
+            } else {
+                auto it = prev_slot.find(p);
+                if (it == prev_slot.end()) throw std::logic_error("refinement: refined box without its expansion");
+                hcheck(hipMemcpyAsync(W + slot_el * a, prev.as<double2>() + slot_el * it->second, sb,
+                                      hipMemcpyDeviceToDevice, st), "copy loc");
+            }
+        }
+        DBuf d_a, d_b, d_c;
+        auto run_splits = [&]() {
+            for (auto& kv : splits) {
+                const int ne = (int)kv.second[0].size();
+                d_a.upload(kv.second[0]);
+                d_b.upload(kv.second[1]);
+                d_c.upload(kv.second[2]);
+                fgt::launch_split(g, S.tabs, ne, d_a.as<int>(), d_b.as<int>(), d_c.as<int>(), kv.first, W, st);
+                hcheck(hipStreamSynchronize(st), "refinement split");
+            }
+            splits.clear();
+        };
+        run_splits();
+        std::vector<int> pw_box, pw_lev, pw_slot;
+        for (int i = 0; i < nk; ++i) {
+            if (bslot[i] < 0) continue;
+            const int p = kids[i].parent;
+            auto& e = splits[P.nlevels - T.level[p]];
+            e[0].push_back(aslot[p]);
+            e[1].push_back(bslot[i]);
+            e[2].push_back(split_sign(T.centers, ndim, kids[i].id, p));
+            pw_box.push_back(i);
+            pw_lev.push_back(kids[i].level);
+            pw_slot.push_back(bslot[i]);
+        }
+        run_splits();
+        // ---- outputs of the new boxes, local index i
+        const int ifpgh = T.ifpgh;
+        DBuf d_p, d_g, d_h;
+        d_p.alloc(sizeof(double) * T.wp() * nk);
+        hcheck(hipMemsetAsync(d_p.p, 0, d_p.bytes, st), "zero");
+        if (ifpgh >= 2) {
+            d_g.alloc(sizeof(double) * T.wg() * nk);
+            hcheck(hipMemsetAsync(d_g.p, 0, d_g.bytes, st), "zero");
+        }
+        if (ifpgh >= 3) {
+            d_h.alloc(sizeof(double) * T.wh() * nk);
+            hcheck(hipMemsetAsync(d_h.p, 0, d_h.bytes, st), "zero");
+        }
+        const std::vector<Comp> comps = components_on(ndim, nd, ifpgh, d_p.as<double>(), d_g.as<double>(),
+                                                      d_h.as<double>());
+        if (!pw_box.empty()) {
+            DBuf db, dl, dsl;
+            db.upload(pw_box);
+            dl.upload(pw_lev);
+            dsl.upload(pw_slot);
+            for (const Comp& c : comps)
+                fgt::launch_pw2pot(g, S.tabs, (int)pw_box.size(), db.as<int>(), dl.as<int>(), dsl.as<int>(), W, c.kd,
+                                   c.od, st);
+        }
+        // ---- near field: the original leaf's list, tables at the new box's level
+        std::vector<int> dt, dlv, doff{0}, dsrc, dcode, dfl;
+        const double bs0 = P.bs0;
+        for (int i = 0; i < nk; ++i) {
+            const int c = kids[i].id, lc = kids[i].level;
+            std::vector<std::pair<int, int>> ent;   // (mapped code, source)
+            for (int src : list1_of(kids[i].origin)) {
+                // gnd_find_loctab_ind (bfgt_local.f:1624-1660), mrefinelev = 3
+                const double bsq = P.tree.boxsize[P.tree.level[src]] / 32.0;
+                int code = 0;
+                bool empty = false;
+                for (int a = 0; a < ndim; ++a) {
+                    double dx = (T.centers[(size_t)ndim * c + a] - P.tree.centers[(size_t)ndim * src + a]) / bsq;
+                    if (P.iper_eff == 1) {
+                        const double dp = dx - bs0 / bsq, dm = dx + bs0 / bsq;
+                        if (std::fabs(dx) > std::fabs(dp)) dx = dp;
+                        if (std::fabs(dx) > std::fabs(dm)) dx = dm;
+                    }
+                    const int ix = static_cast<int>(dx);
+                    if (ix < -nlt || ix > nlt) throw std::logic_error("refinement: near-field offset beyond the tables");
+                    const int slot = ix + nlt;
+                    const int* cr = &loccols[(size_t)2 * (nslot * lc + slot)];
+                    if (cr[0] > cr[1]) empty = true;
+                    code |= slot << (8 * a);
+                }
+                if (empty) continue;   // the reference returns early for an empty band
+                ent.push_back({map_code(ndim, code, nlt), src});
+            }
+            if (ent.empty()) continue;
+            std::stable_sort(ent.begin(), ent.end(),
+                             [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return (x.first >> 8) < (y.first >> 8); });
+            for (size_t j = 0; j < ent.size(); ++j) {
+                dsrc.push_back(ent[j].second);
+                dcode.push_back(ent[j].first);
+                const bool last = j + 1 == ent.size();
+                int f = 0;
+                if (last || (ent[j].first >> 8) != (ent[j + 1].first >> 8)) f |= 1;
+                if (last || (ent[j].first >> 16) != (ent[j + 1].first >> 16)) f |= 2;
+                dfl.push_back(f);
+            }
+            dt.push_back(i);
+            dlv.push_back(lc);
+            doff.push_back((int)dsrc.size());
+        }
+        if (!dt.empty()) {
+            DBuf a1, a2, a3, a4, a5, a6;
+            a1.upload(dt);
+            a2.upload(dlv);
+            a3.upload(doff);
+            a4.upload(dsrc);
+            a5.upload(dcode);
+            a6.upload(dfl);
+            for (const Comp& c : comps)
+                fgt::launch_direct(g, tr, (int)dt.size(), a1.as<int>(), a2.as<int>(), a3.as<int>(), a4.as<int>(),
+                                   a5.as<int>(), a6.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, st);
+        }
+        hcheck(hipStreamSynchronize(st), "refinement evaluation");
+        hcheck(hipMemcpy(np.data(), d_p.p, d_p.bytes, hipMemcpyDeviceToHost), "download");
+        if (ifpgh >= 2) hcheck(hipMemcpy(ng.data(), d_g.p, d_g.bytes, hipMemcpyDeviceToHost), "download");
+        if (ifpgh >= 3) hcheck(hipMemcpy(nh.data(), d_h.p, d_h.bytes, hipMemcpyDeviceToHost), "download");
+        // the new boxes' expansions seed the next generation
+        prev_slot.clear();
+        for (int i = 0; i < nk; ++i)
+            if (bslot[i] >= 0) prev_slot[kids[i].id] = bslot[i];
+        prev = std::move(ws);
+        (void)mc;
+    }
+};
+
+// boxfgt with ifnewtree = 1 after the solve: outputs already accumulated
+// into the caller's arrays.  Outputs formed only for the targets (ifpgh <
+// ifpghtarg) live in zeroed host arrays of the caller's box capacity, as the
+// reference's grad2 / hess2 (boxfgt.f:207-272).  timeinfo(9..12): refine,
+// coarsen + rebalance, 0, targets (seconds).
+void rewrite_tree(fgt_solver& S, double eps, int cap, int ifpgh_caller, int ifpgh_dev, int* itree, const int* iptr,
+                  double* centers, const double* boxsize, int* nlevels, double* pot, double* grad, double* hess,
+                  int nt, const double* targs, int ifpghtarg, double* pote, double* grade, double* hesse,
+                  double* timeinfo) {
+    const fgt::Plan& P = S.plan;
+    const double ta = now_s();
+    fgt::NewTree T;
+    T.ndim = P.ndim;
+    T.mc = 1 << P.ndim;
+    T.mnb = P.ndim == 1 ? 3 : (P.ndim == 2 ? 9 : 27);
+    T.nd = P.nd;
+    T.n = P.n;
+    T.npbox = P.npbox;
+    T.ipoly = P.ipoly;
+    T.iperiod = P.iper_eff;
+    T.ifpgh = ifpgh_dev;
+    T.cap = cap;
+    T.nlevels = P.nlevels;
+    std::vector<double> g2, h2;
+    T.pot = pot;
+    if (ifpgh_dev >= 2) {
+        if (ifpgh_caller >= 2) {
+            T.grad = grad;
+        } else {
+            g2.assign((size_t)cap * P.nd * P.ndim * P.npbox, 0.0);
+            download(S.d_grad, g2.data(), 1, "download grad");
+            T.grad = g2.data();
+        }
+    }
+    if (ifpgh_dev >= 3) {
+        if (ifpgh_caller >= 3) {
+            T.hess = hess;
+        } else {
+            h2.assign((size_t)cap * P.nd * (P.ndim * (P.ndim + 1) / 2) * P.npbox, 0.0);
+            download(S.d_hess, h2.data(), 1, "download hess");
+            T.hess = h2.data();
+        }
+    }
+    fgt::nt_read(T, itree, iptr, centers, boxsize);
+    Refiner R(S);
+    fgt::NewTreeStats st;
+    fgt::nt_rewrite(
+        T, eps, P.ifpwexp,
+        [&](const fgt::NewTree& t, const std::vector<fgt::NewBox>& kids, std::vector<double>& np,
+            std::vector<double>& ng, std::vector<double>& nh) { R.eval(t, kids, np, ng, nh); },
+        &st);
+    fgt::nt_write(T, itree, iptr, centers, nlevels);
+    const double tb = now_s();
+    if (timeinfo) {
+        timeinfo[8] = tb - ta;
+    }
+    if (nt <= 0) return;
+    // STEP 11 on the rewritten tree: a solver shell holding only the tree
+    // and the outputs
+    fgt_solver Q;
+    fgt::Plan& Pq = Q.plan;
+    Pq.nd = P.nd;
+    Pq.ndim = P.ndim;
+    Pq.n = P.n;
+    Pq.ipoly = P.ipoly;
+    Pq.npbox = P.npbox;
+    Pq.nboxes = T.count();
+    Pq.nlevels = T.nlevels;
+    fgt::TreeView& V = Pq.tree;
+    V.ndim = T.ndim;
+    V.nboxes = T.count();
+    V.nlevels = T.nlevels;
+    V.mc = T.mc;
+    V.mnbors = T.mnb;
+    V.lfirst = T.lfirst;
+    V.llast = T.llast;
+    V.level = T.level;
+    V.parent = T.parent;
+    V.nchild = T.nchild;
+    V.child = T.child;
+    V.centers = T.centers;
+    V.boxsize = T.boxsize;
+    Q.ifpgh = ifpghtarg;
+    auto up = [&](DBuf& d, const double* h, size_t per) {
+        d.alloc(sizeof(double) * per * T.count());
+        hcheck(hipMemcpy(d.p, h, d.bytes, hipMemcpyHostToDevice), "upload outputs");
+    };
+    up(Q.d_pot, T.pot, T.wp());
+    if (ifpghtarg >= 2) up(Q.d_grad, T.grad, T.wg());
+    if (ifpghtarg >= 3) up(Q.d_hess, T.hess, T.wh());
+    eval_targets(Q, nt, targs, ifpghtarg, pote, grade, hesse, nullptr);
+    if (timeinfo) timeinfo[11] = now_s() - tb;
+}
+
 constexpr int kDropinCacheEntries = 4;
 constexpr size_t kDropinCacheBytes = size_t(32) << 30;
 
@@ -2389,7 +2746,7 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         if (*ifpghtarg >= 1 && (!ntarg || *ntarg < 0)) throw std::invalid_argument("bad ntarg");
         if (nt > 0 && (!targs || !pote || (*ifpghtarg >= 2 && !grade) || (*ifpghtarg >= 3 && !hesse)))
             throw std::invalid_argument("null target array");
-        if (*ifnewtree != 0) throw std::invalid_argument("ifnewtree = 1 is not supported");
+        if (*ifnewtree != 0 && *ifnewtree != 1) throw std::invalid_argument("ifnewtree must be 0 or 1");
         const double t0 = now_s();
         fgt::PlanInput in{*nd, *ndim, *delta, *eps, *ipoly, *iperiod, *norder, *npbox, *nboxes, *nlevels,
                           *ltree, itree, iptr, centers, boxsize};
@@ -2444,7 +2801,19 @@ int fgt_boxfgt(int* nd, int* ndim, double* delta, double* eps, int* ipoly, int* 
         download(S.d_pot, pot, 1, "download pot");
         if (*ifpgh >= 2) download(S.d_grad, grad, 1, "download grad");
         if (*ifpgh >= 3) download(S.d_hess, hess, 1, "download hess");
-        const double t3 = now_s();
+        double t3 = now_s();
+        if (*ifnewtree == 1) {
+            // STEPs 8-10: refine, coarsen and rebalance the tree around the
+            // computed outputs, then STEP 11 on the rewritten tree
+            rewrite_tree(S, *eps, *nboxes, *ifpgh, ifpgh_dev, itree, iptr, centers, boxsize, nlevels, pot, grad,
+                         hess, nt, targs, *ifpghtarg, pote, grade, hesse, timeinfo);
+            if (timeinfo) {
+                timeinfo[0] = t1 - t0;
+                for (int i = 0; i < 6; ++i) timeinfo[1 + i] = S.stage_ms[i] * 1e-3;
+                timeinfo[7] = t3 - t2;
+            }
+            return;
+        }
         if (nt > 0) {
             // STEP 11 evaluates what the reference holds at that point: the
             // caller's accumulated arrays for the outputs it requested, fresh

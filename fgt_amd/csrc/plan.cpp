@@ -554,6 +554,58 @@ void set_err(std::string* err, const std::string& m) {
 
 }  // namespace
 
+// Modified list 1 of leaf b (gnd_compute_modified_list1,
+// src/common/tree_routs_nd.f:864-1055): the boxes whose near field b takes
+// directly -- leaf colleagues (b itself unless b forms a plane-wave
+// expansion), leaf children of non-leaf colleagues below the cutoff level,
+// and leaf colleagues of the parent that touch b.  Empty for non-leaves and
+// for boxes below the cutoff level.  The relation is symmetric, so the list
+// serves b as a target.
+void modified_list1(const Plan& P, int b, std::vector<int>& out) {
+    const TreeView& T = P.tree;
+    const int ndim = T.ndim, mc = T.mc, mnb = T.mnbors;
+    const int l = T.level[b];
+    out.clear();
+    if (l > std::min(P.npwlevel, T.nlevels)) return;
+    const double bs0 = P.bs0;
+    auto wrapdis = [&](double d) {
+        double dis = std::fabs(d);
+        if (P.iper_eff == 1) dis = std::min(dis, bs0 - dis);
+        return dis;
+    };
+    if (l == 0) {
+        if (T.nchild[0] == 0) out.push_back(0);
+        return;
+    }
+    if (T.nchild[b] != 0 || P.all_pw) return;
+    const int dad = T.parent[b];
+    for (int i = 0; i < T.ncoll[b]; ++i) {
+        const int jb = T.coll[static_cast<size_t>(mnb) * b + i];
+        if (T.nchild[jb] == 0) {
+            if (P.ifpwexp[b] != 1 || jb != b) out.push_back(jb);
+        } else if (l < P.npwlevel) {
+            const double dt = 1.05 * (T.boxsize[l] + T.boxsize[l + 1]) / 2.0;
+            for (int j = 0; j < mc; ++j) {
+                const int kb = T.child[static_cast<size_t>(mc) * jb + j];
+                if (kb < 0 || T.nchild[kb] != 0) continue;
+                bool ok = true;
+                for (int a = 0; a < ndim; ++a)
+                    if (wrapdis(T.centers[ndim * kb + a] - T.centers[ndim * b + a]) >= dt) { ok = false; break; }
+                if (ok) out.push_back(kb);
+            }
+        }
+    }
+    for (int i = 0; i < T.ncoll[dad]; ++i) {
+        const int jb = T.coll[static_cast<size_t>(mnb) * dad + i];
+        if (T.nchild[jb] != 0) continue;
+        const double dt = 1.05 * (T.boxsize[l] + T.boxsize[l - 1]) / 2.0;
+        bool ok = true;
+        for (int a = 0; a < ndim; ++a)
+            if (wrapdis(T.centers[ndim * jb + a] - T.centers[ndim * b + a]) >= dt) { ok = false; break; }
+        if (ok) out.push_back(jb);
+    }
+}
+
 int make_plan(const PlanInput& in, Plan& P, std::string* err) {
     // ---- validation (the reference has none; we return a status instead) ----
     if (in.ndim < 1 || in.ndim > 3) { set_err(err, "ndim must be 1, 2 or 3"); return 2; }
@@ -579,12 +631,12 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
 
     P = Plan();
     P.nd = in.nd; P.ndim = ndim; P.n = n; P.ipoly = in.ipoly; P.iperiod = in.iperiod;
-    P.npbox = in.npbox; P.nboxes = nb; P.nlevels = nl; P.delta = in.delta; P.eps = in.eps;
+    P.npbox = in.npbox; P.nlevels = nl; P.delta = in.delta; P.eps = in.eps;
     P.bs0 = in.boxsize[0];
 
     // ---- tree arrays (1-based Fortran -> 0-based) ----
     TreeView& T = P.tree;
-    T.ndim = ndim; T.nboxes = nb; T.nlevels = nl; T.mc = mc; T.mnbors = mnb;
+    T.ndim = ndim; T.nlevels = nl; T.mc = mc; T.mnbors = mnb;
     const int* it = in.itree;
     T.lfirst.resize(nl + 1); T.llast.resize(nl + 1);
     for (int l = 0; l <= nl; ++l) {
@@ -595,11 +647,19 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
         }
     }
     if (T.lfirst[0] != 0 || T.llast[0] != 0) { set_err(err, "level 0 must hold only box 1"); return 2; }
-    T.level.resize(nb); T.parent.resize(nb); T.nchild.resize(nb);
-    T.child.resize(static_cast<size_t>(mc) * nb); T.ncoll.resize(nb);
-    T.coll.resize(static_cast<size_t>(mnb) * nb);
-    auto bad_id = [&](int v) { return v < -1 || v == 0 || v > nb; };
-    for (int b = 0; b < nb; ++b) {
+    // the tree's boxes are 1..laddr(2,nlevels); nboxes may be larger (room
+    // a caller keeps for refinement, vol_tree_mem with ifnewtree = 1): the
+    // sections of itree are laid out for nboxes, boxes past the tree's own
+    // are not read
+    const int nba = T.llast[nl] + 1;
+    if (nba < 1) { set_err(err, "laddr out of range"); return 2; }
+    P.nboxes = nba;
+    T.nboxes = nba;
+    T.level.resize(nba); T.parent.resize(nba); T.nchild.resize(nba);
+    T.child.resize(static_cast<size_t>(mc) * nba); T.ncoll.resize(nba);
+    T.coll.resize(static_cast<size_t>(mnb) * nba);
+    auto bad_id = [&](int v) { return v < -1 || v == 0 || v > nba; };
+    for (int b = 0; b < nba; ++b) {
         T.level[b] = it[in.iptr[1] - 1 + b];
         T.parent[b] = it[in.iptr[2] - 1 + b] - (it[in.iptr[2] - 1 + b] > 0 ? 1 : 0);
         T.nchild[b] = it[in.iptr[3] - 1 + b];
@@ -611,12 +671,12 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
         for (int j = 0; j < mc; ++j) {
             const int v = it[in.iptr[4] - 1 + mc * b + j];
             if (bad_id(v) && j < T.nchild[b]) { set_err(err, "invalid ichild entry"); return 2; }
-            T.child[static_cast<size_t>(mc) * b + j] = (v > 0 && v <= nb) ? v - 1 : -1;
+            T.child[static_cast<size_t>(mc) * b + j] = (v > 0 && v <= nba) ? v - 1 : -1;
         }
         for (int j = 0; j < mnb; ++j) {
             const int v = it[in.iptr[6] - 1 + mnb * b + j];
-            if (j < T.ncoll[b] && (v < 1 || v > nb)) { set_err(err, "invalid coll entry"); return 2; }
-            T.coll[static_cast<size_t>(mnb) * b + j] = (v > 0 && v <= nb) ? v - 1 : -1;
+            if (j < T.ncoll[b] && (v < 1 || v > nba)) { set_err(err, "invalid coll entry"); return 2; }
+            T.coll[static_cast<size_t>(mnb) * b + j] = (v > 0 && v <= nba) ? v - 1 : -1;
         }
         for (int j = 0; j < T.nchild[b]; ++j)
             if (T.child[static_cast<size_t>(mc) * b + j] < 0) { set_err(err, "missing child id"); return 2; }
@@ -626,19 +686,19 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
     for (int l = 0; l <= nl; ++l)
         for (int b = T.lfirst[l]; b >= 0 && b <= T.llast[l]; ++b)
             if (T.level[b] != l) { set_err(err, "box level disagrees with laddr"); return 2; }
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < nba; ++b) {
         const int l = T.level[b];
         if (b < T.lfirst[l] || b > T.llast[l]) { set_err(err, "box outside its level's laddr range"); return 2; }
         if (l >= 1) {
             const int p = T.parent[b];
-            if (p < 0 || p >= nb || T.level[p] != l - 1) { set_err(err, "invalid iparent entry"); return 2; }
+            if (p < 0 || p >= nba || T.level[p] != l - 1) { set_err(err, "invalid iparent entry"); return 2; }
         }
         for (int j = 0; j < T.nchild[b]; ++j)
             if (T.level[T.child[static_cast<size_t>(mc) * b + j]] != l + 1) {
                 set_err(err, "child not one level below its parent"); return 2;
             }
     }
-    T.centers.assign(in.centers, in.centers + static_cast<size_t>(ndim) * nb);
+    T.centers.assign(in.centers, in.centers + static_cast<size_t>(ndim) * nba);
     T.boxsize.assign(in.boxsize, in.boxsize + nl + 1);
 
     // ---- cutoff level and plane-wave length (boxfgt.f:132-162) ----
@@ -720,7 +780,7 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
 
     // ---- which boxes carry PW expansions (gnd_find_pwexp_boxes,
     //      tree_routs_nd.f:1266-1366) ----
-    P.ifpwexp.assign(nb, 0);
+    P.ifpwexp.assign(nba, 0);
     if (npwlevel <= 0) {
         P.ifpwexp[0] = 1;
         // Every box forms and evaluates its PW expansion on this path, as
@@ -744,7 +804,7 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
     }
 
     // ---- leaves ----
-    for (int b = 0; b < nb; ++b) if (T.nchild[b] == 0) P.nleaves++;
+    for (int b = 0; b < nba; ++b) if (T.nchild[b] == 0) P.nleaves++;
 
     // ---- near-field tables per level ----
     const size_t nn = static_cast<size_t>(n) * n;
@@ -811,38 +871,10 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
             const int b = order_box[q], l = order_lev[q];
             std::vector<int>& srcs = qsrc[q];
             std::vector<int>& codes = qcode[q];
-            {
-                if (l == 0) {
-                    if (T.nchild[0] == 0) add_pair(0, 0, srcs, codes);
-                } else if (T.nchild[b] == 0 && !P.all_pw) {
-                    const int dad = T.parent[b];
-                    for (int i = 0; i < T.ncoll[b]; ++i) {
-                        const int jb = T.coll[static_cast<size_t>(mnb) * b + i];
-                        if (T.nchild[jb] == 0) {
-                            if (P.ifpwexp[b] != 1 || jb != b) add_pair(b, jb, srcs, codes);
-                        } else if (l < npwlevel) {
-                            const double dt = 1.05 * (T.boxsize[l] + T.boxsize[l + 1]) / 2.0;
-                            for (int j = 0; j < mc; ++j) {
-                                const int kb = T.child[static_cast<size_t>(mc) * jb + j];
-                                if (kb < 0 || T.nchild[kb] != 0) continue;
-                                bool ok = true;
-                                for (int a = 0; a < ndim; ++a)
-                                    if (wrapdis(T.centers[ndim * kb + a] - T.centers[ndim * b + a]) >= dt) { ok = false; break; }
-                                if (ok) add_pair(b, kb, srcs, codes);
-                            }
-                        }
-                    }
-                    for (int i = 0; i < T.ncoll[dad]; ++i) {
-                        const int jb = T.coll[static_cast<size_t>(mnb) * dad + i];
-                        if (T.nchild[jb] != 0) continue;
-                        const double dt = 1.05 * (T.boxsize[l] + T.boxsize[l - 1]) / 2.0;
-                        bool ok = true;
-                        for (int a = 0; a < ndim; ++a)
-                            if (wrapdis(T.centers[ndim * jb + a] - T.centers[ndim * b + a]) >= dt) { ok = false; break; }
-                        if (ok) add_pair(b, jb, srcs, codes);
-                    }
-                }
-            }
+            std::vector<int> raw;
+            modified_list1(P, b, raw);
+            for (int jb : raw) add_pair(b, jb, srcs, codes);
+            (void)l;
         }
         for (int q = 0; q < nq; ++q) {
             if (qsrc[q].empty()) continue;
@@ -900,7 +932,7 @@ int make_plan(const PlanInput& in, Plan& P, std::string* err) {
 
     // ---- expansion slots: every box at levels >= nlevstart that forms or
     //      receives an expansion (non-leaf, or PW leaf) ----
-    P.slot_of_box.assign(nb, -1);
+    P.slot_of_box.assign(nba, -1);
     for (int l = nls; l <= nl; ++l)
         for (int b = T.lfirst[l]; b <= T.llast[l]; ++b)
             if (T.nchild[b] > 0 || P.ifpwexp[b]) P.slot_of_box[b] = P.nslots++;

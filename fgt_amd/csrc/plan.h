@@ -107,6 +107,10 @@ struct Plan {
     int64_t ndirect_pairs = 0;
 };
 
+// Modified list 1 of box b (raw, before the empty-band filter of the work
+// lists); needs the plan's tree, cutoff level and plane-wave flags.
+void modified_list1(const Plan& P, int b, std::vector<int>& out);
+
 // Builds the plan.  Returns 0 on success, otherwise a nonzero status with a
 // message in *err.
 int make_plan(const PlanInput& in, Plan& plan, std::string* err);

@@ -29,11 +29,25 @@ extern "C" {
  * targets targs(ndim,ntarg) into pote(nd,ntarg), grade(nd,ndim,ntarg),
  * hesse(nd,nhess,ntarg) (overwritten), from the polynomial expansions of
  * the leaf values, as the reference's STEP 11 (boxfgt.f:1343-1378).
- * Supported: ifpgh = 1..3, ifpghtarg = 0..3, ifnewtree = 0.  Anything else leaves
- * the outputs untouched and records a nonzero status (fgt_last_status); the
- * reference has no error channel (it neither checks nor reports).
+ * ifnewtree = 1 rewrites the tree after the solve, as the reference's STEPs
+ * 8-10 (boxfgt.f:1026-1337): leaves whose potential is under-resolved are
+ * refined (the potential, and grad / hess, of each new box evaluated from its
+ * parent's local expansion and near-field list), sibling leaves whose merged
+ * potential is resolved are coarsened when the refinement added a tenth of
+ * the boxes, and the 2:1 level restriction is restored with interpolated
+ * values; itree, centers, *nlevels and pot (grad, hess) are rewritten and the
+ * targets are evaluated on the new tree.  nboxes is then the capacity of the
+ * arrays (vol_tree_mem with ifnewtree = 1 gives three times the tree's
+ * boxes; the tree's own boxes are 1..laddr(2,nlevels)); a refinement that
+ * needs more returns status 6 with the outputs partly rewritten.  Boxes at
+ * the finest level are not refined again (the reference's level arrays end
+ * there).  Rows past the new box count are left unspecified.
+ * Supported: ifpgh = 1..3, ifpghtarg = 0..3, ifnewtree = 0, 1.  Anything else
+ * leaves the outputs untouched and records a nonzero status
+ * (fgt_last_status); the reference has no error channel (it neither checks
+ * nor reports).
  * timeinfo(1..20) is zeroed; (1) plan+upload, (2..7) STEP 1..6 device
- * seconds, (8) download, (12) extra targets.
+ * seconds, (8) download, (9) STEPs 8-10, (12) extra targets.
  */
 void boxfgt_(int *nd, int *ndim, double *delta, double *eps, int *ipoly,
              int *iperiod, int *norder, int *npbox, int *nboxes, int *nlevels,

@@ -255,6 +255,80 @@ def tables():
     print(f"tables: {len(out)} arrays, {os.path.getsize(path)/1e6:.2f} MB")
 
 
+# boxfgt with ifnewtree = 1 (refine / coarsen / 2:1 rebalance after the
+# solve).  Generate these with OMP_NUM_THREADS=1: the reference's threaded
+# 2:1 rebalance shares one coefficient scratch array between threads
+# (vol_tree_refine_boxes_flag_interp, tree_vol_coeffs_nd.f:3129-3227, gcoefs /
+# hcoefs are not private), so with several threads the gradient and Hessian
+# of the boxes it adds depend on thread timing.  Spec: (ndim, n, ipoly, iperiod, tree_eps, eps, delta, seed, ng, width
+# range, nd, ifpgh, ifpghtarg, ntarg).  Trees are built with room for three
+# times their boxes, as vol_tree_mem does for ifnewtree = 1; tree_eps above
+# eps leaves the potential under-resolved so that boxes get refined.
+NEWTREE_CASES = {
+    "nt2_coarsen":   (2, 8, 0, 0, 1e-3, 1e-6, 1e-2, 7, 3, (1e-3, 1e-2), 1, 1, 0, 0),
+    "nt2_per":       (2, 8, 0, 1, 1e-2, 1e-4, 1e-3, 7, 3, (1e-3, 1e-2), 1, 1, 0, 0),
+    "nt2_gen2_n6":   (2, 6, 0, 0, 1e-6, 1e-9, 1e-3, 7, 3, (1e-3, 1e-2), 1, 1, 0, 0),
+    "nt3_free":      (3, 8, 0, 0, 1e-2, 1e-4, 1e-3, 7, 3, (1e-3, 1e-2), 1, 1, 0, 0),
+    "nt3_per":       (3, 8, 0, 1, 1e-2, 1e-4, 1e-3, 7, 3, (1e-3, 1e-2), 1, 1, 0, 0),
+    "nt1_per":       (1, 8, 0, 1, 1e-6, 1e-9, 1e-3, 7, 3, (1e-4, 1e-3), 1, 1, 0, 0),
+    "nt2_pgh3":      (2, 8, 0, 1, 1e-2, 1e-4, 1e-3, 7, 3, (1e-3, 1e-2), 1, 3, 0, 0),
+    "nt2_nd2":       (2, 8, 0, 0, 1e-3, 1e-5, 1e-3, 7, 3, (1e-3, 1e-2), 2, 1, 0, 0),
+    "nt2_cheb":      (2, 6, 1, 1, 1e-2, 1e-4, 1e-3, 7, 3, (1e-3, 1e-2), 1, 2, 0, 0),
+    "nt2_tgt":       (2, 12, 0, 1, 1e-3, 1e-6, 1e-3, 7, 3, (1e-3, 1e-2), 1, 1, 2, 300),
+}
+
+
+def newtree_case(name, spec):
+    if os.environ.get("OMP_NUM_THREADS") != "1":
+        raise SystemExit("run the ifnewtree fixtures with OMP_NUM_THREADS=1")
+    ndim, n, ipoly, iper, teps, eps, delta, seed, ng, wr, nd, ifpgh, ifpghtarg, nt = spec
+    c, s, a = make_source(ndim, seed, ng, wr, iper)
+    dp = refrun.gauss_params(c, s, a)
+    tr = refrun.build_tree(ndim, n, ipoly, iper, teps, dp, ng, ifnewtree=1)
+    nb0 = int(tr.itree[2 * tr.nlevels + 1])
+    f = tr.fvals
+    if nd == 2:
+        f = np.stack([tr.fvals, np.cos(3.0 * tr.fvals)], axis=-1)
+    nch = tr.itree[tr.iptr[3] - 1: tr.iptr[3] - 1 + nb0]
+    leaves = np.nonzero(nch == 0)[0].astype(np.int32)
+    targs = None
+    if nt:
+        rng = np.random.default_rng(seed + 2000)
+        targs = rng.uniform(-0.5, 0.5, (nt, ndim))
+    out = {}
+    res = refrun.boxfgt(tr, delta, eps, nd=nd, fvals=f, ifpgh=ifpgh, targs=targs, ifpghtarg=ifpghtarg,
+                        ifnewtree=1, out_tree=out)
+    nl1 = out["nlevels"]
+    nb1 = int(out["itree"][2 * nl1 + 1])
+    extra = {}
+    if nt:
+        pot, grad, hess, pote, grade, hesse, _ = res
+        extra.update(targs=targs, pote=pote)
+        if grade is not None:
+            extra["grade"] = grade
+        if hesse is not None:
+            extra["hesse"] = hesse
+    elif ifpgh == 1:
+        pot, _ = res
+        grad = hess = None
+    else:
+        pot, grad, hess, _ = res
+    if ifpgh >= 2:
+        extra["grad_out"] = grad[:nb1]
+    if ifpgh >= 3:
+        extra["hess_out"] = hess[:nb1]
+    path = os.path.join(HERE, f"boxfgt_{name}.npz")
+    np.savez_compressed(path, ndim=ndim, norder=n, ipoly=ipoly, iperiod=iper, nboxes=tr.nboxes,
+                        nlevels=tr.nlevels, ltree=tr.ltree, itree=tr.itree, iptr=tr.iptr,
+                        centers=tr.centers[:nb0], boxsize=tr.boxsize, leaves=leaves, fvals_leaf=f[leaves],
+                        eps=eps, delta=delta, nd=nd, ifpgh=ifpgh, ifpghtarg=ifpghtarg, tree_eps=teps,
+                        gauss_c=c, gauss_s=s, gauss_a=a, ifnewtree=1,
+                        itree_out=out["itree"], centers_out=out["centers"][:nb1], nlevels_out=nl1,
+                        pot_out=pot[:nb1], **extra)
+    print(f"{name}: capacity={tr.nboxes} boxes {nb0} -> {nb1}, nlevels {tr.nlevels} -> {nl1}, "
+          f"size={os.path.getsize(path)/1e6:.2f} MB")
+
+
 def tables_mref3():
     """mk_loctab_all with mrefinelev = 3 (193 offsets per level), the tables
     boxfgt uses when it refines the tree after the solve."""
@@ -293,3 +367,6 @@ if __name__ == "__main__":
     for name, spec in TARG_CASES.items():
         if not only or name in only:
             targ_case(name, spec)
+    for name, spec in NEWTREE_CASES.items():
+        if not only or name in only or "newtree" in only:
+            newtree_case(name, spec)

@@ -36,13 +36,13 @@ def write_input(c, path):
 
 
 @needs_driver
-def test_fortran_caller_gets_unsupported_status(tmp_path):
-    """ifnewtree = 1 through the module: status 2, message, outputs untouched
-    (no kernel runs, so this needs no GPU)."""
+def test_fortran_caller_gets_invalid_option_status(tmp_path):
+    """An invalid ifnewtree (2) through the module: status 2, message,
+    outputs untouched (no kernel runs, so this needs no GPU)."""
     c = load_case("c1_small")
     inp = tmp_path / "in.bin"
     write_input(c, inp)
-    r = subprocess.run([FORTRAN_DRIVER, str(inp), str(tmp_path / "out.bin"), "1"], capture_output=True,
+    r = subprocess.run([FORTRAN_DRIVER, str(inp), str(tmp_path / "out.bin"), "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.split("\n")

@@ -199,7 +199,7 @@ def _call_boxfgt(c, ifpgh=1, ifpghtarg=0, ifnewtree=0, ipoly=None, norder=None):
 
 
 @pytest.mark.parametrize("kw", [dict(ifpgh=4), dict(ifpgh=0), dict(ifpghtarg=5), dict(ifpghtarg=-1),
-                                dict(ifnewtree=1), dict(ipoly=3), dict(norder=40)])
+                                dict(ifnewtree=2), dict(ipoly=3), dict(norder=40)])
 def test_unsupported_options_report_status_and_leave_outputs(kw):
     c = load_case("c1_small")
     rc, pot = _call_boxfgt(c, **kw)
