@@ -61,7 +61,7 @@ def stage_model(st, n, nd=1):
     STEP 5 (k_pw2pot3, per PW leaf, 3-D): 4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
     flops (the x/y stages complex x complex, the last stage real output);
     bytes: read one local expansion (16 nexp) + read/write pot (16 n^3).
-    STEP 3 (k_m2l27, per cutoff-level box): 8 nexp flops per list entry; compulsory
+    STEP 3 (k_m2l_brick / k_m2l27, per cutoff-level box): 8 nexp flops per list entry; compulsory
     bytes: every cutoff-level mp read once and every loc written once,
     2 x 16 nexp per box (re-reads of a shared source are not algorithmic).
     """
@@ -72,7 +72,8 @@ def stage_model(st, n, nd=1):
     return {
         "pw2pot": {"kernel": "k_pw2pot3", "bound": "mfma", "flops": L * nd * f15,
                    "bytes": L * nd * (16 * nexp + 16 * n ** 3)},
-        "m2l": {"kernel": "k_m2l27", "bound": "hbm", "flops": 8 * nd * nexp * st["n_m2l_pairs"],
+        "m2l": {"kernel": "k_m2l_brick" if st.get("m2l_bricks", -1) > 0 else "k_m2l27", "bound": "hbm",
+                "flops": 8 * nd * nexp * st["n_m2l_pairs"],
                 "bytes": 2 * 16 * nd * nexp * st["n_m2l_targets"]},
     }
 
@@ -193,11 +194,12 @@ def batch_roofline(probs, prof):
             byts += 2 * elem * ps["nexp"] * ps["n_m2l_targets"]
         ms = c["stage_ms"]
         rows.append({"norder": n, "prec": prec, "npw_cap": c["npw_cap"], "members": c["members"],
-                     "group": c["group"], "xs1": c["xs1"], "xs5": c["xs5"],
+                     "group": c["group"], "xs1": c["xs1"], "xs5": c["xs5"], "m2l_brick": c["m2l_brick"],
+                     "m2l_ept": c["m2l_ept"],
                      "stage_ms": [round(v, 4) for v in ms], "step15_gflop": round(flops / 1e9, 3),
                      "step3_gb": round(byts / 1e9, 3)})
         for stg, kern, kind, work in ((4, "k_pw2pot3", "flops", flops), (0, "k_p2pwF3+k_p2pw3", "flops", flops),
-                                      (2, "k_m2l27", "bytes", byts)):
+                                      (2, "k_m2l_brick" if c["m2l_brick"] else "k_m2l27", "bytes", byts)):
             if ms[stg] > 0 and work > 0 and (best is None or ms[stg] > best[0]):
                 best = (ms[stg], kern, kind, work, n, prec, c["npw_cap"])
     if best is None:
@@ -599,6 +601,29 @@ def c2_leg(args):
     return leg, roofline
 
 
+def copy_bandwidth(reps=10):
+    """Device-to-device copy of 1 GiB (torch copy_, HIP events around the
+    timed copies): the practical HBM ceiling, counting bytes read + written."""
+    import torch
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * a.numel() * 8
+    del a, b
+    return {"bytes_per_copy": moved, "ms": round(ms, 4), "gbs": round(moved / ms / 1e6, 1),
+            "frac_of_peak": round(moved / ms / 1e6 / PEAK_HBM_GBS, 3),
+            "what": "1 GiB device-to-device copy (read + write bytes), mean of 10"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -658,6 +683,8 @@ def main():
                   "budget_gb": round((200 << 30) / 1e9, 1),
                   "rates": [serving(pool, r, args.serve_seconds, 4, 3) for r in rates]}
         log(f"served: {served}")
+    hbm_copy = copy_bandwidth()
+    log(f"hbm copy: {hbm_copy}")
     cpu = None
     if not args.no_cpu_baseline and probs is not None:
         cpu = cpu_baseline(probs)
@@ -697,6 +724,7 @@ def main():
         "mixed_precision_sweep": sweep,
         "batched": batched,
         "served": served,
+        "hbm_copy": hbm_copy,
     }
     print(json.dumps(out), flush=True)
 

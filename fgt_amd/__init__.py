@@ -38,7 +38,7 @@ class _SolverStats(C.Structure):
                 ("ndirect_pairs", C.c_longlong), ("nout", C.c_longlong),
                 ("expansion_bytes", C.c_longlong), ("stage_ms", C.c_double * 8),
                 ("n_m2l_targets", C.c_longlong), ("n_m2l_pairs", C.c_longlong),
-                ("n_parent_child", C.c_longlong)]
+                ("n_parent_child", C.c_longlong), ("m2l_bricks", C.c_longlong)]
 
 
 class _PlanInfo(C.Structure):
@@ -363,10 +363,10 @@ class Batch:
         member count and the chosen group / x-stage variants."""
         out = []
         for k in range(lib().fgt_batch_nclasses(self._h)):
-            v = (C.c_int * 10)()
+            v = (C.c_int * 12)()
             _check(lib().fgt_batch_class_info(self._h, C.c_int(k), v))
             out.append(dict(zip(("norder", "nd", "prec", "npw_cap", "members", "group", "xs1", "xs5",
-                                 "step5_tasks", "near_targets"), list(v))))
+                                 "step5_tasks", "near_targets", "m2l_ept", "m2l_brick"), list(v))))
         return out
 
     def get_pot(self, i):

@@ -132,6 +132,8 @@ struct fgt_solver {
     DBuf d_m2l_tgt, d_m2l_off, d_m2l_src, d_m2l_idx;
     DBuf d_m2l_toff, d_m2l_poff, d_m2l_soff, d_m2l_so;   // STEP 3 entry streams per group
     int n_m2l_groups = 0;
+    DBuf d_mb_toff, d_mb_gent, d_mb_soff, d_mb_code;     // STEP 3 over sibling bricks
+    int n_m2l_bricks = -1;                                // -1: bricks not used
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
@@ -248,6 +250,86 @@ int m2l_streams(const fgt::Plan& P, int G, std::vector<long long>& toff, std::ve
     return ng;
 }
 
+// STEP 3 entry streams over sibling bricks (3-D, cutoff level below the
+// root): the targets that share a parent form one brick; each source of
+// their lists (and each target's own expansion) becomes one entry at its
+// position in the 4x4x4 neighbourhood, with the mask of the siblings whose
+// list holds it.  Per brick: toff[8] loc element offsets (-1 = no target),
+// entries ordered by position, padded with zero-slot entries.  Returns the
+// brick count, or -1 when the lists do not fit that shape (a position
+// holding two different sources), in which case the grouped streams serve.
+int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int>& gent,
+               std::vector<long long>& soff, std::vector<int>& code) {
+    if (P.ndim != 3 || P.nlevstart < 1 || !P.do_pw) return -1;
+    const fgt::TreeView& T = P.tree;
+    const long long se = 2LL * P.nd * P.nexp;
+    const int nt = (int)P.m2l_tgt.size();
+    std::vector<int> box_of_slot(P.nslots, -1);
+    for (int b = 0; b < T.nboxes; ++b)
+        if (P.slot_of_box[b] >= 0) box_of_slot[P.slot_of_box[b]] = b;
+    std::unordered_map<int, int> brick_of_parent;
+    std::vector<std::array<int, 8>> tg;
+    for (int tk = 0; tk < nt; ++tk) {
+        const int b = box_of_slot[P.m2l_tgt[tk]];
+        if (b < 0) return -1;
+        const int par = T.parent[b];
+        if (par < 0) return -1;
+        int tp = 0;
+        for (int a = 0; a < 3; ++a)
+            if (T.centers[3 * b + a] > T.centers[3 * par + a]) tp |= 1 << a;
+        auto it = brick_of_parent.find(par);
+        if (it == brick_of_parent.end()) {
+            it = brick_of_parent.emplace(par, (int)tg.size()).first;
+            std::array<int, 8> none;
+            none.fill(-1);
+            tg.push_back(none);
+        }
+        if (tg[it->second][tp] >= 0) return -1;
+        tg[it->second][tp] = tk;
+    }
+    const int nb = (int)tg.size();
+    toff.assign((size_t)nb * 8, -1);
+    gent.assign(nb + 1, 0);
+    soff.clear();
+    code.clear();
+    for (int k = 0; k < nb; ++k) {
+        gent[k] = (int)soff.size();
+        int src_at[64], mask_at[64];
+        for (int i = 0; i < 64; ++i) src_at[i] = -1, mask_at[i] = 0;
+        for (int tp = 0; tp < 8; ++tp) {
+            const int tk = tg[k][tp];
+            if (tk < 0) continue;
+            toff[(size_t)8 * k + tp] = se * P.m2l_tgt[tk];
+            // a source at offset j (target minus source, per axis, in boxes)
+            // sits at brick position tp - j, stored + 1
+            auto add = [&](int slot, int jx, int jy, int jz) {
+                const int px = (tp & 1) - jx + 1, py = ((tp >> 1) & 1) - jy + 1, pz = ((tp >> 2) & 1) - jz + 1;
+                const int pos = px + 4 * py + 16 * pz;
+                if ((src_at[pos] >= 0 && src_at[pos] != slot) || ((mask_at[pos] >> tp) & 1)) return false;
+                src_at[pos] = slot;
+                mask_at[pos] |= 1 << tp;
+                return true;
+            };
+            if (!add(P.m2l_tgt[tk], 0, 0, 0)) return -1;
+            for (int j = P.m2l_off[tk]; j < P.m2l_off[tk + 1]; ++j) {
+                const int idx = P.m2l_idx[j];
+                if (!add(P.m2l_src[j], idx % 3 - 1, (idx / 3) % 3 - 1, idx / 9 - 1)) return -1;
+            }
+        }
+        for (int pos = 0; pos < 64; ++pos)
+            if (mask_at[pos]) {
+                soff.push_back(se * src_at[pos]);
+                code.push_back(pos | mask_at[pos] << 6);
+            }
+        while ((soff.size() - gent[k]) % fgt::kM2lBatch) {
+            soff.push_back(se * P.nslots);   // the zero slot, applied to no target
+            code.push_back(0);
+        }
+    }
+    gent[nb] = (int)soff.size();
+    return nb;
+}
+
 // On the 3-D fast path a leaf below the first plane-wave level never holds
 // an expansion of its own: STEP 1 writes its parent's (fused with STEP 2)
 // and STEP 5 evaluates its parent's through phased tables.  Only non-leaf
@@ -286,6 +368,13 @@ void compact_slots(fgt::Plan& P) {
 // 0 FP64; 1 FP32 where eps >= 1e-6; 2 BF16 hi/lo where eps >= 1e-3; 3 the
 // cheapest one eps allows.  ok: the problem is on the 3-D fast path and
 // forms plane-wave expansions.
+// STEP 3 elements per thread: FGT_M2L_EPT when set (2 only where allowed),
+// else 1 (two per thread measured slower for FP64 at norder 16: 1.05 vs
+// 0.91 ms); batch classes time both in the autotuner
+int m2l_ept_default(bool ept2_ok) {
+    return fgt::m2l_ept_env() == 2 && ept2_ok ? 2 : 1;
+}
+
 int mode_precision(int mode, double eps, bool ok) {
     if (!ok || mode == 0) return 0;
     if (mode == 1) return eps >= kEpsFp32 ? 1 : 0;
@@ -610,6 +699,17 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true, f
         S.h.m2l_soff = soff;
         S.d_m2l_so.upload(code);
         S.h.m2l_code = code;
+        if (fgt::m2l_brick_env() != 0) {
+            std::vector<long long> bt, bs;
+            std::vector<int> bg, bc;
+            S.n_m2l_bricks = m2l_bricks(P, bt, bg, bs, bc);
+            if (S.n_m2l_bricks > 0) {
+                S.d_mb_toff.upload(bt);
+                S.d_mb_gent.upload(bg);
+                S.d_mb_soff.upload(bs);
+                S.d_mb_code.upload(bc);
+            }
+        }
     }
 
     // near-field targets, longest list first (one block per target: the
@@ -801,9 +901,14 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
             mark(2);
         }
         // STEP 3
-        fgt::launch_m2l27(g, t, S.n_m2l_groups, fgt::m2l_group(), 2LL * P.nd * P.nexp * P.nslots,
-                          S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(),
-                          S.d_m2l_so.as<int>(), ex, st);
+        if (S.n_m2l_bricks > 0)
+            fgt::launch_m2l_brick(g, t, S.n_m2l_bricks, 2LL * P.nd * P.nexp * P.nslots, S.d_mb_toff.as<long long>(),
+                                  S.d_mb_gent.as<int>(), S.d_mb_soff.as<long long>(), S.d_mb_code.as<int>(), ex, st);
+        else
+            fgt::launch_m2l27(g, t, S.n_m2l_groups, fgt::m2l_group(), 2LL * P.nd * P.nexp * P.nslots,
+                              S.d_m2l_toff.as<long long>(), S.d_m2l_poff.as<int>(), S.d_m2l_soff.as<long long>(),
+                              S.d_m2l_so.as<int>(), ex, st, fgt::Multi{},
+                              m2l_ept_default((((long long)P.nd * P.nexp) & 1) == 0));
         mark(3);
         // STEP 4: coarse to fine (fast 3-D path: only into non-leaf children;
         // leaves read their parent's expansion in STEP 5 through phased tables)
@@ -983,6 +1088,9 @@ struct BClass {
     std::vector<int> members;
     DBuf pp;
     int group = fgt::kM2lGroup;
+    int m2l_ept = 1;          // STEP 3 elements per thread
+    int m2l_brick = 0;        // STEP 3 over sibling bricks (every member allows them)
+    bool m2l_ept2_ok = false; // every member's nd * nexp is even
     DBuf fz_lev, fz_slot, fz_kids, fz_ksg, fz_prob;
     int n_fz = 0;
     DBuf s1_box, s1_lev, s1_slot, s1_prob;
@@ -1004,7 +1112,7 @@ struct BClass {
 // precision, plane-wave bucket) -> (STEP 3 targets per block, STEP 1 and
 // STEP 5 x-stage variants).
 struct Tuned {
-    int group, xs1, xs5;
+    int group, xs1, xs5, m2l_ept, m2l_brick;
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, Tuned> g_tuned;
@@ -1049,12 +1157,17 @@ bool class_member(const fgt_solver& S) {
 void class_m2l(fgt_batch& B, BClass& C) {
     std::vector<long long> toff, soff;
     std::vector<int> gent(1, 0), code, prob;
+    for (int q = 0; q < (int)C.members.size() && C.m2l_brick; ++q) {
+        const fgt_solver& S = *B.solvers[C.members[q]];
+        if (S.plan.do_pw && S.n_m2l_bricks <= 0) C.m2l_brick = 0;   // a member without bricks
+    }
     for (int q = 0; q < (int)C.members.size(); ++q) {
         const fgt_solver& S = *B.solvers[C.members[q]];
         if (!S.plan.do_pw) continue;
         std::vector<long long> to, so;
         std::vector<int> ge, co;
-        const int ng = m2l_streams(S.plan, C.group, to, ge, so, co);
+        const int ng = C.m2l_brick ? m2l_bricks(S.plan, to, ge, so, co)
+                                   : m2l_streams(S.plan, C.group, to, ge, so, co);
         const int base = (int)soff.size();
         toff.insert(toff.end(), to.begin(), to.end());
         for (int gi = 0; gi < ng; ++gi) {
@@ -1110,6 +1223,11 @@ void build_classes(fgt_batch& B) {
         C.t.p2pw = C.t.p2pwS = C.t.pw2pot = C.t.pw2potT = C.t.pw2potS = C.t.wms = C.t.wm2l = nullptr;
         C.t.loctab = nullptr;
         C.t.pw2potS_ks = C.t.loctab_ks = 0;   // the value tables (kind 0) only
+        C.m2l_ept2_ok = true;
+        for (int i : C.members)
+            if (((long long)B.solvers[i]->plan.nd * B.solvers[i]->plan.nexp) & 1) C.m2l_ept2_ok = false;
+        C.m2l_ept = m2l_ept_default(C.m2l_ept2_ok);
+        C.m2l_brick = fgt::m2l_brick_env() != 0 ? 1 : 0;   // class_m2l drops it where a member cannot
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tuned.find(std::make_tuple(C.n, C.nd, C.prec, C.npw_cap));
@@ -1117,6 +1235,8 @@ void build_classes(fgt_batch& B) {
                 C.group = it->second.group;
                 C.t.xs1 = it->second.xs1;
                 C.t.xs5 = it->second.xs5;
+                if (it->second.m2l_ept == 1 || C.m2l_ept2_ok) C.m2l_ept = it->second.m2l_ept;
+                if (fgt::m2l_brick_env() < 0) C.m2l_brick = it->second.m2l_brick;
             }
         }
         std::vector<fgt::ProbPtrs> pp;
@@ -1304,8 +1424,13 @@ void class_step2(const BClass& C, hipStream_t st) {
     }
 }
 void class_step3(const BClass& C, hipStream_t st) {
-    fgt::launch_m2l27(C.g, C.t, C.n_m2l, C.group, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
-                      C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob));
+    if (C.m2l_brick)
+        fgt::launch_m2l_brick(C.g, C.t, C.n_m2l, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
+                              C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob));
+    else
+        fgt::launch_m2l27(C.g, C.t, C.n_m2l, C.group, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
+                          C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob),
+                          C.m2l_ept);
 }
 void class_step4(const BClass& C, hipStream_t st) {
     for (size_t ph = 0; ph + 1 < C.pcs_phase.size(); ++ph) {
@@ -1421,15 +1546,33 @@ void batch_autotune(fgt_batch& B, int reps) {
         BClass& C = *cp;
         if (C.n_fz + C.n_s1 + C.n_m2l + C.n_s5 == 0) continue;
         if (C.n_m2l > 0) {
-            int best = C.group;
+            int best = C.group, best_e = C.m2l_ept, best_b = 0;
             float bt = 1e30f;
-            for (int G : {2, 3, 4, 6, 8}) {
-                C.group = G;
+            // sibling bricks (when every member allows them), then the groups
+            if (fgt::m2l_brick_env() != 0) {
+                C.m2l_brick = 1;
                 class_m2l(B, C);
-                const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
-                if (ms < bt) bt = ms, best = G;
+                if (C.m2l_brick) {
+                    bt = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
+                    best_b = 1;
+                }
+            }
+            C.m2l_brick = 0;
+            for (int E : {1, 2}) {
+                if (fgt::m2l_brick_env() == 1 && best_b) break;   // bricks forced
+                if (E == 2 && !C.m2l_ept2_ok) continue;
+                if (fgt::m2l_ept_env() && E != fgt::m2l_ept_env()) continue;
+                C.m2l_ept = E;
+                for (int G : {2, 3, 4, 6, 8}) {
+                    C.group = G;
+                    class_m2l(B, C);
+                    const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
+                    if (ms < bt) bt = ms, best = G, best_e = E, best_b = 0;
+                }
             }
             C.group = best;
+            C.m2l_ept = best_e;
+            C.m2l_brick = best_b;
             class_m2l(B, C);
         }
         {
@@ -1456,7 +1599,7 @@ void batch_autotune(fgt_batch& B, int reps) {
             C.t.xs1 = best;
         }
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[std::make_tuple(C.n, C.nd, C.prec, C.npw_cap)] = Tuned{C.group, C.t.xs1, C.t.xs5};
+        g_tuned[std::make_tuple(C.n, C.nd, C.prec, C.npw_cap)] = Tuned{C.group, C.t.xs1, C.t.xs5, C.m2l_ept, C.m2l_brick};
     }
 }
 
@@ -1542,6 +1685,7 @@ int fgt_solver_stats_get(const fgt_solver* s, fgt_solver_stats* o) {
     o->n_m2l_targets = (long long)P.m2l_tgt.size();
     o->n_m2l_pairs = (long long)P.m2l_src.size();
     o->n_parent_child = (long long)P.pc_child.size();
+    o->m2l_bricks = s->n_m2l_bricks > 0 ? s->n_m2l_bricks : -1;
     return 0;
 }
 
@@ -2494,8 +2638,8 @@ int fgt_batch_class_info(const fgt_batch* b, int k, int* info) {
         if (!b || !info) throw std::invalid_argument("null argument");
         if (k < 0 || k >= (int)b->classes.size()) throw std::invalid_argument("no such class");
         const BClass& C = *b->classes[k];
-        const int v[10] = {C.n, C.nd, C.prec, C.npw_cap, (int)C.members.size(), C.group, C.t.xs1, C.t.xs5,
-                           C.n_s5, C.n_dir};
+        const int v[12] = {C.n, C.nd, C.prec, C.npw_cap, (int)C.members.size(), C.group, C.t.xs1, C.t.xs5,
+                           C.n_s5, C.n_dir, C.m2l_ept, C.m2l_brick};
         std::memcpy(info, v, sizeof v);
     });
 }

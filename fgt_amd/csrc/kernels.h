@@ -209,9 +209,23 @@ constexpr int kM2lBatch = 4;
 // longest entry stream of a group: kM2lGroupMax targets x (own + 26 list
 // entries), padded
 constexpr int kM2lMaxEntries = kM2lGroupMax * 27 + kM2lBatch;
+// ept: elements per thread, 1 or 2 (2 needs an even nd * nexp in every
+// problem of the launch)
 void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
                   const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
-                  hipStream_t s, const Multi& mu = Multi{});
+                  hipStream_t s, const Multi& mu = Multi{}, int ept = 1);
+// FGT_M2L_EPT (1 or 2) forces the elements per thread of STEP 3; 0 = policy
+int m2l_ept_env();
+// STEP 3 over sibling bricks (3-D): per brick 8 loc element offsets (-1 =
+// none), an entry stream of source mp offsets and codes (position 6 bits |
+// target mask << 6), padded to a multiple of kM2lBatch
+constexpr int kM2lBrickMax = 64 + kM2lBatch;
+void launch_m2l_brick(const Geom& g, const DevTables& t, int nbrick, long long zero_off, const long long* toff,
+                      const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s,
+                      const Multi& mu = Multi{});
+// FGT_M2L_BRICK: 0 keeps STEP 3 on the target groups, 1 on sibling bricks
+// where a problem allows them; unset (-1) = policy
+int m2l_brick_env();
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,
                   hipStream_t s, const Multi& mu = Multi{});

@@ -231,7 +231,33 @@ constexpr unsigned kM2lRow = (0u << 0) | (3u << 3) | (1u << 6) | (2u << 9) | (4u
 // conjugated at oxy = 0, 1, 2 (oy = 0) and 3 ((ox, oy) = (0, 1))
 constexpr unsigned kM2lConj = 0x0fu;
 
-template <typename ET>
+// EPT elements per thread (1 or 2, consecutive): with 2 a source load is
+// one 16-byte (FP32) or two 16-byte (FP64) accesses per lane, so a wave
+// keeps twice the bytes in flight per entry; the problem's nd * nexp must
+// then be even (every slot offset and element pair stays aligned and inside
+// its slot).
+template <typename ET, int EPT> struct M2lVec;
+template <> struct M2lVec<float2, 1> {
+    __device__ static void load(const float2* p, double2* v) { v[0] = ldE(*p); }
+};
+template <> struct M2lVec<double2, 1> {
+    __device__ static void load(const double2* p, double2* v) { v[0] = *p; }
+};
+template <> struct M2lVec<float2, 2> {
+    __device__ static void load(const float2* p, double2* v) {
+        const float4 w = *reinterpret_cast<const float4*>(p);
+        v[0] = make_double2(w.x, w.y);
+        v[1] = make_double2(w.z, w.w);
+    }
+};
+template <> struct M2lVec<double2, 2> {
+    __device__ static void load(const double2* p, double2* v) {
+        v[0] = p[0];
+        v[1] = p[1];
+    }
+};
+
+template <typename ET, int EPT>
 __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngroup, int group, long long zero_off,
                                                   const long long* __restrict__ toff,
                                                   const int* __restrict__ gent,
@@ -250,28 +276,35 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
         zero_off = q.zero_off;
     }
     const long long tot = (long long)g.nd * g.nexp;
+    // the grid covers the class's largest problem: a block past this
+    // problem's elements has nothing to do (uniform exit, before any barrier)
+    if ((long long)chunk * blockDim.x * EPT >= tot) return;
     const int npw = t.npw;
-    long long e = (long long)chunk * blockDim.x + threadIdx.x;
+    long long e = ((long long)chunk * blockDim.x + threadIdx.x) * EPT;
     const bool ok = e < tot;
-    if (!ok) e = tot - 1;   // valid duplicate, never stored
-    const int r = (int)(e % g.nexp);
-    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    // w_x w_y of the element in LDS (one column per thread, read at a
-    // wave-uniform row), w_z in registers picked by a uniform select.
-    // Rows: 0 a b, 1 conj(a) b, 2 a, 3 b, 4 one; per (ox, oy) the row is
-    // kM2lRow and the conjugation kM2lConj (a = w_x(2), b = w_y(2)).
-    __shared__ double2 s_fxy[5][kBlock];
+    if (!ok) e = tot - EPT;   // valid duplicate, never stored
+    // w_x w_y of each element in LDS (one column per thread and element,
+    // read at a wave-uniform row), w_z in registers picked by a uniform
+    // select.  Rows: 0 a b, 1 conj(a) b, 2 a, 3 b; row 4 (one) is a
+    // constant.  Per (ox, oy) the row is kM2lRow and the conjugation
+    // kM2lConj (a = w_x(2), b = w_y(2)).
+    __shared__ double2 s_fxy[4][EPT * kBlock];
     const int tid = threadIdx.x;
-    {
+    double2 fz0[EPT], fz1[EPT], fz2[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int r = (int)((e + q) % g.nexp);
+        const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
         const double2 a = g.px > 1 ? t.wm2l[2 * npw + k1] : make_double2(1.0, 0.0);
         const double2 b = g.py > 1 ? t.wm2l[2 * npw + k2] : make_double2(1.0, 0.0);
-        s_fxy[0][tid] = cmul(a, b);
-        s_fxy[1][tid] = cmul(conjd(a), b);
-        s_fxy[2][tid] = a;
-        s_fxy[3][tid] = b;
-        s_fxy[4][tid] = make_double2(1.0, 0.0);
+        s_fxy[0][q * kBlock + tid] = cmul(a, b);
+        s_fxy[1][q * kBlock + tid] = cmul(conjd(a), b);
+        s_fxy[2][q * kBlock + tid] = a;
+        s_fxy[3][q * kBlock + tid] = b;
+        fz0[q] = t.wm2l[k3];
+        fz1[q] = t.wm2l[npw + k3];
+        fz2[q] = t.wm2l[2 * npw + k3];
     }
-    const double2 fz0 = t.wm2l[k3], fz1 = t.wm2l[npw + k3], fz2 = t.wm2l[2 * npw + k3];
     const ET* __restrict__ eb = expn + e;
     // the group's entry stream, staged in LDS: read back at a wave-uniform
     // address (broadcast), it keeps the vector-memory counter for the source
@@ -287,24 +320,33 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     const int j0 = 0, j1 = n;
     const long long* __restrict__ to = toff + (size_t)grp * group;
     int ti = 0;
-    double2 acc = make_double2(0.0, 0.0);
-    double2 va[kM2lBatch], vb[kM2lBatch], vc[kM2lBatch];
+    double2 acc[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) acc[q] = make_double2(0.0, 0.0);
+    double2 va[kM2lBatch][EPT], vb[kM2lBatch][EPT], vc[kM2lBatch][EPT];
 #define M2L_LOAD(v, j)                                                                   \
-    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[s_soff[(j) + u]]);
+    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) M2lVec<ET, EPT>::load(eb + s_soff[(j) + u], v[u]);
     // code = o | last-entry-of-its-target << 5
 #define M2L_APPLY(v, j)                                                                  \
     _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
         const int code = __builtin_amdgcn_readfirstlane(s_code[(j) + u]);                \
         const int o = code & 31, oz = o / 9, oxy = o - 9 * oz;                           \
-        const double2 zf = oz == 0 ? fz0 : (oz == 1 ? fz1 : fz2);                        \
-        double2 fxy = s_fxy[(kM2lRow >> (3 * oxy)) & 7][tid];                            \
-        fxy.y = __longlong_as_double(__double_as_longlong(fxy.y) ^                       \
-                                     ((long long)((kM2lConj >> oxy) & 1) << 63));        \
-        acc = cfma(cmul(v[u], fxy), zf, acc);                                            \
+        const int row = (kM2lRow >> (3 * oxy)) & 7;                                      \
+        const long long cj = (long long)((kM2lConj >> oxy) & 1) << 63;                   \
+        _Pragma("unroll") for (int q = 0; q < EPT; ++q) {                                \
+            const double2 zf = oz == 0 ? fz0[q] : (oz == 1 ? fz1[q] : fz2[q]);           \
+            double2 fxy = s_fxy[row & 3][q * kBlock + tid];                              \
+            if (row == 4) fxy = make_double2(1.0, 0.0);                                  \
+            fxy.y = __longlong_as_double(__double_as_longlong(fxy.y) ^ cj);              \
+            acc[q] = cfma(cmul(v[u][q], fxy), zf, acc[q]);                               \
+        }                                                                                \
         if (code >> 5) {                                                                 \
-            if (ok) stE(expn + to[ti] + tot + e, acc);                                   \
+            if (ok) {                                                                    \
+                _Pragma("unroll") for (int q = 0; q < EPT; ++q)                          \
+                    stE(expn + to[ti] + tot + e + q, acc[q]);                            \
+            }                                                                            \
             ++ti;                                                                        \
-            acc = make_double2(0.0, 0.0);                                                \
+            _Pragma("unroll") for (int q = 0; q < EPT; ++q) acc[q] = make_double2(0.0, 0.0); \
         }                                                                                \
     }
     // The batch two ahead is always loaded before the current one is
@@ -325,6 +367,111 @@ __global__ void __launch_bounds__(kBlock) k_m2l27(Geom g, DevTables t, int ngrou
     }
 #undef M2L_APPLY
 #undef M2L_LOAD
+}
+
+// STEP 3 over sibling bricks (3-D): a block owns one element range and the
+// (up to) eight cutoff-level children of one parent, whose M2L sources lie
+// in the 4x4x4 boxes around them.  Each source is loaded once per brick and
+// applied to every sibling whose list holds it: an entry's code carries the
+// source position p (2 bits per axis, box offset + 1 from the brick's
+// first child) and an 8-bit target mask; for target t at brick position
+// tp the offset index is o = sum_a (tp_a + 2 - p_a) 3^a, the same phase
+// selection as k_m2l27.  Every target's own expansion is the entry at its
+// own position (o = 13, unit phase).  toff: 8 loc offsets per brick (-1 =
+// no target there); entries padded with zero-slot entries of mask 0.
+template <typename ET>
+__global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int nbrick, long long zero_off,
+                                                      const long long* __restrict__ toff,
+                                                      const int* __restrict__ gent,
+                                                      const long long* __restrict__ soff,
+                                                      const int* __restrict__ scode,
+                                                      ET* __restrict__ expn_a, Multi mu) {
+    const int lin = xcd_task(blockIdx.x, gridDim.x);
+    const int brk = lin % nbrick, chunk = lin / nbrick;
+    ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[brk]];
+        g = q.g;
+        t.npw = q.g.npw;
+        expn = static_cast<ET*>(q.expn);
+        t.wm2l = q.wm2l;
+        zero_off = q.zero_off;
+    }
+    const long long tot = (long long)g.nd * g.nexp;
+    if ((long long)chunk * blockDim.x >= tot) return;   // past this problem's elements
+    const int npw = t.npw;
+    long long e = (long long)chunk * blockDim.x + threadIdx.x;
+    const bool ok = e < tot;
+    if (!ok) e = tot - 1;   // valid duplicate, never stored
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    __shared__ double2 s_fxy[5][kBlock];
+    const int tid = threadIdx.x;
+    {
+        const double2 a = t.wm2l[2 * npw + k1];
+        const double2 b = t.wm2l[2 * npw + k2];
+        s_fxy[0][tid] = cmul(a, b);
+        s_fxy[1][tid] = cmul(conjd(a), b);
+        s_fxy[2][tid] = a;
+        s_fxy[3][tid] = b;
+        s_fxy[4][tid] = make_double2(1.0, 0.0);
+    }
+    const double2 fz0 = t.wm2l[k3], fz1 = t.wm2l[npw + k3], fz2 = t.wm2l[2 * npw + k3];
+    const ET* __restrict__ eb = expn + e;
+    __shared__ long long s_soff[kM2lBrickMax + 2 * kM2lBatch];
+    __shared__ int s_code[kM2lBrickMax];
+    const int g0 = gent[brk], n = gent[brk + 1] - g0;   // a positive multiple of kM2lBatch
+    for (int i = threadIdx.x; i < n + 2 * kM2lBatch; i += blockDim.x) {
+        s_soff[i] = i < n ? soff[g0 + i] : zero_off;
+        if (i < n) s_code[i] = scode[g0 + i];
+    }
+    __syncthreads();
+    double2 acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = make_double2(0.0, 0.0);
+    double2 va[kM2lBatch], vb[kM2lBatch], vc[kM2lBatch];
+#define M2L_LOAD(v, j)                                                                   \
+    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) v[u] = ldE(eb[s_soff[(j) + u]]);
+#define M2L_APPLY(v, j)                                                                  \
+    _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
+        const int code = __builtin_amdgcn_readfirstlane(s_code[(j) + u]);                \
+        const int px = code & 3, py = (code >> 2) & 3, pz = (code >> 4) & 3;            \
+        const int mask = code >> 6;                                                      \
+        _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                  \
+            if ((mask >> q) & 1) {                                                       \
+                const int ox = (q & 1) + 2 - px, oy = ((q >> 1) & 1) + 2 - py;           \
+                const int oz = ((q >> 2) & 1) + 2 - pz, oxy = ox + 3 * oy;               \
+                const double2 zf = oz == 0 ? fz0 : (oz == 1 ? fz1 : fz2);                \
+                double2 fxy = s_fxy[(kM2lRow >> (3 * oxy)) & 7][tid];                    \
+                fxy.y = __longlong_as_double(__double_as_longlong(fxy.y) ^               \
+                                             ((long long)((kM2lConj >> oxy) & 1) << 63)); \
+                acc[q] = cfma(cmul(v[u], fxy), zf, acc[q]);                              \
+            }                                                                            \
+        }                                                                                \
+    }
+    constexpr int B = kM2lBatch;
+    M2L_LOAD(va, 0);
+    M2L_LOAD(vb, B);
+    for (int j = 0; j < n; j += 3 * B) {
+        M2L_LOAD(vc, j + 2 * B);
+        M2L_APPLY(va, j);
+        if (j + B >= n) break;
+        M2L_LOAD(va, j + 3 * B);
+        M2L_APPLY(vb, j + B);
+        if (j + 2 * B >= n) break;
+        M2L_LOAD(vb, j + 4 * B);
+        M2L_APPLY(vc, j + 2 * B);
+    }
+#undef M2L_APPLY
+#undef M2L_LOAD
+    if (ok) {
+        const long long* __restrict__ to = toff + (size_t)brk * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long long o = to[q];
+            if (o >= 0) stE(expn + o + tot + e, acc[q]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- STEP 4
@@ -664,18 +811,65 @@ int m2l_group() {
     return v;
 }
 
+void launch_m2l_brick(const Geom& g, const DevTables& t, int nbrick, long long zero_off, const long long* toff,
+                      const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s,
+                      const Multi& mu) {
+    if (nbrick <= 0) return;
+    if (g.ndim != 3) throw std::runtime_error("k_m2l_brick: 3-D only");
+    const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * nbrick;
+    if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l_brick: grid too large");
+    if (t.expn_f32)
+        hipLaunchKernelGGL(k_m2l_brick<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, nbrick, zero_off,
+                           toff, gent, soff, scode, reinterpret_cast<float2*>(expn), mu);
+    else
+        hipLaunchKernelGGL(k_m2l_brick<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, nbrick, zero_off,
+                           toff, gent, soff, scode, expn, mu);
+    check(hipGetLastError(), "k_m2l_brick launch");
+}
+
+int m2l_brick_env() {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_M2L_BRICK");
+        return e ? std::atoi(e) : -1;
+    }();
+    return v;
+}
+
+int m2l_ept_env() {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_M2L_EPT");
+        const int x = e ? std::atoi(e) : 0;
+        return x == 1 || x == 2 ? x : 0;
+    }();
+    return v;
+}
+
 void launch_m2l27(const Geom& g, const DevTables& t, int ngroup, int group, long long zero_off,
                   const long long* toff, const int* gent, const long long* soff, const int* scode, double2* expn,
-                  hipStream_t s, const Multi& mu) {
+                  hipStream_t s, const Multi& mu, int ept) {
     if (ngroup <= 0) return;
-    const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * ngroup;
+    if (ept != 1 && ept != 2) throw std::runtime_error("k_m2l27: elements per thread must be 1 or 2");
+    const long long tot = (long long)g.nd * g.nexp;
+    if (ept == 2 && !mu.pp && (tot & 1)) throw std::runtime_error("k_m2l27: two elements per thread need even nd*nexp");
+    const long long nblk = (long long)blocks_for((tot + ept - 1) / ept) * ngroup;
     if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l27: grid too large");
-    if (t.expn_f32)
-        hipLaunchKernelGGL(k_m2l27<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
-                           soff, scode, reinterpret_cast<float2*>(expn), mu);
-    else
-        hipLaunchKernelGGL(k_m2l27<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff, gent,
-                           soff, scode, expn, mu);
+    const dim3 grid((unsigned)nblk);
+    if (t.expn_f32) {
+        float2* x = reinterpret_cast<float2*>(expn);
+        if (ept == 2)
+            hipLaunchKernelGGL((k_m2l27<float2, 2>), grid, dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff,
+                               gent, soff, scode, x, mu);
+        else
+            hipLaunchKernelGGL((k_m2l27<float2, 1>), grid, dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff,
+                               gent, soff, scode, x, mu);
+    } else {
+        if (ept == 2)
+            hipLaunchKernelGGL((k_m2l27<double2, 2>), grid, dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff,
+                               gent, soff, scode, expn, mu);
+        else
+            hipLaunchKernelGGL((k_m2l27<double2, 1>), grid, dim3(kBlock), 0, s, g, t, ngroup, group, zero_off, toff,
+                               gent, soff, scode, expn, mu);
+    }
     check(hipGetLastError(), "k_m2l27 launch");
 }
 

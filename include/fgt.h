@@ -119,6 +119,7 @@ typedef struct {
     double stage_ms[8];   /* last execute: STEP 1..6 in [0..5] (when timed) */
     long long n_m2l_targets, n_m2l_pairs;   /* cutoff-level boxes receiving a loc; list entries */
     long long n_parent_child;                /* child->parent merge edges */
+    long long m2l_bricks;                    /* STEP 3 sibling bricks, -1 when it runs on target groups */
 } fgt_solver_stats;
 
 int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly,
@@ -216,9 +217,10 @@ int fgt_batch_set_graph(fgt_batch *b, int on);
  * keeps the fastest; the choices are remembered for the process per class
  * key and applied to later batches. */
 int fgt_batch_autotune(fgt_batch *b, int reps);
-/* number of shape classes; info[10] of class k: norder, nd, precision,
+/* number of shape classes; info[12] of class k: norder, nd, precision,
  * plane-wave cap, members, STEP 3 group, STEP 1 / STEP 5 variant (-1 =
- * default policy), STEP 5 tasks, near-field targets */
+ * default policy), STEP 5 tasks, near-field targets, STEP 3 elements per
+ * thread, STEP 3 over sibling bricks (1) or target groups (0) */
 int fgt_batch_nclasses(const fgt_batch *b);
 int fgt_batch_class_info(const fgt_batch *b, int k, int *info);
 /* batch problem indices of class k (info[4] of them) */
