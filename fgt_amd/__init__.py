@@ -391,7 +391,7 @@ class Batch:
             pass
 
 
-from .tree import Tree, TreeTooLarge, vol_tree_build  # noqa: E402  (host tree builder)
+from .tree import Tree, TreeTooLarge, vol_tree_build, vol_tree_build_fun  # noqa: E402  (host tree builder)
 
-__all__ = ["boxfgt", "Solver", "Batch", "FgtError", "plan_summary", "Tree", "TreeTooLarge", "vol_tree_build", "lib",
+__all__ = ["boxfgt", "Solver", "Batch", "FgtError", "plan_summary", "Tree", "TreeTooLarge", "vol_tree_build", "vol_tree_build_fun", "lib",
            "last_status"]

@@ -17,6 +17,7 @@
 //     (computecoll, src/common/tree_routs_nd.f:106-242).
 // One pass suffices: the running norm seen by each level is the same value
 // the reference's second pass reads back from its first pass.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -33,6 +34,17 @@ struct Builder {
     double eps = 1e-6, boxlen = 1.0;
     std::vector<double> gauss;  // ng rows of (c[ndim], s[ndim], a)
     int ng = 0;
+    // a caller's source instead of the Gaussians: fun(nd, xyz, dpars, zpars,
+    // ipars, f) at one point, nd values (the reference's `fun` convention)
+    fgt_source_fn fun = nullptr;
+    double* dpars = nullptr;
+    void* zpars = nullptr;
+    int* ipars = nullptr;
+    int nd = 1;          // densities (values per node)
+    int iptype = 2;      // refinement norm: 0 max, 1 L1, 2 L2
+    double eta = 0.0;    // tail error scaled by (box size / 2)^eta
+    double zk = 0.0;     // real part of the Helmholtz parameter: boxes with zk * size > 5 always refine
+    int cap = 0;         // box capacity of the output arrays (0: the tree's own count)
     // tree (0-based box ids)
     std::vector<int> level, parent, nchild, child, ncoll, coll, iflag;
     std::vector<double> centers, fvals, boxsize;
@@ -81,7 +93,7 @@ struct Builder {
         for (int k = 0; k < ndim; ++k) centers.push_back(cen[k]);
         iflag.push_back(0);
         bsize_of.push_back(bs);
-        fvals.resize(static_cast<size_t>(b + 1) * npbox);
+        fvals.resize(static_cast<size_t>(b + 1) * npbox * nd);
         return b;
     }
 
@@ -92,6 +104,10 @@ struct Builder {
     std::vector<double> bsize_of;
     int evaluated = 0;
     void flush() {
+        if (fun) {
+            flush_fun();
+            return;
+        }
         const int n = nb();
         const int stride = 2 * ndim + 1;
         int nimg = 1;
@@ -147,6 +163,25 @@ struct Builder {
         evaluated = n;
     }
 
+    // the caller's source at every node of the boxes created since the last
+    // call (xyz = centre + node * box size, as vol_tree_refine_boxes forms
+    // it); boxes in parallel, so fun must be safe to call concurrently (the
+    // reference calls it from OpenMP loops too)
+    void flush_fun() {
+        const int n = nb();
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int b = evaluated; b < n; ++b) {
+            const double bs = bsize_of[b];
+            double xyz[3];
+            int ndv = nd;
+            for (int l = 0; l < npbox; ++l) {
+                for (int k = 0; k < ndim; ++k) xyz[k] = centers[static_cast<size_t>(ndim) * b + k] + grid[ndim * l + k] * bs;
+                fun(&ndv, xyz, dpars, zpars, ipars, &fvals[(static_cast<size_t>(b) * npbox + l) * nd]);
+            }
+        }
+        evaluated = n;
+    }
+
     // children of b at level lev+1 (box size bs); sign of axis k for child j
     // is + when bit k of j is set
     void refine(int b, double bs) {
@@ -161,39 +196,62 @@ struct Builder {
         }
     }
 
-    // sum over tensor nodes of f^2 w (scaled) -- the box's squared L2 mass
-    double box_l2sq(int b, double rsc) {
+    // the box's part of the running source norm (update_rints,
+    // tree_vol_coeffs_nd.f:949-1092): iptype 2 sum f^2 w rsc (squared L2
+    // mass), 1 sum |f| w rsc, 0 the largest value (signed, as the reference
+    // takes maxval)
+    double box_norm(int b, double rsc) {
         flush();
-        double s = 0;
-        for (int l = 0; l < npbox; ++l) {
-            const double f = fvals[static_cast<size_t>(b) * npbox + l];
-            s += f * f * wts[l] * rsc;
+        const double* v = &fvals[static_cast<size_t>(b) * npbox * nd];
+        if (iptype == 0) {
+            double m = v[0];
+            for (size_t i = 1; i < static_cast<size_t>(npbox) * nd; ++i) m = std::max(m, v[i]);
+            return m;
         }
+        double s = 0;
+        for (int l = 0; l < npbox; ++l)
+            for (int q = 0; q < nd; ++q) {
+                const double f = v[static_cast<size_t>(l) * nd + q];
+                s += (iptype == 2 ? f * f : std::fabs(f)) * wts[l] * rsc;
+            }
         return s;
     }
 
-    // L2 norm of the tail coefficients of box b (values already evaluated)
+    // tail error of box b (values already evaluated): the tensor
+    // coefficients of each density, masked to the tail, in the iptype norm
+    // (fun_err, tree_vol_coeffs_nd.f:860-943), largest over the densities
     double tail_error_flushed(int b) const {
         const int n = norder;
-        std::vector<double> a(fvals.begin() + static_cast<size_t>(b) * npbox,
-                              fvals.begin() + static_cast<size_t>(b + 1) * npbox);
-        std::vector<double> t(npbox);
-        int stride = 1;
-        for (int ax = 0; ax < ndim; ++ax) {
-            for (int i = 0; i < npbox; ++i) {
-                const int ia = (i / stride) % n;
-                const int base = i - ia * stride;
-                double s = 0;
-                for (int j = 0; j < n; ++j) s += umat[ia + j * n] * a[base + j * stride];
-                t[i] = s;
+        double err = 0;
+        std::vector<double> a(npbox), t(npbox);
+        for (int q = 0; q < nd; ++q) {
+            for (int i = 0; i < npbox; ++i) a[i] = fvals[(static_cast<size_t>(b) * npbox + i) * nd + q];
+            int stride = 1;
+            for (int ax = 0; ax < ndim; ++ax) {
+                for (int i = 0; i < npbox; ++i) {
+                    const int ia = (i / stride) % n;
+                    const int base = i - ia * stride;
+                    double s = 0;
+                    for (int j = 0; j < n; ++j) s += umat[ia + j * n] * a[base + j * stride];
+                    t[i] = s;
+                }
+                a.swap(t);
+                stride *= n;
             }
-            a.swap(t);
-            stride *= n;
+            double e = 0;
+            for (int i = 0; i < npbox; ++i) {
+                if (iptype == 0) {
+                    if (rmask[i] > 0.5 && e < std::fabs(a[i])) e = std::fabs(a[i]);
+                } else if (iptype == 1) {
+                    e += std::fabs(a[i]) * rmask[i];
+                } else {
+                    e += a[i] * a[i] * rmask[i];
+                }
+            }
+            if (iptype == 2) e = std::sqrt(e);
+            err = std::max(err, e);
         }
-        double e = 0;
-        for (int i = 0; i < npbox; ++i)
-            if (rmask[i] > 0.5) e += a[i] * a[i];
-        return std::sqrt(e);
+        return err;
     }
 
     double wrap(double dis) const {
@@ -276,7 +334,7 @@ struct Builder {
         perm_i(child, mc);
         perm_i(iflag, 1);
         perm_d(centers, ndim);
-        perm_d(fvals, npbox);
+        perm_d(fvals, npbox * nd);
         perm_d(bsize_of, 1);
         for (auto& p : parent)
             if (p >= 0) p = newid[p];
@@ -419,41 +477,61 @@ struct Builder {
             for (int k = ndim - 2; k >= 0; --k) wt *= w[ik[k]];
             wts[l] = wt;
         }
-        // tail mask: sum_k (p_k+1)^2 >= m^2, m = n (n-1 in 1-D)
+        // tail mask: sum_k (p_k+1)^e >= m^e, e = max(iptype, 1), m = n (n-1
+        // in 1-D); rsum = the mask's count in the iptype norm
         rmask.assign(npbox, 0.0);
         const int morder = (ndim == 1) ? n - 1 : n;
+        const int pe = std::max(iptype, 1);
+        auto ipw = [](int v, int e) { int r = 1; while (e-- > 0) r *= v; return r; };
         double cnt = 0;
         for (int l = 0; l < npbox; ++l) {
             int r = l, i1 = 0;
             for (int k = 0; k < ndim; ++k) {
                 const int p = r % n;
                 r /= n;
-                i1 += (p + 1) * (p + 1);
+                i1 += ipw(p + 1, pe);
             }
-            if (i1 >= morder * morder) {
+            if (i1 >= ipw(morder, pe)) {
                 rmask[l] = 1.0;
                 cnt += 1;
             }
         }
-        rsum = std::sqrt(cnt);
+        rsum = iptype == 2 ? std::sqrt(cnt) : (iptype == 0 ? 1.0 : cnt);
 
         boxsize.assign(1, boxlen);
         double cen0[3] = {0, 0, 0};
         add_box(0, -1, cen0, boxlen);
         lfirst.assign(1, 0);
         llast.assign(1, 0);
-        std::vector<double> rintbs(1, box_l2sq(0, boxlen * boxlen / mc));
-        double rint = std::sqrt(rintbs[0]);
+        // the root's norm takes |f| for iptype 0 (its children take maxval)
+        std::vector<double> rintbs(1, 0.0);
+        if (iptype == 0) {
+            flush();
+            for (size_t i = 0; i < static_cast<size_t>(npbox) * nd; ++i)
+                rintbs[0] = std::max(rintbs[0], std::fabs(fvals[i]));
+        } else {
+            rintbs[0] = box_norm(0, boxlen * boxlen / mc);
+        }
+        double rint = iptype == 2 ? std::sqrt(rintbs[0]) : rintbs[0];
         const int nlmax = 200;
         int lev = 0;
         for (lev = 0; lev < nlmax - 1; ++lev) {
-            const double rsc = std::sqrt(1.0 / std::pow(boxsize[0], ndim)) * rint;
+            double rsc = 1.0;
+            if (iptype == 2) rsc = std::sqrt(1.0 / std::pow(boxsize[0], ndim));
+            if (iptype == 1) rsc = 1.0 / std::pow(boxsize[0], ndim);
+            rsc *= rint;
+            const double rscale2 = std::pow(boxsize[lev] / 2, eta);
             const int f = lfirst[lev], l = llast[lev];
             std::vector<char> ref(l - f + 1, 0);
             flush();
-            // boxes of one level are tested independently
+            if (zk * boxsize[lev] > 5) {
+                std::fill(ref.begin(), ref.end(), 1);
+            } else {
+                // boxes of one level are tested independently
 #pragma omp parallel for schedule(dynamic, 16)
-            for (int b = f; b <= l; ++b) ref[b - f] = tail_error_flushed(b) / rsum > eps * rsc ? 1 : 0;
+                for (int b = f; b <= l; ++b)
+                    ref[b - f] = (tail_error_flushed(b) * rscale2) / rsum > eps * rsc ? 1 : 0;
+            }
             bool any = false;
             for (char c : ref) any = any || c;
             if (!any) break;
@@ -463,20 +541,31 @@ struct Builder {
                 if (ref[b - f]) refine(b, boxsize[lev + 1]);
             llast.push_back(nb() - 1);
             if (max_leaves > 0 && nb() - (nb() - 1) / mc > max_leaves) throw TooLarge{};
-            // running L2 norm: replace refined boxes by their children
+            // running norm: refined boxes replaced by their children
+            // (iptype 0: the largest child maximum so far)
             const double crs = std::pow(boxsize[lev + 1], ndim) / mc;
-            double rsq = rint * rint;
             rintbs.resize(nb(), 0.0);
-            for (int b = f; b <= l; ++b)
-                if (nchild[b] > 0) rsq -= rintbs[b];
-            for (int b = f; b <= l; ++b)
-                if (nchild[b] > 0)
-                    for (int j = 0; j < mc; ++j) {
-                        const int c = child[static_cast<size_t>(mc) * b + j];
-                        rintbs[c] = box_l2sq(c, crs);
-                        rsq += rintbs[c];
-                    }
-            rint = std::sqrt(rsq);
+            if (iptype == 0) {
+                for (int b = f; b <= l; ++b)
+                    if (nchild[b] > 0)
+                        for (int j = 0; j < mc; ++j) {
+                            const int c = child[static_cast<size_t>(mc) * b + j];
+                            rintbs[c] = box_norm(c, crs);
+                            if (rintbs[c] > rint) rint = rintbs[c];
+                        }
+            } else {
+                double acc = iptype == 2 ? rint * rint : rint;
+                for (int b = f; b <= l; ++b)
+                    if (nchild[b] > 0) acc -= rintbs[b];
+                for (int b = f; b <= l; ++b)
+                    if (nchild[b] > 0)
+                        for (int j = 0; j < mc; ++j) {
+                            const int c = child[static_cast<size_t>(mc) * b + j];
+                            rintbs[c] = box_norm(c, crs);
+                            acc += rintbs[c];
+                        }
+                rint = iptype == 2 ? std::sqrt(acc) : acc;
+            }
         }
         nlevels = lev;
         computecoll();
@@ -537,19 +626,70 @@ int fgt_vol_tree_mem(int ndim, int ipoly, int iperiod, double eps, int norder, d
                                    ltree);
 }
 
+int fgt_vol_tree_mem_fun(int ndim, int ipoly, int iperiod, double eps, double zk, double boxlen, int norder,
+                         int iptype, double eta, fgt_source_fn fun, int nd, double* dpars, void* zpars, int* ipars,
+                         int ifnewtree, fgt_tree_builder** out, int* nboxes, int* nlevels, int* ltree) {
+    if (!out || !nboxes || !nlevels || !ltree || !fun) return 2;
+    *out = nullptr;
+    if (ndim < 1 || ndim > 3 || norder < 2 || norder > 16 || nd < 1 || (ipoly != 0 && ipoly != 1) ||
+        (iperiod != 0 && iperiod != 1) || iptype < 0 || iptype > 2 || (ifnewtree != 0 && ifnewtree != 1) ||
+        !(eps > 0) || !(boxlen > 0) || !(eta >= 0))
+        return 2;
+    try {
+        auto tb = std::make_unique<fgt_tree_builder>();
+        Builder& B = tb->b;
+        B.ndim = ndim;
+        B.ipoly = ipoly;
+        B.iperiod = iperiod;
+        B.eps = eps;
+        B.norder = norder;
+        B.boxlen = boxlen;
+        B.fun = fun;
+        B.dpars = dpars;
+        B.zpars = zpars;
+        B.ipars = ipars;
+        B.nd = nd;
+        B.iptype = iptype;
+        B.eta = eta;
+        B.zk = zk;
+        B.build();
+        B.cap = ifnewtree ? 3 * B.nb() : B.nb();
+        *nboxes = B.cap;
+        *nlevels = B.nlevels;
+        *ltree = (4 + B.mc + B.mnb) * B.cap + 2 * (B.nlevels + 1);
+        *out = tb.release();
+        return 0;
+    } catch (const std::exception& e) {
+        g_tree_err = e.what();
+        return 5;
+    }
+}
+
+int fgt_vol_tree_newtree_room(fgt_tree_builder* tb, int* nboxes, int* ltree) {
+    if (!tb || !nboxes || !ltree) return 2;
+    Builder& B = tb->b;
+    B.cap = 3 * B.nb();
+    *nboxes = B.cap;
+    *ltree = (4 + B.mc + B.mnb) * B.cap + 2 * (B.nlevels + 1);
+    return 0;
+}
+
 int fgt_vol_tree_build(fgt_tree_builder* tb, int* itree, int* iptr, double* centers, double* boxsize,
                        double* fvals) {
     if (!tb || !itree || !iptr || !centers || !boxsize || !fvals) return 2;
     const Builder& B = tb->b;
     const int nb = B.nb(), nl = B.nlevels, mc = B.mc, mnb = B.mnb;
+    // sections laid out for the capacity; boxes past the tree's own are zero
+    const int cap = std::max(B.cap, nb);
     iptr[0] = 1;
     iptr[1] = 2 * (nl + 1) + 1;
-    iptr[2] = iptr[1] + nb;
-    iptr[3] = iptr[2] + nb;
-    iptr[4] = iptr[3] + nb;
-    iptr[5] = iptr[4] + mc * nb;
-    iptr[6] = iptr[5] + nb;
-    iptr[7] = iptr[6] + mnb * nb;
+    iptr[2] = iptr[1] + cap;
+    iptr[3] = iptr[2] + cap;
+    iptr[4] = iptr[3] + cap;
+    iptr[5] = iptr[4] + mc * cap;
+    iptr[6] = iptr[5] + cap;
+    iptr[7] = iptr[6] + mnb * cap;
+    std::memset(itree, 0, sizeof(int) * ((size_t)(4 + mc + mnb) * cap + 2 * (nl + 1)));
     for (int l = 0; l <= nl; ++l) {
         itree[2 * l] = B.lfirst[l] + 1;
         itree[2 * l + 1] = B.llast[l] + 1;
@@ -569,8 +709,10 @@ int fgt_vol_tree_build(fgt_tree_builder* tb, int* itree, int* iptr, double* cent
         }
     }
     std::memcpy(centers, B.centers.data(), sizeof(double) * B.centers.size());
+    std::memset(centers + B.centers.size(), 0, sizeof(double) * (size_t)B.ndim * (cap - nb));
     std::memcpy(boxsize, B.boxsize.data(), sizeof(double) * B.boxsize.size());
     std::memcpy(fvals, B.fvals.data(), sizeof(double) * B.fvals.size());
+    std::memset(fvals + B.fvals.size(), 0, sizeof(double) * (size_t)B.npbox * B.nd * (cap - nb));
     delete tb;
     return 0;
 }

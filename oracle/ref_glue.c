@@ -111,3 +111,32 @@ void refglue_tree_build(int ndim, int ipoly, int iperiod, double eps,
                     zpars, ipars, g_rintl, &nboxes, &nlevels, &ltree, itree,
                     iptr, centers, boxsize, fvals);
 }
+
+/* vol_tree_mem / vol_tree_build for a caller's source (the reference's
+ * `fun` convention), refinement norm iptype, eta, nd densities. */
+void refglue_tree_mem_fun(int ndim, int ipoly, int iperiod, double eps,
+                          double zkr, int norder, int iptype, double eta, void (*fun)(),
+                          int nd, double *dpars, int *ipars, int ifnewtree,
+                          int *nboxes, int *nlevels, int *ltree)
+{
+    double _Complex zk = zkr, zpars[1] = {0.0};
+    double boxlen = 1.0;
+    refglue_silence();
+    vol_tree_mem_(&ndim, &ipoly, &iperiod, &eps, &zk, &boxlen, &norder,
+                  &iptype, &eta, fun, &nd, dpars, zpars, ipars, &ifnewtree,
+                  nboxes, nlevels, ltree, g_rintl);
+}
+
+void refglue_tree_build_fun(int ndim, int ipoly, int iperiod, double eps,
+                            double zkr, int norder, int iptype, double eta, void (*fun)(),
+                            int nd, double *dpars, int *ipars, int nboxes,
+                            int nlevels, int ltree, int *itree, int *iptr,
+                            double *centers, double *boxsize, double *fvals)
+{
+    double _Complex zk = zkr, zpars[1] = {0.0};
+    double boxlen = 1.0;
+    vol_tree_build_(&ndim, &ipoly, &iperiod, &eps, &zk, &boxlen, &norder,
+                    &iptype, &eta, fun, &nd, dpars, zpars, ipars, g_rintl,
+                    &nboxes, &nlevels, &ltree, itree, iptr, centers, boxsize,
+                    fvals);
+}

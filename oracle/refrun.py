@@ -89,6 +89,34 @@ def build_tree(ndim, norder, ipoly, iperiod, eps, dpars, ng, ifnewtree=0) -> Tre
                 iptr, centers, boxsize, fvals)
 
 
+def build_tree_fun(ndim, norder, ipoly, iperiod, eps, fun, dpars, ipars, nd=1, iptype=2, eta=0.0, zk=0.0,
+                   ifnewtree=0) -> Tree:
+    """The reference's vol_tree_mem / vol_tree_build for a compiled source
+    function ``fun`` (a ctypes function pointer with the reference's
+    by-reference ``fun(nd, xyz, dpars, zpars, ipars, f)`` convention).
+    fvals is (nboxes, npbox) for nd = 1, else (nboxes, npbox, nd)."""
+    L = lib()
+    dp = np.ascontiguousarray(dpars, np.float64)
+    ip = np.ascontiguousarray(ipars, np.int32)
+    fp = C.cast(fun, C.c_void_p)
+    nb, nl, lt = C.c_int(), C.c_int(), C.c_int()
+    L.refglue_tree_mem_fun(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod), C.c_double(eps), C.c_double(zk),
+                           C.c_int(norder), C.c_int(iptype), C.c_double(eta), fp, C.c_int(nd), _p(dp), _p(ip),
+                           C.c_int(ifnewtree), C.byref(nb), C.byref(nl), C.byref(lt))
+    nboxes, nlevels, ltree = nb.value, nl.value, lt.value
+    npbox = norder ** ndim
+    itree = np.zeros(ltree, np.int32)
+    iptr = np.zeros(8, np.int32)
+    centers = np.zeros((nboxes, ndim), np.float64)
+    boxsize = np.zeros(nlevels + 1, np.float64)
+    fvals = np.zeros((nboxes, npbox) if nd == 1 else (nboxes, npbox, nd), np.float64)
+    L.refglue_tree_build_fun(C.c_int(ndim), C.c_int(ipoly), C.c_int(iperiod), C.c_double(eps), C.c_double(zk),
+                             C.c_int(norder), C.c_int(iptype), C.c_double(eta), fp, C.c_int(nd), _p(dp), _p(ip),
+                             C.c_int(nboxes), C.c_int(nlevels), C.c_int(ltree), _p(itree), _p(iptr), _p(centers),
+                             _p(boxsize), _p(fvals))
+    return Tree(ndim, norder, ipoly, iperiod, nboxes, nlevels, ltree, itree, iptr, centers, boxsize, fvals)
+
+
 def boxfgt(tree: Tree, delta: float, eps: float, nd: int = 1, fvals=None, ifpgh: int = 1, targs=None,
            ifpghtarg: int = 0, ifnewtree: int = 0, out_tree=None):
     """Run the reference boxfgt.

@@ -12,6 +12,10 @@ Fixtures are data only: tree arrays, inputs and the reference's outputs.
   * boxfgt_pgh*/pg*.npz : the same with ifpgh = 2/3 (gradients, Hessians).
   * boxfgt_tgt*.npz : with extra targets (ifpghtarg = 1..3): targets and the
     reference's pote / grade / hesse.
+  * tree_<name>.npz : a tree from vol_tree_mem / vol_tree_build for the
+    compiled source function of tests/csrc/sources.c, for the refinement
+    norms (iptype), box-size powers (eta), nd, zk and ifnewtree stated in
+    TREE_FUN_CASES.
   * tables.npz : the reference's 1-D tables and scalars (fgtterms.f,
     legeexps.f, chebexps.f, besseljs3d.f, bfgt_volrouts.f) for a grid of
     parameters.
@@ -329,6 +333,52 @@ def newtree_case(name, spec):
           f"size={os.path.getsize(path)/1e6:.2f} MB")
 
 
+# name: (ndim, norder, ipoly, iperiod, eps, iptype, eta, nd, zk, ifnewtree, seed, ng, width range)
+TREE_FUN_CASES = {
+    "f2_l2":         (2, 8, 0, 0, 1e-6, 2, 0.0, 1, 0.0, 0, 31, 3, (1e-3, 1e-2)),
+    "f2_l1_eta1":    (2, 8, 0, 0, 1e-6, 1, 1.0, 1, 0.0, 0, 32, 3, (1e-3, 1e-2)),
+    "f2_max_nd2_per": (2, 6, 1, 1, 1e-3, 0, 0.0, 2, 0.0, 0, 33, 3, (1e-3, 1e-2)),
+    "f3_l2_eta2":    (3, 6, 0, 0, 1e-3, 2, 2.0, 1, 0.0, 0, 34, 2, (3e-3, 1e-2)),
+    "f3_max_per_nd2": (3, 4, 1, 1, 5e-2, 0, 0.5, 2, 0.0, 0, 35, 2, (1e-2, 3e-2)),
+    "f1_l1":         (1, 8, 0, 0, 1e-9, 1, 0.0, 1, 0.0, 0, 36, 3, (1e-4, 1e-3)),
+    "f2_zk":         (2, 8, 0, 0, 1e-4, 2, 0.0, 1, 60.0, 0, 37, 2, (1e-2, 3e-2)),
+    "f2_newtree":    (2, 8, 0, 0, 1e-5, 2, 0.0, 1, 0.0, 1, 38, 3, (1e-3, 1e-2)),
+}
+
+
+def test_source_lib(outdir):
+    """tests/csrc/sources.c compiled into outdir (the source function handed
+    to both tree builders)."""
+    import subprocess
+    so = os.path.join(outdir, "libtestsource.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-o", so, os.path.join(ROOT, "tests", "csrc", "sources.c"),
+                    "-lm"], check=True)
+    L = C.CDLL(so)
+    return L, L.test_source
+
+
+def source_pars(ndim, seed, ng, wr, iperiod):
+    c, s, a = make_source(ndim, seed, ng, wr, iperiod)
+    dp = np.ascontiguousarray(np.hstack([c, s, a.reshape(-1, 1)]).ravel())
+    ip = np.array([ndim, ng, iperiod], np.int32)
+    return dp, ip
+
+
+def tree_fun_case(name, spec, fun):
+    ndim, n, ipoly, iper, eps, iptype, eta, nd, zk, ifnt, seed, ng, wr = spec
+    dp, ip = source_pars(ndim, seed, ng, wr, iper)
+    tr = refrun.build_tree_fun(ndim, n, ipoly, iper, eps, fun, dp, ip, nd=nd, iptype=iptype, eta=eta, zk=zk,
+                               ifnewtree=ifnt)
+    nused = int(tr.itree[2 * tr.nlevels + 1])
+    path = os.path.join(HERE, f"tree_{name}.npz")
+    np.savez_compressed(path, ndim=ndim, norder=n, ipoly=ipoly, iperiod=iper, eps=eps, iptype=iptype, eta=eta,
+                        nd=nd, zk=zk, ifnewtree=ifnt, dpars=dp, ipars=ip, nboxes=tr.nboxes, nlevels=tr.nlevels,
+                        ltree=tr.ltree, itree=tr.itree, iptr=tr.iptr, centers=tr.centers[:nused],
+                        boxsize=tr.boxsize, fvals=tr.fvals[:nused])
+    print(f"tree_{name}: boxes {nused} (arrays {tr.nboxes}), nlevels {tr.nlevels}, "
+          f"size={os.path.getsize(path)/1e6:.2f} MB")
+
+
 def tables_mref3():
     """mk_loctab_all with mrefinelev = 3 (193 offsets per level), the tables
     boxfgt uses when it refines the tree after the solve."""
@@ -370,3 +420,10 @@ if __name__ == "__main__":
     for name, spec in NEWTREE_CASES.items():
         if not only or name in only or "newtree" in only:
             newtree_case(name, spec)
+    if not only or "tree_fun" in only or any(k in only for k in TREE_FUN_CASES):
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            _, fun = test_source_lib(td)
+            for name, spec in TREE_FUN_CASES.items():
+                if not only or name in only or "tree_fun" in only:
+                    tree_fun_case(name, spec, fun)
