@@ -55,13 +55,17 @@ def make_workload(seed=1, ng=20, n=16, eps=1e-9, delta=1e-4, iperiod=1):
     return tree, dict(n=n, eps=eps, delta=delta, iperiod=iperiod, ng=ng, seed=seed)
 
 
+# STEP 3 kernel by form: target groups, brick entry streams, separable bricks
+M2L_KERNEL = {0: "k_m2l27", 1: "k_m2l_brick", 2: "k_m2l_sep"}
+
+
 def stage_model(st, n, nd=1):
     """Algorithmic FLOPs and HBM bytes of one launch of the single-kernel stages.
 
     STEP 5 (k_pw2pot3, per PW leaf, 3-D): 4 n^3 nh + 8 n^2 npw nh + 8 n npw^2 nh
     flops (the x/y stages complex x complex, the last stage real output);
     bytes: read one local expansion (16 nexp) + read/write pot (16 n^3).
-    STEP 3 (k_m2l_brick / k_m2l27, per cutoff-level box): 8 nexp flops per list entry; compulsory
+    STEP 3 (k_m2l_sep / k_m2l_brick / k_m2l27, per cutoff-level box): 8 nexp flops per list entry; compulsory
     bytes: every cutoff-level mp read once and every loc written once,
     2 x 16 nexp per box (re-reads of a shared source are not algorithmic).
     """
@@ -72,7 +76,7 @@ def stage_model(st, n, nd=1):
     return {
         "pw2pot": {"kernel": "k_pw2pot3", "bound": "mfma", "flops": L * nd * f15,
                    "bytes": L * nd * (16 * nexp + 16 * n ** 3)},
-        "m2l": {"kernel": "k_m2l_brick" if st.get("m2l_bricks", -1) > 0 else "k_m2l27", "bound": "hbm",
+        "m2l": {"kernel": M2L_KERNEL[st.get("m2l_form", 0)], "bound": "hbm",
                 "flops": 8 * nd * nexp * st["n_m2l_pairs"],
                 "bytes": 2 * 16 * nd * nexp * st["n_m2l_targets"]},
     }
@@ -199,7 +203,7 @@ def batch_roofline(probs, prof):
                      "stage_ms": [round(v, 4) for v in ms], "step15_gflop": round(flops / 1e9, 3),
                      "step3_gb": round(byts / 1e9, 3)})
         for stg, kern, kind, work in ((4, "k_pw2pot3", "flops", flops), (0, "k_p2pwF3+k_p2pw3", "flops", flops),
-                                      (2, "k_m2l_brick" if c["m2l_brick"] else "k_m2l27", "bytes", byts)):
+                                      (2, M2L_KERNEL[c["m2l_brick"]], "bytes", byts)):
             if ms[stg] > 0 and work > 0 and (best is None or ms[stg] > best[0]):
                 best = (ms[stg], kern, kind, work, n, prec, c["npw_cap"])
     if best is None:

@@ -38,7 +38,7 @@ class _SolverStats(C.Structure):
                 ("ndirect_pairs", C.c_longlong), ("nout", C.c_longlong),
                 ("expansion_bytes", C.c_longlong), ("stage_ms", C.c_double * 8),
                 ("n_m2l_targets", C.c_longlong), ("n_m2l_pairs", C.c_longlong),
-                ("n_parent_child", C.c_longlong), ("m2l_bricks", C.c_longlong)]
+                ("n_parent_child", C.c_longlong), ("m2l_bricks", C.c_longlong), ("m2l_form", C.c_int)]
 
 
 class _PlanInfo(C.Structure):

@@ -134,6 +134,7 @@ struct fgt_solver {
     int n_m2l_groups = 0;
     DBuf d_mb_toff, d_mb_gent, d_mb_soff, d_mb_code;     // STEP 3 over sibling bricks
     int n_m2l_bricks = -1;                                // -1: bricks not used
+    bool m2l_sep_ok = false;                              // bricks' lists geometric: separable form
     DBuf d_dir_tgt, d_dir_lev, d_dir_off, d_dir_src, d_dir_code, d_dir_flags;
     // 3-D fast path: STEP 4 folded into STEP 5 for leaves below the cutoff level
     bool fast3d = false;
@@ -250,17 +251,24 @@ int m2l_streams(const fgt::Plan& P, int G, std::vector<long long>& toff, std::ve
     return ng;
 }
 
-// STEP 3 entry streams over sibling bricks (3-D, cutoff level below the
-// root): the targets that share a parent form one brick; each source of
+// STEP 3 entry streams over sibling bricks (3-D): the targets that share a
+// parent form one brick (the root, when it is the cutoff level, one alone); each source of
 // their lists (and each target's own expansion) becomes one entry at its
 // position in the 4x4x4 neighbourhood, with the mask of the siblings whose
 // list holds it.  Per brick: toff[8] loc element offsets (-1 = no target),
 // entries ordered by position, padded with zero-slot entries.  Returns the
 // brick count, or -1 when the lists do not fit that shape (a position
 // holding two different sources), in which case the grouped streams serve.
+// Code bits: 0-5 position, 6-13 target mask, 14 the source box is a leaf,
+// 15 the target at this position is a leaf.  *geometric: every entry's mask
+// is exactly the siblings present whose 3x3x3 neighbourhood holds its
+// position, less the leaf targets for a leaf source other than themselves
+// (the list rule of the cutoff level: leaf-leaf pairs are near field), so
+// the separable form applies.
 int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int>& gent,
-               std::vector<long long>& soff, std::vector<int>& code) {
-    if (P.ndim != 3 || P.nlevstart < 1 || !P.do_pw) return -1;
+               std::vector<long long>& soff, std::vector<int>& code, bool* geometric = nullptr) {
+    if (geometric) *geometric = false;
+    if (P.ndim != 3 || !P.do_pw) return -1;
     const fgt::TreeView& T = P.tree;
     const long long se = 2LL * P.nd * P.nexp;
     const int nt = (int)P.m2l_tgt.size();
@@ -272,14 +280,16 @@ int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int
     for (int tk = 0; tk < nt; ++tk) {
         const int b = box_of_slot[P.m2l_tgt[tk]];
         if (b < 0) return -1;
+        // the root (cutoff level 0) has no parent: a brick of its own, at
+        // brick position 0
         const int par = T.parent[b];
-        if (par < 0) return -1;
         int tp = 0;
-        for (int a = 0; a < 3; ++a)
-            if (T.centers[3 * b + a] > T.centers[3 * par + a]) tp |= 1 << a;
-        auto it = brick_of_parent.find(par);
+        if (par >= 0)
+            for (int a = 0; a < 3; ++a)
+                if (T.centers[3 * b + a] > T.centers[3 * par + a]) tp |= 1 << a;
+        auto it = brick_of_parent.find(par >= 0 ? par : -1 - b);
         if (it == brick_of_parent.end()) {
-            it = brick_of_parent.emplace(par, (int)tg.size()).first;
+            it = brick_of_parent.emplace(par >= 0 ? par : -1 - b, (int)tg.size()).first;
             std::array<int, 8> none;
             none.fill(-1);
             tg.push_back(none);
@@ -288,6 +298,7 @@ int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int
         tg[it->second][tp] = tk;
     }
     const int nb = (int)tg.size();
+    bool geo = true;
     toff.assign((size_t)nb * 8, -1);
     gent.assign(nb + 1, 0);
     soff.clear();
@@ -316,10 +327,32 @@ int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int
                 if (!add(P.m2l_src[j], idx % 3 - 1, (idx / 3) % 3 - 1, idx / 9 - 1)) return -1;
             }
         }
+        auto leaf_slot = [&](int slot) { return T.nchild[box_of_slot[slot]] == 0; };
         for (int pos = 0; pos < 64; ++pos)
             if (mask_at[pos]) {
+                const bool leaf_src = leaf_slot(src_at[pos]);
+                // the target at this position (its own entry), if any
+                int own = -1;
+                if ((pos & 3) >= 1 && (pos & 3) <= 2 && ((pos >> 2) & 3) >= 1 && ((pos >> 2) & 3) <= 2 &&
+                    (pos >> 4) >= 1 && (pos >> 4) <= 2)
+                    own = ((pos & 3) - 1) + 2 * (((pos >> 2) & 3) - 1) + 4 * ((pos >> 4) - 1);
+                const bool leaf_tgt = own >= 0 && tg[k][own] >= 0 && leaf_slot(P.m2l_tgt[tg[k][own]]);
                 soff.push_back(se * src_at[pos]);
-                code.push_back(pos | mask_at[pos] << 6);
+                code.push_back(pos | mask_at[pos] << 6 | (leaf_src ? 1 << 14 : 0) | (leaf_tgt ? 1 << 15 : 0));
+                int want = 0;
+                for (int tp = 0; tp < 8; ++tp) {
+                    if (tg[k][tp] < 0) continue;
+                    bool in = true;
+                    for (int a = 0; a < 3; ++a) {
+                        const int d = ((pos >> (2 * a)) & 3) - (((tp >> a) & 1) + 1);
+                        if (d < -1 || d > 1) in = false;
+                    }
+                    if (in && leaf_src && tp != own && leaf_slot(P.m2l_tgt[tg[k][tp]])) in = false;
+                    if (in) want |= 1 << tp;
+                }
+                if (want != mask_at[pos]) geo = false;
+                // an own entry must hold the target's own expansion
+                if (own >= 0 && tg[k][own] >= 0 && src_at[pos] != P.m2l_tgt[tg[k][own]]) geo = false;
             }
         while ((soff.size() - gent[k]) % fgt::kM2lBatch) {
             soff.push_back(se * P.nslots);   // the zero slot, applied to no target
@@ -327,6 +360,7 @@ int m2l_bricks(const fgt::Plan& P, std::vector<long long>& toff, std::vector<int
         }
     }
     gent[nb] = (int)soff.size();
+    if (geometric) *geometric = geo;
     return nb;
 }
 
@@ -702,7 +736,9 @@ void build_solver(fgt_solver& S, const fgt::PlanInput& in, bool own_io = true, f
         if (fgt::m2l_brick_env() != 0) {
             std::vector<long long> bt, bs;
             std::vector<int> bg, bc;
-            S.n_m2l_bricks = m2l_bricks(P, bt, bg, bs, bc);
+            bool geo = false;
+            S.n_m2l_bricks = m2l_bricks(P, bt, bg, bs, bc, &geo);
+            S.m2l_sep_ok = S.n_m2l_bricks > 0 && geo;
             if (S.n_m2l_bricks > 0) {
                 S.d_mb_toff.upload(bt);
                 S.d_mb_gent.upload(bg);
@@ -901,7 +937,10 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
             mark(2);
         }
         // STEP 3
-        if (S.n_m2l_bricks > 0)
+        if (S.n_m2l_bricks > 0 && S.m2l_sep_ok && fgt::m2l_brick_env() == 2)
+            fgt::launch_m2l_sep(g, t, S.n_m2l_bricks, 2LL * P.nd * P.nexp * P.nslots, S.d_mb_toff.as<long long>(),
+                                S.d_mb_gent.as<int>(), S.d_mb_soff.as<long long>(), S.d_mb_code.as<int>(), ex, st);
+        else if (S.n_m2l_bricks > 0)
             fgt::launch_m2l_brick(g, t, S.n_m2l_bricks, 2LL * P.nd * P.nexp * P.nslots, S.d_mb_toff.as<long long>(),
                                   S.d_mb_gent.as<int>(), S.d_mb_soff.as<long long>(), S.d_mb_code.as<int>(), ex, st);
         else
@@ -1160,6 +1199,7 @@ void class_m2l(fgt_batch& B, BClass& C) {
     for (int q = 0; q < (int)C.members.size() && C.m2l_brick; ++q) {
         const fgt_solver& S = *B.solvers[C.members[q]];
         if (S.plan.do_pw && S.n_m2l_bricks <= 0) C.m2l_brick = 0;   // a member without bricks
+        if (S.plan.do_pw && C.m2l_brick == 2 && !S.m2l_sep_ok) C.m2l_brick = 1;   // not separable
     }
     for (int q = 0; q < (int)C.members.size(); ++q) {
         const fgt_solver& S = *B.solvers[C.members[q]];
@@ -1227,7 +1267,9 @@ void build_classes(fgt_batch& B) {
         for (int i : C.members)
             if (((long long)B.solvers[i]->plan.nd * B.solvers[i]->plan.nexp) & 1) C.m2l_ept2_ok = false;
         C.m2l_ept = m2l_ept_default(C.m2l_ept2_ok);
-        C.m2l_brick = fgt::m2l_brick_env() != 0 ? 1 : 0;   // class_m2l drops it where a member cannot
+        // brick entry streams (separable bricks with FGT_M2L_BRICK=2);
+        // class_m2l steps down where a member cannot
+        C.m2l_brick = fgt::m2l_brick_env() == 0 ? 0 : (fgt::m2l_brick_env() == 2 ? 2 : 1);
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tuned.find(std::make_tuple(C.n, C.nd, C.prec, C.npw_cap));
@@ -1424,7 +1466,10 @@ void class_step2(const BClass& C, hipStream_t st) {
     }
 }
 void class_step3(const BClass& C, hipStream_t st) {
-    if (C.m2l_brick)
+    if (C.m2l_brick == 2)
+        fgt::launch_m2l_sep(C.g, C.t, C.n_m2l, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
+                            C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob));
+    else if (C.m2l_brick)
         fgt::launch_m2l_brick(C.g, C.t, C.n_m2l, 0, C.m2l_toff.as<long long>(), C.m2l_gent.as<int>(),
                               C.m2l_soff.as<long long>(), C.m2l_code.as<int>(), nullptr, st, multi(C, C.m2l_prob));
     else
@@ -1548,18 +1593,22 @@ void batch_autotune(fgt_batch& B, int reps) {
         if (C.n_m2l > 0) {
             int best = C.group, best_e = C.m2l_ept, best_b = 0;
             float bt = 1e30f;
-            // sibling bricks (when every member allows them), then the groups
-            if (fgt::m2l_brick_env() != 0) {
-                C.m2l_brick = 1;
+            // sibling bricks (entry streams; the separable form only when
+            // FGT_M2L_BRICK=2 asks for it), where every member allows them,
+            // then the groups
+            const int benv = fgt::m2l_brick_env();
+            for (int form : {2, 1}) {
+                if (benv == 0 || (benv < 0 && form == 2) || (benv > 0 && benv != form)) continue;
+                C.m2l_brick = form;
                 class_m2l(B, C);
-                if (C.m2l_brick) {
-                    bt = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
-                    best_b = 1;
+                if (C.m2l_brick == form) {
+                    const float ms = time_ms(st, reps, [&](hipStream_t s) { class_step3(C, s); });
+                    if (ms < bt) bt = ms, best_b = form;
                 }
             }
             C.m2l_brick = 0;
             for (int E : {1, 2}) {
-                if (fgt::m2l_brick_env() == 1 && best_b) break;   // bricks forced
+                if (benv > 0 && best_b) break;   // bricks forced
                 if (E == 2 && !C.m2l_ept2_ok) continue;
                 if (fgt::m2l_ept_env() && E != fgt::m2l_ept_env()) continue;
                 C.m2l_ept = E;
@@ -1686,6 +1735,7 @@ int fgt_solver_stats_get(const fgt_solver* s, fgt_solver_stats* o) {
     o->n_m2l_pairs = (long long)P.m2l_src.size();
     o->n_parent_child = (long long)P.pc_child.size();
     o->m2l_bricks = s->n_m2l_bricks > 0 ? s->n_m2l_bricks : -1;
+    o->m2l_form = s->n_m2l_bricks <= 0 ? 0 : (s->m2l_sep_ok && fgt::m2l_brick_env() == 2 ? 2 : 1);
     return 0;
 }
 

@@ -64,6 +64,17 @@ __device__ __forceinline__ int xcd_task(int b, int nb) {
     const int x = b & 7, i = b >> 3, per = nb >> 3, rem = nb & 7;
     return x * per + min(x, rem) + i;
 }
+// Runs of C consecutive tasks per XCD, dealt round-robin: the XCDs work on
+// the same region of the task list at any time (balanced when task costs
+// drift along the list) while each run (e.g. the children of one parent)
+// shares one L2.  Blocks past the last whole round keep their own index.
+template <int C>
+__device__ __forceinline__ int xcd_run_task(int b, int nb) {
+    const int whole = nb / (8 * C) * (8 * C);
+    if (b >= whole) return b;
+    const int x = b & 7, i = b >> 3;
+    return (i / C) * (8 * C) + x * C + (i % C);
+}
 #endif
 
 // One output component of STEP 5 / STEP 6 (potential, one gradient or one
@@ -223,8 +234,22 @@ constexpr int kM2lBrickMax = 64 + kM2lBatch;
 void launch_m2l_brick(const Geom& g, const DevTables& t, int nbrick, long long zero_off, const long long* toff,
                       const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s,
                       const Multi& mu = Multi{});
+// The same bricks as a separable stencil (the brick's lists must be
+// geometric: every source holds every sibling whose 3x3x3 neighbourhood
+// contains it): the phase of offset (ox, oy, oz) is w_x(ox) w_y(oy)
+// w_z(oz) with w(0) = conj(w(2)), w(1) = 1, so the sum over a sibling's 27
+// neighbours runs as three 3-tap passes along x, y and z over the brick's
+// 4x4x4 source grid, each source loaded once.  Same arrays as
+// launch_m2l_brick.
+void launch_m2l_sep(const Geom& g, const DevTables& t, int nbrick, long long zero_off, const long long* toff,
+                    const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s,
+                    const Multi& mu = Multi{});
 // FGT_M2L_BRICK: 0 keeps STEP 3 on the target groups, 1 on sibling bricks
-// where a problem allows them; unset (-1) = policy
+// (entry streams), 2 on sibling bricks as a separable stencil, where a
+// problem allows them; unset (-1) = policy (entry-stream bricks: the
+// separable form does a sixth of the arithmetic but was measured slower,
+// 1.17 vs 0.74 ms on a norder-16 periodic problem, bound by the cache
+// traffic of the source loads that both forms share)
 int m2l_brick_env();
 void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_parent,
                   const int* pc_child, const int* pc_sign, int klev, double2* expn,

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <type_traits>
 
 namespace fgt {
 
@@ -436,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int n
     _Pragma("unroll") for (int u = 0; u < kM2lBatch; ++u) {                              \
         const int code = __builtin_amdgcn_readfirstlane(s_code[(j) + u]);                \
         const int px = code & 3, py = (code >> 2) & 3, pz = (code >> 4) & 3;            \
-        const int mask = code >> 6;                                                      \
+        const int mask = (code >> 6) & 255;                                              \
         _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                  \
             if ((mask >> q) & 1) {                                                       \
                 const int ox = (q & 1) + 2 - px, oy = ((q >> 1) & 1) + 2 - py;           \
@@ -464,6 +465,194 @@ __global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int n
     }
 #undef M2L_APPLY
 #undef M2L_LOAD
+    if (ok) {
+        const long long* __restrict__ to = toff + (size_t)brk * 8;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long long o = to[q];
+            if (o >= 0) stE(expn + o + tot + e, acc[q]);
+        }
+    }
+}
+
+// STEP 3 over a sibling brick as a separable stencil (see launch_m2l_sep).
+// Source grid positions p = (px, py, pz) in 0..3, siblings q in 0..1 per
+// axis at p = q + 1; sibling q takes the source at p with the offset
+// o_a = q_a + 2 - p_a in 0..2.  Rows (py, pz) of four sources go x first:
+//   X_qx = w(2) v(qx) + v(qx + 1) + conj(w(2)) v(qx + 2)
+// then into the plane sums Y(qx, qy) += w_y(qy + 2 - py) X_qx, and after a
+// plane's four rows into acc(q) += w_z(qz + 2 - pz) Y(qx, qy).  Every index
+// is a compile-time constant of the unrolled loops, so each phase is a
+// register (no selects).  Rows without a source are skipped (uniform
+// branches); loads run two rows ahead of the arithmetic.
+//
+// A leaf target's list leaves out the leaf sources other than itself (their
+// pairs are near field).  The first sweep applies every source to every
+// sibling; a second sweep over the leaf sources alone, with the values
+// negated, goes to the leaf targets only, which then take their own
+// expansion back.  Entry code bits: 0-5 position, 6-13 target mask, 14 the
+// source is a leaf, 15 the target at this position is a leaf.
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+// w u + conj(w) v
+__device__ __forceinline__ double2 csym(double2 w, double2 u, double2 v) {
+    const double sx = u.x + v.x, sy = u.y + v.y, dx = u.x - v.x, dy = u.y - v.y;
+    return make_double2(w.x * sx - w.y * dy, w.x * sy + w.y * dx);
+}
+// acc += w^(O) x with w^(2) = w, w^(1) = 1, w^(0) = conj(w)
+template <int O>
+__device__ __forceinline__ void tap(double2& acc, const double2 w, const double2 x) {
+    if constexpr (O == 1) {
+        acc.x += x.x;
+        acc.y += x.y;
+    } else if constexpr (O == 2) {
+        acc = cfma(x, w, acc);
+    } else {
+        acc = cfma(x, conjd(w), acc);
+    }
+}
+
+template <typename ET>
+__global__ void __launch_bounds__(kBlock) k_m2l_sep(Geom g, DevTables t, int nbrick, long long zero_off,
+                                                    const long long* __restrict__ toff,
+                                                    const int* __restrict__ gent,
+                                                    const long long* __restrict__ soff,
+                                                    const int* __restrict__ scode,
+                                                    ET* __restrict__ expn_a, Multi mu) {
+    const int lin = xcd_task(blockIdx.x, gridDim.x);
+    const int brk = lin % nbrick, chunk = lin / nbrick;
+    ET* __restrict__ expn = expn_a;
+    if (mu.pp) {
+        const ProbPtrs& q = mu.pp[mu.tprob[brk]];
+        g = q.g;
+        t.npw = q.g.npw;
+        expn = static_cast<ET*>(q.expn);
+        t.wm2l = q.wm2l;
+        zero_off = q.zero_off;
+    }
+    const long long tot = (long long)g.nd * g.nexp;
+    if ((long long)chunk * blockDim.x >= tot) return;   // past this problem's elements
+    const int npw = t.npw;
+    long long e = (long long)chunk * blockDim.x + threadIdx.x;
+    const bool ok = e < tot;
+    if (!ok) e = tot - 1;   // valid duplicate, never stored
+    const int r = (int)(e % g.nexp);
+    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+    const double2 wx = t.wm2l[2 * npw + k1], wy = t.wm2l[2 * npw + k2], wz = t.wm2l[2 * npw + k3];
+    // the brick's entries as a dense 4x4x4 grid of source offsets, with
+    // 4-bit presence masks per row (py, pz) for all and for leaf sources,
+    // and the mask of leaf targets
+    __shared__ long long s_poff[64];
+    __shared__ int s_rows[2][16];
+    __shared__ int s_leaft;
+    const int tid = threadIdx.x;
+    if (tid < 64) s_poff[tid] = zero_off;
+    if (tid < 32) s_rows[tid >> 4][tid & 15] = 0;
+    if (tid == 0) s_leaft = 0;
+    __syncthreads();
+    const int g0 = gent[brk], n = gent[brk + 1] - g0;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int c = scode[g0 + i];
+        if ((c >> 6) & 255) {   // padding entries carry no target
+            const int pos = c & 63;
+            s_poff[pos] = soff[g0 + i];
+            atomicOr(&s_rows[0][pos >> 2], 1 << (pos & 3));
+            if ((c >> 14) & 1) atomicOr(&s_rows[1][pos >> 2], 1 << (pos & 3));
+            if ((c >> 15) & 1) {
+                const int qx = (pos & 3) - 1, qy = ((pos >> 2) & 3) - 1, qz = (pos >> 4) - 1;
+                atomicOr(&s_leaft, 1 << (qx + 2 * qy + 4 * qz));
+            }
+        }
+    }
+    __syncthreads();
+    const int leaft = __builtin_amdgcn_readfirstlane(s_leaft);
+    const ET* __restrict__ eb = expn + e;
+    double2 acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = make_double2(0.0, 0.0);
+    // one sweep: LEAF = 0 every source to every sibling; LEAF = 1 the leaf
+    // sources, negated, to the leaf targets
+    auto sweep = [&](auto leaf_tag) {
+        constexpr int LEAF = decltype(leaf_tag)::value;
+        double2 Y[4];
+        ET v[3][4];
+        int rb[3];
+        auto load_row = [&](int row, int buf) {
+            const int m = __builtin_amdgcn_readfirstlane(s_rows[LEAF][row]);
+            rb[buf] = m;
+            if (m) {
+#pragma unroll
+                for (int px = 0; px < 4; ++px)
+                    if (!LEAF || ((m >> px) & 1)) v[buf][px] = eb[s_poff[4 * row + px]];
+            }
+        };
+        load_row(0, 0);
+        load_row(1, 1);
+#pragma unroll
+        for (int row = 0; row < 16; ++row) {
+            const int py = row & 3, pz = row >> 2;
+            if (row + 2 < 16) load_row(row + 2, (row + 2) % 3);
+            if (py == 0) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Y[k] = make_double2(0.0, 0.0);
+            }
+            const int b = row % 3;
+            const int m = rb[b];
+            if (m) {
+                double2 u[4];
+#pragma unroll
+                for (int px = 0; px < 4; ++px) {
+                    u[px] = ldE(v[b][px]);
+                    if (LEAF) {
+                        if ((m >> px) & 1) u[px] = make_double2(-u[px].x, -u[px].y);
+                        else u[px] = make_double2(0.0, 0.0);
+                    }
+                }
+                double2 X[2];
+                X[0] = cadd(u[1], csym(wx, u[0], u[2]));
+                X[1] = cadd(u[2], csym(wx, u[1], u[3]));
+                // Y(qx, qy) at k = qx + 2 qy, offset oy = qy + 2 - py
+#pragma unroll
+                for (int qx = 0; qx < 2; ++qx) {
+                    switch (py) {
+                        case 0: tap<2>(Y[qx], wy, X[qx]); break;
+                        case 1: tap<1>(Y[qx], wy, X[qx]); tap<2>(Y[qx + 2], wy, X[qx]); break;
+                        case 2: tap<0>(Y[qx], wy, X[qx]); tap<1>(Y[qx + 2], wy, X[qx]); break;
+                        default: tap<0>(Y[qx + 2], wy, X[qx]); break;
+                    }
+                }
+            }
+            if (py == 3) {
+                // acc(q) at q = k + 4 qz, offset oz = qz + 2 - pz
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const bool lo = !LEAF || ((leaft >> k) & 1), hi = !LEAF || ((leaft >> (k + 4)) & 1);
+                    switch (pz) {
+                        case 0: if (lo) tap<2>(acc[k], wz, Y[k]); break;
+                        case 1:
+                            if (lo) tap<1>(acc[k], wz, Y[k]);
+                            if (hi) tap<2>(acc[k + 4], wz, Y[k]);
+                            break;
+                        case 2:
+                            if (lo) tap<0>(acc[k], wz, Y[k]);
+                            if (hi) tap<1>(acc[k + 4], wz, Y[k]);
+                            break;
+                        default: if (hi) tap<0>(acc[k + 4], wz, Y[k]); break;
+                    }
+                }
+            }
+        }
+    };
+    sweep(std::integral_constant<int, 0>{});
+    if (leaft) {
+        sweep(std::integral_constant<int, 1>{});
+        // the leaf targets' own expansions, taken out with the leaf sources
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if ((leaft >> q) & 1) {
+                const double2 own = ldE(eb[s_poff[((q & 1) + 1) + 4 * (((q >> 1) & 1) + 1) + 16 * ((q >> 2) + 1)]]);
+                acc[q] = cadd(acc[q], own);
+            }
+    }
     if (ok) {
         const long long* __restrict__ to = toff + (size_t)brk * 8;
 #pragma unroll
@@ -825,6 +1014,22 @@ void launch_m2l_brick(const Geom& g, const DevTables& t, int nbrick, long long z
         hipLaunchKernelGGL(k_m2l_brick<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, nbrick, zero_off,
                            toff, gent, soff, scode, expn, mu);
     check(hipGetLastError(), "k_m2l_brick launch");
+}
+
+void launch_m2l_sep(const Geom& g, const DevTables& t, int nbrick, long long zero_off, const long long* toff,
+                    const int* gent, const long long* soff, const int* scode, double2* expn, hipStream_t s,
+                    const Multi& mu) {
+    if (nbrick <= 0) return;
+    if (g.ndim != 3) throw std::runtime_error("k_m2l_sep: 3-D only");
+    const long long nblk = (long long)blocks_for((long long)g.nd * g.nexp) * nbrick;
+    if (nblk > 0x7fffffffLL) throw std::runtime_error("k_m2l_sep: grid too large");
+    if (t.expn_f32)
+        hipLaunchKernelGGL(k_m2l_sep<float2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, nbrick, zero_off,
+                           toff, gent, soff, scode, reinterpret_cast<float2*>(expn), mu);
+    else
+        hipLaunchKernelGGL(k_m2l_sep<double2>, dim3((unsigned)nblk), dim3(kBlock), 0, s, g, t, nbrick, zero_off,
+                           toff, gent, soff, scode, expn, mu);
+    check(hipGetLastError(), "k_m2l_sep launch");
 }
 
 int m2l_brick_env() {

@@ -10,27 +10,45 @@
 //     pot += sum_z A_z ( sum_{y | z} A_y ( sum_{x | y,z} A_x f_s ) )
 // which needs one x contraction per source, one y contraction per (y,z)
 // group and one z contraction per z group.
-// One 256-thread block per target leaf; the 1-D tables (masked to their
+// One block of N^2 threads (rounded up to whole waves: 64 / 192 / 256 for
+// n = 8 / 12 / 16) per target leaf; the 1-D tables (masked to their
 // significant band, zero elsewhere) are wave-uniform and read into SGPRs;
 // the partial sums live in registers between LDS transposes.
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace fgt {
 
 namespace {
 
+// FGT_DIRECT_RUNS=1: runs of eight consecutive targets per XCD (siblings
+// read their shared sources through one L2); default: launch order, which
+// measured faster on a norder-16 periodic problem (0.71 vs 0.88 ms: the
+// runs put targets of similar cost on the same XCD)
+int direct_runs_env() {
+    static const int v = [] {
+        const char* e = std::getenv("FGT_DIRECT_RUNS");
+        return e ? std::atoi(e) : 0;
+    }();
+    return v;
+}
+
 // flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group;
 // nls table slots per level, slot codes 8 bits per axis
 template <int N>
-__global__ void __launch_bounds__(256, 3) k_direct3(Geom g, const double* __restrict__ loctab_a, int nls,
+constexpr int direct3_block() { return (N * N + 63) / 64 * 64; }
+
+template <int N>
+__global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3(Geom g, const double* __restrict__ loctab_a, int nls,
                                                 const int* __restrict__ tgt, const int* __restrict__ lev,
                                                 const int* __restrict__ off, const int* __restrict__ src,
                                                 const int* __restrict__ code, const int* __restrict__ flags,
                                                 const double* __restrict__ fvals_a, size_t ks, Kinds kd,
-                                                OutDesc od, Multi mu) {
+                                                OutDesc od, Multi mu, int runs) {
     extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
     constexpr int NN = N * N;
-    const int task = blockIdx.x;
+    const int task = runs ? xcd_run_task<8>(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const double* __restrict__ loctab = loctab_a;
     const double* __restrict__ fvals = fvals_a;
     if (mu.pp) {
@@ -147,8 +165,8 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
                  hipStream_t s, const Multi& mu) {
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
-    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(256), lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src, code,
-                       flags, fvals, t.loctab_ks, kd, od, mu);
+    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(direct3_block<N>()), lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src, code,
+                       flags, fvals, t.loctab_ks, kd, od, mu, direct_runs_env());
     return true;
 }
 

@@ -120,6 +120,8 @@ typedef struct {
     long long n_m2l_targets, n_m2l_pairs;   /* cutoff-level boxes receiving a loc; list entries */
     long long n_parent_child;                /* child->parent merge edges */
     long long m2l_bricks;                    /* STEP 3 sibling bricks, -1 when it runs on target groups */
+    int m2l_form;                            /* STEP 3 kernel: 0 target groups, 1 brick entry streams,
+                                                2 separable brick stencil */
 } fgt_solver_stats;
 
 int fgt_solver_create(int nd, int ndim, double delta, double eps, int ipoly,
