@@ -22,16 +22,18 @@ namespace fgt {
 
 namespace {
 
-// FGT_DIRECT_RUNS=1: runs of eight consecutive targets per XCD (siblings
-// read their shared sources through one L2); default: launch order, which
-// measured faster on a norder-16 periodic problem (0.71 vs 0.88 ms: the
-// runs put targets of similar cost on the same XCD)
-int direct_runs_env() {
+// Task order: runs of eight consecutive targets per XCD (neighbours of one
+// problem, whose list-1 sources overlap, read them through one L2) or the
+// launch order.  Batches take the runs (their many small near-field-only
+// problems: 1.1-1.8 vs 4.4 ms for a norder-16 class), single problems the
+// launch order (0.71 vs 0.88 ms on a norder-16 periodic problem);
+// FGT_DIRECT_RUNS=0/1 forces one.
+int direct_runs(const Multi& mu) {
     static const int v = [] {
         const char* e = std::getenv("FGT_DIRECT_RUNS");
-        return e ? std::atoi(e) : 0;
+        return e ? std::atoi(e) : -1;
     }();
-    return v;
+    return v >= 0 ? v : (mu.pp ? 1 : 0);
 }
 
 // flags per source entry: bit0 = last of its (y,z) group, bit1 = last of its z group;
@@ -166,7 +168,7 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
     hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(direct3_block<N>()), lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src, code,
-                       flags, fvals, t.loctab_ks, kd, od, mu, direct_runs_env());
+                       flags, fvals, t.loctab_ks, kd, od, mu, direct_runs(mu));
     return true;
 }
 

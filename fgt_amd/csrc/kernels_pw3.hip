@@ -361,7 +361,38 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
         ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
         for (int r = rb; r < re; r += 2) {
             // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
-            if (tid < N * N) {
+            if constexpr (N * N == 64) {
+                // one wave per group g = (sx, sy): its (up to two) children
+                const int q = wave;
+                double2 H[2];
+                H[0] = H[1] = make_double2(0.0, 0.0);
+                for (int c = 0; c < nk; ++c) {
+                    const int sg = ksg[8 * task + c];
+                    if ((sg & 3) != q) continue;
+                    const int b = kids[8 * task + c];
+                    const int vz = 1 + ((sg >> 2) & 1);
+                    const CTab2 Tz = T + base0 + vz * per;
+                    double fr[N];
+#pragma unroll
+                    for (int m3 = 0; m3 < N; ++m3)
+                        fr[m3] = fvals[((size_t)b * g.npbox + lane + N * N * m3) * nd + ind];
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int k3 = min(r + s, nh - 1);
+                        double ar = 0.0, ai = 0.0;
+#pragma unroll
+                        for (int m3 = 0; m3 < N; ++m3) {
+                            const double2 t = Tz[k3 * N + m3];
+                            ar += t.x * fr[m3];
+                            ai += t.y * fr[m3];
+                        }
+                        H[s].x += ar;
+                        H[s].y += ai;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < 2; ++s) sH[((q * 2 + s) * N) * N + lane] = H[s];
+            } else if (tid < N * N) {
                 double2 H[4][2];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) H[q][0] = H[q][1] = make_double2(0.0, 0.0);
@@ -500,26 +531,56 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
     }
     double* __restrict__ sffd = reinterpret_cast<double*>(sff);
     // one row tile (16 frequencies k2) at a time, two accumulator chains per
-    // column tile; table operands are re-read per row tile from the cache
+    // column tile
     for (int mt = 0; mt < MT; ++mt) {
         const int k2r = min(16 * mt + li, npw - 1);
         flt4 acc[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = flt4{0.f, 0.f, 0.f, 0.f};
         if constexpr (XS == 2) {
+            // FP32: K steps in chunks of KC, the next chunk's operand row and
+            // table entries loaded (from valid clamped addresses, zeroed past
+            // npw) before the current chunk's products, into two register sets
+            // (two steps per chunk where two column tiles hold the registers)
+            constexpr int KC = NT == 1 ? 4 : 2;
             const int ks = (npw + 3) >> 2;
-            for (int st = 0; st < ks; ++st) {
-                const int k1 = 4 * st + lq;
-                const bool kv = k1 < npw;
-                const int k1c = kv ? k1 : npw - 1;
-                const float2 e = Ek[(size_t)k1c * npw + k2r];
+            const int nchunk = (ks + KC - 1) / KC;
+            float2 ea[KC], eb[KC];
+            float ba[NT][KC][2], bb[NT][KC][2];
+            auto load = [&](float2 (&e)[KC], float (&bv)[NT][KC][2], int c) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const double2 u = UX[k1c * N + ckk[t]];
-                    const bool ok = kv && cval[t];
-                    acc[t][0] = mfma32(e.x, ok ? bcol(u, cpart[t], 0) : 0.f, acc[t][0]);
-                    acc[t][1] = mfma32(e.y, ok ? bcol(u, cpart[t], 1) : 0.f, acc[t][1]);
+                for (int j = 0; j < KC; ++j) {
+                    const int k1 = 4 * (KC * c + j) + lq;
+                    const bool kv = k1 < npw;
+                    const int k1c = kv ? k1 : npw - 1;
+                    e[j] = Ek[(size_t)k1c * npw + k2r];
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const double2 u = UX[k1c * N + ckk[t]];
+                        const bool ok = kv && cval[t];
+                        bv[t][j][0] = ok ? bcol(u, cpart[t], 0) : 0.f;
+                        bv[t][j][1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                    }
                 }
+            };
+            auto apply = [&](const float2 (&e)[KC], const float (&bv)[NT][KC][2]) {
+#pragma unroll
+                for (int j = 0; j < KC; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        acc[t][0] = mfma32(e[j].x, bv[t][j][0], acc[t][0]);
+                        acc[t][1] = mfma32(e[j].y, bv[t][j][1], acc[t][1]);
+                    }
+            };
+            load(ea, ba, 0);
+            for (int c = 0; c < nchunk; c += 2) {
+                load(eb, bb, min(c + 1, nchunk - 1));
+                __builtin_amdgcn_sched_barrier(0);
+                apply(ea, ba);
+                if (c + 1 >= nchunk) break;
+                load(ea, ba, min(c + 2, nchunk - 1));
+                __builtin_amdgcn_sched_barrier(0);
+                apply(eb, bb);
             }
         } else {
             // K = (k1, part): 16 frequencies per step; lane K entries 8 lq .. 8 lq + 7
@@ -637,13 +698,20 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const int nh = g.pz, nd = g.nd;
     const int kk1 = lane & 15, s = lane >> 4;
     const bool halve = (npw & 1) != 0;
-    // z stage: one thread per output column (zk1, zk2), all kk3 in registers
-    const bool zact = tid < N * N;
-    const int zk1 = tid % N, zk2 = tid / N;
+    // z stage: one thread per output column (zk1, zk2), all kk3 in
+    // registers; where the block holds exactly one column per thread and leaf
+    // (N = 8, four leaves) each wave takes one leaf (ZW), else every thread
+    // below N^2 takes all the leaves
+    constexpr bool ZW = C > 1 && N * N * C == 256;
+    constexpr int CG = ZW ? 1 : C;
+    const bool zact = ZW || tid < N * N;
+    const int zt = ZW ? lane : tid;
+    const int zk1 = zt % N, zk2 = zt / N;
+    const int wbox = ZW ? T->box[wave] : -1, wcode = ZW ? T->code[wave] : 0;
     for (int ind = 0; ind < nd; ++ind) {
-        double G[C][N];
+        double G[CG][N];
 #pragma unroll
-        for (int c = 0; c < C; ++c)
+        for (int c = 0; c < CG; ++c)
 #pragma unroll
             for (int k = 0; k < N; ++k) G[c][k] = 0.0;
         const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
@@ -793,14 +861,25 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                         const double2 yv = yres[(s2 * 16 + zk1) * 16 + (zk2 ^ zk1)];
                         const double w = (halve && k3z == nh - 1) ? 0.5 : 1.0;
                         const double yr = yv.x * w, yi = yv.y * w;
-#pragma unroll
-                        for (int c = 0; c < C; ++c) {
-                            if (cbox[c] >= 0 && (ccode[c] & 1) == j) {
-                                const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * 4 + s2) * N;
+                        if constexpr (ZW) {
+                            if (wbox >= 0 && (wcode & 1) == j) {
+                                const double2* __restrict__ uz = sUr + ((wcode >> 1) * 4 + s2) * N;
 #pragma unroll
                                 for (int kk3 = 0; kk3 < N; ++kk3) {
                                     const double2 u = uz[kk3];
-                                    G[c][kk3] += u.x * yr - u.y * yi;
+                                    G[0][kk3] += u.x * yr - u.y * yi;
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < C; ++c) {
+                                if (cbox[c] >= 0 && (ccode[c] & 1) == j) {
+                                    const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * 4 + s2) * N;
+#pragma unroll
+                                    for (int kk3 = 0; kk3 < N; ++kk3) {
+                                        const double2 u = uz[kk3];
+                                        G[c][kk3] += u.x * yr - u.y * yi;
+                                    }
                                 }
                             }
                         }
@@ -815,12 +894,14 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             int col = zk1 + N * zk2;
             asm volatile("" : "+v"(col));
 #pragma unroll
-            for (int c = 0; c < C; ++c)
-                if (cbox[c] >= 0) {
-                    double* __restrict__ o = od.p + (size_t)cbox[c] * g.npbox * od.ns + od.coff + ind;
+            for (int c = 0; c < CG; ++c) {
+                const int b = ZW ? wbox : cbox[c];
+                if (b >= 0) {
+                    double* __restrict__ o = od.p + (size_t)b * g.npbox * od.ns + od.coff + ind;
 #pragma unroll
                     for (int kk3 = 0; kk3 < N; ++kk3) o[(size_t)(col + N * N * kk3) * od.ns] += 2.0 * G[c][kk3];
                 }
+            }
         }
     }
 }
