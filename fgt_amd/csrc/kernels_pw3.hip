@@ -105,8 +105,26 @@ __device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, 
             cval[gi] = (c >> 1) < npw && c0 + gi * ctstep < CT;
             k1c[gi] = cval[gi] ? (c >> 1) : 0;
         }
-        // one row tile at a time (few live accumulators, high occupancy);
-        // the table operands are re-read per row tile from the cache
+        // FP32: the column tiles' table operands of every K step, read once
+        // for all row tiles (the BF16 form re-reads them per row tile)
+        constexpr int KS = (KN + 3) / 4;
+        float tb1[CG][XS == 2 ? KS : 1], tb2[CG][XS == 2 ? KS : 1];
+        if constexpr (XS == 2) {
+#pragma unroll
+            for (int st = 0; st < KS; ++st) {
+                const int kk = 4 * st + lq;
+                const bool kv = kk < KN;
+                const int kkc = kv ? kk : KN - 1;
+#pragma unroll
+                for (int gi = 0; gi < CG; ++gi) {
+                    const double2 u = fb(kkc, k1c[gi]);
+                    const bool ok = kv && cval[gi];
+                    tb1[gi][st] = ok ? bcol(u, part[gi], 0) : 0.f;
+                    tb2[gi][st] = ok ? bcol(u, part[gi], 1) : 0.f;
+                }
+            }
+        }
+        // one row tile at a time (few live accumulators, high occupancy)
         for (int mt = 0; mt < MT; ++mt) {
             // two accumulator chains per tile (real / imaginary A parts, or
             // hi*hi / the correction products) so dependent MFMAs interleave
@@ -116,19 +134,13 @@ __device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, 
             const int k2r = min(16 * mt + li, npw - 1);
             if constexpr (XS == 2) {
 #pragma unroll
-                for (int st = 0; st < (KN + 3) / 4; ++st) {
+                for (int st = 0; st < KS; ++st) {
                     const int kk = 4 * st + lq;
-                    const bool kv = kk < KN;
-                    const int kkc = kv ? kk : KN - 1;
-                    const double2 a = fa(k2r, kkc);
+                    const double2 a = fa(k2r, kk < KN ? kk : KN - 1);
 #pragma unroll
                     for (int gi = 0; gi < CG; ++gi) {
-                        const double2 u = fb(kkc, k1c[gi]);
-                        const bool ok = kv && cval[gi];
-                        const float b1 = ok ? bcol(u, part[gi], 0) : 0.f;
-                        const float b2 = ok ? bcol(u, part[gi], 1) : 0.f;
-                        acc[gi][0] = mfma32((float)a.x, b1, acc[gi][0]);
-                        acc[gi][1] = mfma32((float)a.y, b2, acc[gi][1]);
+                        acc[gi][0] = mfma32((float)a.x, tb1[gi][st], acc[gi][0]);
+                        acc[gi][1] = mfma32((float)a.y, tb2[gi][st], acc[gi][1]);
                     }
                 }
             } else {
