@@ -665,11 +665,18 @@ __global__ void __launch_bounds__(kBlock) k_m2l_sep(Geom g, DevTables t, int nbr
 
 // ---------------------------------------------------------------- STEP 4
 template <typename ET>
-__global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int* __restrict__ pc_parent,
+__global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, int nentry, const int* __restrict__ pc_parent,
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign, int klev,
                                                   ET* __restrict__ expn_a, Multi mu) {
+    // the entries of one parent are consecutive: the block of the first
+    // reads the parent's element once and writes every child of the run;
+    // the blocks of the others have nothing to do
     const int entry = blockIdx.y;
+    auto same = [&](int a, int b) {
+        return pc_parent[a] == pc_parent[b] && (!mu.pp || mu.tprob[a] == mu.tprob[b]);
+    };
+    if (entry > 0 && same(entry - 1, entry)) return;
     ET* __restrict__ expn = expn_a;
     const double2* w1 = t.wms + (size_t)(klev - 1) * t.npw;
     if (mu.pp) {
@@ -684,7 +691,17 @@ __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, const int
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
     const double2 v = ldE(expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e]);
-    stE(expn + ((size_t)pc_child[entry] * 2 + 1) * tot + e, cmul(v, ms_phase(w1, pc_sign[entry], k1, k2, k3, g)));
+    // ms_phase per child from the two x-y products (a conjugate is exact)
+    const double2 wx = g.px > 1 ? w1[k1] : make_double2(1.0, 0.0);
+    const double2 wy = g.py > 1 ? w1[k2] : make_double2(1.0, 0.0);
+    const double2 wz = w1[k3];
+    const double2 pa = cmul(wx, wy), pb = cmul(conjd(wx), wy);
+    for (int j = entry; j < nentry && (j == entry || same(entry, j)); ++j) {
+        const int sg = pc_sign[j];
+        const double2 xy = (sg & 1) ? ((sg & 2) ? conjd(pa) : pb) : ((sg & 2) ? conjd(pb) : pa);
+        const double2 ph = cmul(xy, (sg & 4) ? conjd(wz) : wz);
+        stE(expn + ((size_t)pc_child[j] * 2 + 1) * tot + e, cmul(v, ph));
+    }
 }
 
 // ---------------------------------------------------------------- STEP 5, 2-D
@@ -1087,10 +1104,10 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
         Multi ma = mu;
         if (ma.pp) ma.tprob += a, ma.tklev += a;
         if (t.expn_f32)
-            hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
+            hipLaunchKernelGGL(k_split<float2>, grid, dim3(kBlock), 0, s, g, t, cnt, pc_parent + a, pc_child + a,
                                pc_sign + a, klev, reinterpret_cast<float2*>(expn), ma);
         else
-            hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, pc_parent + a, pc_child + a,
+            hipLaunchKernelGGL(k_split<double2>, grid, dim3(kBlock), 0, s, g, t, cnt, pc_parent + a, pc_child + a,
                                pc_sign + a, klev, expn, ma);
         check(hipGetLastError(), "k_split launch");
     }
