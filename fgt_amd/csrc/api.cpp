@@ -833,6 +833,39 @@ std::vector<Comp> components_on(int ndim, int nd, int ifpgh, double* pot, double
     return c;
 }
 
+// STEP 6 for a list of output components: on the 3-D register-blocked
+// path the components that share their x and y table kinds go in one pass
+// (their x and y sums are formed once; gnd_tens_prod_to_pghloc,
+// bfgt_local.f:756-920, shares the partial sums the same way), elsewhere
+// one pass per component.
+void launch_direct_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask, const int* tgt, const int* lev,
+                         const int* off, const int* src, const int* code, const int* flags, const double* fvals,
+                         const std::vector<Comp>& comps, hipStream_t st) {
+    std::vector<fgt::ZOut> groups;
+    std::vector<fgt::Kinds> gkd;
+    for (const Comp& c : comps) {
+        size_t k = 0;
+        while (k < groups.size() && !(gkd[k].x == c.kd.x && gkd[k].y == c.kd.y && groups[k].nz < 3)) ++k;
+        if (k == groups.size()) {
+            groups.push_back(fgt::ZOut{});
+            gkd.push_back(c.kd);
+        }
+        fgt::ZOut& z = groups[k];
+        z.kz[z.nz] = c.kd.z;
+        z.od[z.nz] = c.od;
+        ++z.nz;
+    }
+    for (size_t k = 0; k < groups.size(); ++k) {
+        if (fgt::launch_direct_3d_z(g, t, ntask, tgt, lev, off, src, code, flags, fvals, gkd[k], groups[k], st)) {
+            hcheck(hipGetLastError(), "k_direct3 launch");
+            continue;
+        }
+        for (int q = 0; q < groups[k].nz; ++q)
+            fgt::launch_direct(g, t, ntask, tgt, lev, off, src, code, flags, fvals,
+                               fgt::Kinds{gkd[k].x, gkd[k].y, groups[k].kz[q]}, groups[k].od[q], st);
+    }
+}
+
 std::vector<Comp> components(fgt_solver& S, double* pot_override = nullptr) {
     const fgt::Plan& P = S.plan;
     if (pot_override) return components_on(P.ndim, P.nd, 1, pot_override, nullptr, nullptr);
@@ -881,10 +914,9 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
     }
     const std::vector<Comp> comps = components(S, ov ? ov->pot : nullptr);
     auto near_field = [&](hipStream_t sn) {
-        for (const Comp& c : comps)
-            fgt::launch_direct(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
-                               S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
-                               S.d_dir_flags.as<int>(), fv, c.kd, c.od, sn);
+        launch_direct_comps(g, t, (int)P.dir_tgt.size(), S.d_dir_tgt.as<int>(), S.d_dir_lev.as<int>(),
+                            S.d_dir_off.as<int>(), S.d_dir_src.as<int>(), S.d_dir_code.as<int>(),
+                            S.d_dir_flags.as<int>(), fv, comps, sn);
     };
     const bool fork = fork_near && P.do_pw && !P.dir_tgt.empty();
     if (fork) {
@@ -2070,9 +2102,8 @@ struct Refiner {
             a4.upload(dsrc);
             a5.upload(dcode);
             a6.upload(dfl);
-            for (const Comp& c : comps)
-                fgt::launch_direct(g, tr, (int)dt.size(), a1.as<int>(), a2.as<int>(), a3.as<int>(), a4.as<int>(),
-                                   a5.as<int>(), a6.as<int>(), S.d_fvals.as<double>(), c.kd, c.od, st);
+            launch_direct_comps(g, tr, (int)dt.size(), a1.as<int>(), a2.as<int>(), a3.as<int>(), a4.as<int>(),
+                                a5.as<int>(), a6.as<int>(), S.d_fvals.as<double>(), comps, st);
         }
         hcheck(hipStreamSynchronize(st), "refinement evaluation");
         hcheck(hipMemcpy(np.data(), d_p.p, d_p.bytes, hipMemcpyDeviceToHost), "download");

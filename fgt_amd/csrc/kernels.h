@@ -269,6 +269,19 @@ void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
                       const double* fvals, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu = Multi{});
+// Output components that differ only in their z table kind (up to three:
+// value, first, second derivative) share one STEP 6 pass: the x and y sums
+// are formed once and each z kind contracts them into its own output.
+struct ZOut {
+    int nz;
+    int kz[3];
+    OutDesc od[3];
+};
+// 3-D register-blocked path only (norder 8, 12, 16, with flags); false
+// when not covered (nothing launched)
+bool launch_direct_3d_z(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
+                        const int* off, const int* src, const int* code, const int* flags,
+                        const double* fvals, Kinds kd, const ZOut& zo, hipStream_t s);
 
 // Device workspace of the no-allocation entry point: a file-scope device
 // array of kScratchBytes per GPU.  scratch_region() returns its address on

@@ -41,13 +41,14 @@ int direct_runs(const Multi& mu) {
 template <int N>
 constexpr int direct3_block() { return (N * N + 63) / 64 * 64; }
 
-template <int N>
-__global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3(Geom g, const double* __restrict__ loctab_a, int nls,
-                                                const int* __restrict__ tgt, const int* __restrict__ lev,
-                                                const int* __restrict__ off, const int* __restrict__ src,
-                                                const int* __restrict__ code, const int* __restrict__ flags,
-                                                const double* __restrict__ fvals_a, size_t ks, Kinds kd,
-                                                OutDesc od, Multi mu, int runs) {
+// NZ output components that differ only in their z kind (ZOut) share the x
+// and y sums; NZ = 1 is the one-component kernel (od = zo.od[0]).
+template <int N, int NZ>
+__global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? (NZ == 1 ? 3 : 2) : 2)
+    k_direct3(Geom g, const double* __restrict__ loctab_a, int nls, const int* __restrict__ tgt,
+              const int* __restrict__ lev, const int* __restrict__ off, const int* __restrict__ src,
+              const int* __restrict__ code, const int* __restrict__ flags, const double* __restrict__ fvals_a,
+              size_t ks, Kinds kd, ZOut zo, Multi mu, int runs) {
     extern __shared__ double dsm[];   // N^3 doubles (transpose buffer)
     constexpr int NN = N * N;
     const int task = runs ? xcd_run_task<8>(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -57,13 +58,15 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
         const ProbPtrs& q = mu.pp[mu.tprob[task]];
         loctab = q.loctab;
         fvals = q.fvals;
-        od.p = q.pot;
+        zo.od[0].p = q.pot;   // batches evaluate the potential only
     }
     const int tb = tgt[task], l = lev[task];
     // table kind per axis (value / first / second derivative): stride ks
     const CTab1 tabx = ctab(loctab) + (long long)(kd.x * ks + (size_t)l * nls * NN);
     const CTab1 taby = ctab(loctab) + (long long)(kd.y * ks + (size_t)l * nls * NN);
-    const CTab1 tabz = ctab(loctab) + (long long)(kd.z * ks + (size_t)l * nls * NN);
+    CTab1 tabz[NZ];
+#pragma unroll
+    for (int q = 0; q < NZ; ++q) tabz[q] = ctab(loctab) + (long long)(zo.kz[q] * ks + (size_t)l * nls * NN);
     const int tid = threadIdx.x;
     const bool act = tid < NN;
     // x stage: thread = (j2, j3); y stage: thread = (k1, j3); z stage: thread = (k1, k2)
@@ -71,9 +74,13 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
     const int nd = g.nd;
     const int j0 = off[task], j1e = off[task + 1];
     for (int ind = 0; ind < nd; ++ind) {
-        double X[N], Y[N], acc[N];
+        double X[N], Y[N], acc[NZ][N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) X[i] = Y[i] = acc[i] = 0.0;
+        for (int i = 0; i < N; ++i) {
+            X[i] = Y[i] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NZ; ++q) acc[q][i] = 0.0;
+        }
         for (int j = j0; j < j1e; ++j) {
             const int sb = src[j], c = code[j], fl = flags[j];
             const CTab1 Ax = tabx + (long long)(c & 255) * NN;
@@ -127,8 +134,7 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
 #pragma unroll
                 for (int i = 0; i < N; ++i) X[i] = 0.0;
                 if (fl & 2) {
-                    // ---- z: acc[k3] += sum_j3 Az[k3][j3] Y[k1, k2, j3]
-                    const CTab1 Az = tabz + (long long)((c >> 16) & 255) * NN;
+                    // ---- z: acc[q][k3] += sum_j3 Az_q[k3][j3] Y[k1, k2, j3]
                     __syncthreads();
                     if (act) {
 #pragma unroll
@@ -140,11 +146,15 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
 #pragma unroll
                         for (int j3 = 0; j3 < N; ++j3) yv[j3] = dsm[(j3 * N + bq) * N + a];
 #pragma unroll
-                        for (int k3 = 0; k3 < N; ++k3) {
-                            double s = acc[k3];
+                        for (int q = 0; q < NZ; ++q) {
+                            const CTab1 Az = tabz[q] + (long long)((c >> 16) & 255) * NN;
 #pragma unroll
-                            for (int j3 = 0; j3 < N; ++j3) s += Az[k3 * N + j3] * yv[j3];
-                            acc[k3] = s;
+                            for (int k3 = 0; k3 < N; ++k3) {
+                                double s = acc[q][k3];
+#pragma unroll
+                                for (int j3 = 0; j3 < N; ++j3) s += Az[k3 * N + j3] * yv[j3];
+                                acc[q][k3] = s;
+                            }
                         }
                     }
 #pragma unroll
@@ -154,8 +164,12 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
         }
         if (act) {
 #pragma unroll
-            for (int k3 = 0; k3 < N; ++k3)
-                od.p[((size_t)tb * g.npbox + a + N * (bq + N * k3)) * od.ns + od.coff + ind] += acc[k3];
+            for (int q = 0; q < NZ; ++q) {
+                const OutDesc& od = zo.od[q];
+#pragma unroll
+                for (int k3 = 0; k3 < N; ++k3)
+                    od.p[((size_t)tb * g.npbox + a + N * (bq + N * k3)) * od.ns + od.coff + ind] += acc[q][k3];
+            }
         }
         __syncthreads();
     }
@@ -163,13 +177,57 @@ __global__ void __launch_bounds__(direct3_block<N>(), N == 16 ? 3 : 2) k_direct3
 
 template <int N>
 bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev, const int* off,
-                 const int* src, const int* code, const int* flags, const double* fvals, Kinds kd, OutDesc od,
+                 const int* src, const int* code, const int* flags, const double* fvals, Kinds kd, const ZOut& zo,
                  hipStream_t s, const Multi& mu) {
     const size_t lds = sizeof(double) * (size_t)N * N * N;
     if (lds > t.lds_limit) return false;
-    hipLaunchKernelGGL(k_direct3<N>, dim3(ntask), dim3(direct3_block<N>()), lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src, code,
-                       flags, fvals, t.loctab_ks, kd, od, mu, direct_runs(mu));
-    return true;
+    const dim3 grid(ntask), block(direct3_block<N>());
+    const int runs = direct_runs(mu);
+    switch (zo.nz) {
+        case 1:
+            hipLaunchKernelGGL((k_direct3<N, 1>), grid, block, lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src,
+                               code, flags, fvals, t.loctab_ks, kd, zo, mu, runs);
+            return true;
+        case 2:
+            hipLaunchKernelGGL((k_direct3<N, 2>), grid, block, lds, s, g, t.loctab, t.loc_slots, tgt, lev, off, src,
+                               code, flags, fvals, t.loctab_ks, kd, zo, mu, runs);
+            return true;
+        case 3:
+            if constexpr (N == 16) {
+                // three z kinds exceed the registers at norder 16 (spills):
+                // two passes, the first sharing two of them
+                ZOut a = zo, b{};
+                a.nz = 2;
+                b.nz = 1;
+                b.kz[0] = zo.kz[2];
+                b.od[0] = zo.od[2];
+                hipLaunchKernelGGL((k_direct3<N, 2>), grid, block, lds, s, g, t.loctab, t.loc_slots, tgt, lev, off,
+                                   src, code, flags, fvals, t.loctab_ks, kd, a, mu, runs);
+                hipLaunchKernelGGL((k_direct3<N, 1>), grid, block, lds, s, g, t.loctab, t.loc_slots, tgt, lev, off,
+                                   src, code, flags, fvals, t.loctab_ks, kd, b, mu, runs);
+            } else {
+                hipLaunchKernelGGL((k_direct3<N, 3>), grid, block, lds, s, g, t.loctab, t.loc_slots, tgt, lev, off,
+                                   src, code, flags, fvals, t.loctab_ks, kd, zo, mu, runs);
+            }
+            return true;
+        default:
+            return false;
+    }
+}
+
+bool direct3_covered(const Geom& g, const DevTables& t, int ntask, const int* flags) {
+    return g.ndim == 3 && ntask > 0 && flags && (t.n == 8 || t.n == 12 || t.n == 16);
+}
+
+bool direct3_dispatch(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev, const int* off,
+                      const int* src, const int* code, const int* flags, const double* fvals, Kinds kd,
+                      const ZOut& zo, hipStream_t s, const Multi& mu) {
+    switch (t.n) {
+        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, zo, s, mu);
+        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, zo, s, mu);
+        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, zo, s, mu);
+        default: return false;
+    }
 }
 
 }  // namespace
@@ -177,13 +235,19 @@ bool try_direct3(const Geom& g, const DevTables& t, int ntask, const int* tgt, c
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
                       const double* fvals, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu) {
-    if (g.ndim != 3 || ntask <= 0 || !flags) return false;
-    switch (t.n) {
-        case 8: return try_direct3<8>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
-        case 12: return try_direct3<12>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
-        case 16: return try_direct3<16>(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, od, s, mu);
-        default: return false;
-    }
+    if (!direct3_covered(g, t, ntask, flags)) return false;
+    ZOut zo{};
+    zo.nz = 1;
+    zo.kz[0] = kd.z;
+    zo.od[0] = od;
+    return direct3_dispatch(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, zo, s, mu);
+}
+
+bool launch_direct_3d_z(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
+                        const int* off, const int* src, const int* code, const int* flags,
+                        const double* fvals, Kinds kd, const ZOut& zo, hipStream_t s) {
+    if (!direct3_covered(g, t, ntask, flags) || zo.nz < 1 || zo.nz > 3) return false;
+    return direct3_dispatch(g, t, ntask, tgt, lev, off, src, code, flags, fvals, kd, zo, s, Multi{});
 }
 
 }  // namespace fgt

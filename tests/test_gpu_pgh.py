@@ -56,17 +56,19 @@ def test_dropin_pgh_matches_reference(name):
     assert np.array_equal(grad2, g0 + grad)
 
 
-@pytest.mark.parametrize("iperiod,delta", [(1, 1e-3), (0, 2e-3)])
-def test_pgh_3d_vs_oracle_and_closed_form(iperiod, delta):
-    print(f"[gpu] pgh 3-D n=8 iperiod={iperiod} delta={delta}", flush=True)
+@pytest.mark.parametrize("n,iperiod,delta", [(8, 1, 1e-3), (8, 0, 2e-3), (12, 0, 5e-3), (16, 0, 5e-3)])
+def test_pgh_3d_vs_oracle_and_closed_form(n, iperiod, delta):
+    """STEP 6 forms the components that share x and y table kinds in one
+    pass (three z kinds at norder 8 / 12, two plus one at 16)."""
+    print(f"[gpu] pgh 3-D n={n} iperiod={iperiod} delta={delta}", flush=True)
     gauss = (np.array([[0.1, -0.2, 0.3]]), np.array([[2e-2, 1.5e-2, 3e-2]]), np.array([1.0]))
-    tr = fgt_amd.vol_tree_build(3, 8, 0, iperiod, 1e-6, gauss_source(*gauss))
+    tr = fgt_amd.vol_tree_build(3, n, 0, iperiod, 1e-6, gauss_source(*gauss))
     s = fgt_amd.Solver.from_tree(tr, delta, 1e-6)
     s.set_ifpgh(3)
     s.set_fvals(tr.fvals)
     s.execute()
     pot, grad, hess = s.get_pot(), s.get_grad(), s.get_hess()
-    rp, rg, rh, info = oracle.boxfgt_pgh(1, 3, delta, 1e-6, 0, iperiod, 8, tr.nboxes, tr.nlevels, tr.itree,
+    rp, rg, rh, info = oracle.boxfgt_pgh(1, 3, delta, 1e-6, 0, iperiod, n, tr.nboxes, tr.nlevels, tr.itree,
                                          tr.iptr, tr.centers, tr.boxsize, tr.fvals, 3)
     assert 1 <= info.npwlevel < tr.nlevels
     assert rel_l2(pot, rp) < 1e-12
@@ -74,7 +76,7 @@ def test_pgh_3d_vs_oracle_and_closed_form(iperiod, delta):
         assert rel_l2(grad[:, :, k], rg[:, :, k]) < 1e-12, f"grad {k}"
     for k in range(6):
         assert rel_l2(hess[:, :, k], rh[:, :, k]) < 1e-11, f"hess {k}"
-    pts = leaf_points(3, 8, 0, tr.centers, tr.boxsize, tr.level, tr.leaves)
+    pts = leaf_points(3, n, 0, tr.centers, tr.boxsize, tr.level, tr.leaves)
     go, ho = pgh_orders(3)
     for k, o in enumerate(go):
         assert rel_l2(grad[tr.leaves][:, :, k], exact_gauss_deriv(pts, gauss, delta, iperiod, o)) < 1e-3
