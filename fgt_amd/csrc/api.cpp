@@ -1197,6 +1197,11 @@ struct fgt_batch {
     std::vector<hipStream_t> streams;  // [0] stages, [1] near field, [2..] loose problems
     hipEvent_t ev_fork = nullptr, ev_near = nullptr;
     std::vector<hipEvent_t> ev_loose;
+    // further class chains beside streams[0] (classes are independent: each
+    // chain runs its classes' STEPs 1-5 in order), their join events, and
+    // one near-field event per class (STEP 5 of a class waits for its STEP 6)
+    std::vector<hipStream_t> chains;
+    std::vector<hipEvent_t> ev_chain, ev_cnear;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1211,7 +1216,10 @@ struct fgt_batch {
     ~fgt_batch() {
         drop_graph();
         for (auto s : streams) (void)hipStreamDestroy(s);
+        for (auto s : chains) (void)hipStreamDestroy(s);
         for (auto e : ev_loose) (void)hipEventDestroy(e);
+        for (auto e : ev_chain) (void)hipEventDestroy(e);
+        for (auto e : ev_cnear) (void)hipEventDestroy(e);
         for (auto e : {ev_fork, ev_near, t0, t1})
             if (e) (void)hipEventDestroy(e);
     }
@@ -1539,6 +1547,55 @@ struct StageEvents {
 
 // Enqueue one whole batch transform (graph capture records this sequence);
 // prof != nullptr: record events around every stage of every class.
+// Class chains of a batch: the classes are dealt to the chains largest
+// first, each to the chain with the least work so far (work ~ output nodes
+// of the members), so the small classes run beside the large ones instead
+// of after them.
+constexpr int kClassChains = 2;
+
+std::vector<int> class_chains(const fgt_batch& B) {
+    const size_t nc = B.classes.size();
+    std::vector<double> cost(nc, 0.0);
+    for (size_t c = 0; c < nc; ++c)
+        for (int i : B.classes[c]->members) {
+            const fgt::Plan& P = B.solvers[i]->plan;
+            cost[c] += (double)P.nleaves * P.npbox * P.nd;
+        }
+    std::vector<size_t> order(nc);
+    for (size_t c = 0; c < nc; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<int> chain(nc, 0);
+    std::vector<double> load(kClassChains, 0.0);
+    for (size_t c : order) {
+        int k = 0;
+        for (int j = 1; j < kClassChains; ++j)
+            if (load[j] < load[k]) k = j;
+        chain[c] = k;
+        load[k] += cost[c];
+    }
+    return chain;
+}
+
+// The chain streams and events (created before any graph capture: the
+// capture records work, not object creation)
+void batch_chain_objects(fgt_batch& B) {
+    if (B.chains.empty()) {
+        for (int k = 1; k < kClassChains; ++k) {
+            hipStream_t s2;
+            hcheck(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "hipStreamCreate");
+            B.chains.push_back(s2);
+            hipEvent_t e;
+            hcheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            B.ev_chain.push_back(e);
+        }
+    }
+    while (B.ev_cnear.size() < B.classes.size()) {
+        hipEvent_t e;
+        hcheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        B.ev_cnear.push_back(e);
+    }
+}
+
 void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     hipStream_t st = B.streams[0], sn = B.streams[1];
     const size_t nc = B.classes.size();
@@ -1546,6 +1603,13 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
         prof->ev.assign(12 * nc, nullptr);
         for (auto& e : prof->ev) hcheck(hipEventCreate(&e), "hipEventCreate");
     }
+    if ((int)B.chains.size() != kClassChains - 1 || B.ev_cnear.size() < nc)
+        throw std::logic_error("batch chain streams not set up");
+    std::vector<hipStream_t> ch{st};
+    ch.insert(ch.end(), B.chains.begin(), B.chains.end());
+    // a profiled execute keeps every class on one chain, so the per-stage
+    // events time each launch alone
+    const std::vector<int> chain = prof ? std::vector<int>(nc, 0) : class_chains(B);
     auto mark = [&](size_t c, int step, int end, hipStream_t s) {
         if (prof) hcheck(hipEventRecord(prof->ev[12 * c + 2 * step + end], s), "hipEventRecord");
     };
@@ -1553,11 +1617,13 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     hcheck(hipEventRecord(B.ev_fork, st), "hipEventRecord");
     for (size_t k = 1; k < B.streams.size(); ++k)
         hcheck(hipStreamWaitEvent(B.streams[k], B.ev_fork, 0), "hipStreamWaitEvent");
+    for (hipStream_t s2 : B.chains) hcheck(hipStreamWaitEvent(s2, B.ev_fork, 0), "hipStreamWaitEvent");
     // near field of every class beside STEPs 1-4
     for (size_t c = 0; c < nc; ++c) {
         mark(c, 5, 0, sn);
         class_step6(*B.classes[c], sn);
         mark(c, 5, 1, sn);
+        hcheck(hipEventRecord(B.ev_cnear[c], sn), "hipEventRecord");
     }
     hcheck(hipEventRecord(B.ev_near, sn), "hipEventRecord");
     // problems off the class path, spread over the side streams (largest first)
@@ -1568,25 +1634,33 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     }
     for (size_t c = 0; c < nc; ++c) {
         const BClass& C = *B.classes[c];
-        mark(c, 0, 0, st);
-        class_step1(C, st);
-        mark(c, 0, 1, st);
-        mark(c, 1, 0, st);
-        class_step2(C, st);
-        mark(c, 1, 1, st);
-        mark(c, 2, 0, st);
-        class_step3(C, st);
-        mark(c, 2, 1, st);
-        mark(c, 3, 0, st);
-        class_step4(C, st);
-        mark(c, 3, 1, st);
+        hipStream_t s = ch[chain[c]];
+        mark(c, 0, 0, s);
+        class_step1(C, s);
+        mark(c, 0, 1, s);
+        mark(c, 1, 0, s);
+        class_step2(C, s);
+        mark(c, 1, 1, s);
+        mark(c, 2, 0, s);
+        class_step3(C, s);
+        mark(c, 2, 1, s);
+        mark(c, 3, 0, s);
+        class_step4(C, s);
+        mark(c, 3, 1, s);
+    }
+    // STEP 5 of a class after its near field (both add into its potentials)
+    for (size_t c = 0; c < nc; ++c) {
+        hipStream_t s = ch[chain[c]];
+        hcheck(hipStreamWaitEvent(s, B.ev_cnear[c], 0), "hipStreamWaitEvent");
+        mark(c, 4, 0, s);
+        class_step5(*B.classes[c], s);
+        mark(c, 4, 1, s);
+    }
+    for (size_t k = 0; k < B.chains.size(); ++k) {
+        hcheck(hipEventRecord(B.ev_chain[k], B.chains[k]), "hipEventRecord");
+        hcheck(hipStreamWaitEvent(st, B.ev_chain[k], 0), "hipStreamWaitEvent");
     }
     hcheck(hipStreamWaitEvent(st, B.ev_near, 0), "hipStreamWaitEvent");
-    for (size_t c = 0; c < nc; ++c) {
-        mark(c, 4, 0, st);
-        class_step5(*B.classes[c], st);
-        mark(c, 4, 1, st);
-    }
     for (size_t k = 2; k < B.streams.size(); ++k) {
         hcheck(hipEventRecord(B.ev_loose[k - 2], B.streams[k]), "hipEventRecord");
         hcheck(hipStreamWaitEvent(st, B.ev_loose[k - 2], 0), "hipStreamWaitEvent");
@@ -1617,6 +1691,7 @@ void batch_autotune(fgt_batch& B, int reps) {
     hipStream_t st = B.streams[0];
     hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     B.drop_graph();
+    batch_chain_objects(B);
     batch_enqueue(B);   // expansions and potentials valid for the stage timings
     hcheck(hipStreamSynchronize(st), "hipStreamSynchronize");
     for (auto& cp : B.classes) {
@@ -2729,6 +2804,7 @@ int fgt_batch_profile(fgt_batch* b) {
     return guarded([&] {
         if (!b) throw std::invalid_argument("null batch");
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        batch_chain_objects(*b);
         StageEvents prof;
         batch_enqueue(*b, &prof);
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -2765,6 +2841,7 @@ int fgt_batch_execute(fgt_batch* b) {
         hipStream_t st = b->streams[0];
         // earlier work on any stream (uploads, other executes) comes first
         hcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        batch_chain_objects(*b);
         if (b->use_graph && !b->gexec) {
             hcheck(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
