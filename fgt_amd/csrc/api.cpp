@@ -1550,8 +1550,9 @@ struct StageEvents {
 // Class chains of a batch: the classes are dealt to the chains largest
 // first, each to the chain with the least work so far (work ~ output nodes
 // of the members), so the small classes run beside the large ones instead
-// of after them.
-constexpr int kClassChains = 2;
+// of after them.  Headline batch (133 problems, 9 classes): one chain
+// 39.6 ms, two 36.9, three 35.6, four 37.1.
+constexpr int kClassChains = 3;
 
 std::vector<int> class_chains(const fgt_batch& B) {
     const size_t nc = B.classes.size();
