@@ -202,20 +202,33 @@ def batch_roofline(probs, prof):
                      "m2l_ept": c["m2l_ept"],
                      "stage_ms": [round(v, 4) for v in ms], "step15_gflop": round(flops / 1e9, 3),
                      "step3_gb": round(byts / 1e9, 3)})
-        for stg, kern, kind, work in ((4, "k_pw2pot3", "flops", flops), (0, "k_p2pwF3+k_p2pw3", "flops", flops),
-                                      (2, M2L_KERNEL[c["m2l_brick"]], "bytes", byts)):
+        for stg, kern, kind, work, xs in ((4, "k_pw2pot3", "flops", flops, c["xs5"]),
+                                          (0, "k_p2pwF3+k_p2pw3", "flops", flops, c["xs1"]),
+                                          (2, M2L_KERNEL[c["m2l_brick"]], "bytes", byts, -1)):
             if ms[stg] > 0 and work > 0 and (best is None or ms[stg] > best[0]):
-                best = (ms[stg], kern, kind, work, n, prec, c["npw_cap"])
+                best = (ms[stg], kern, kind, work, n, prec, c["npw_cap"], xs)
     if best is None:
         return None, rows
-    ms, kern, kind, work, n, prec, cap = best
+    ms, kern, kind, work, n, prec, cap, xs = best
+    arith = None
     if kind == "flops":
-        ach, peak, unit, bound = work / (ms * 1e-3) / 1e12, PEAK_MODE_TFLOPS[prec], "TFLOP/s", "mfma"
+        # the peak of the arithmetic the launch group runs: its x stage (the
+        # bulk of the flops) on the VALU in FP64 (xs 0), on the FP64 matrix
+        # cores (xs 1) or on the reduced-precision matrix cores of the mode
+        # (xs 2); FP64 vector and matrix peaks are equal on MI355X
+        if xs in (0, 1):
+            peak, arith = PEAK_FP64_TFLOPS, "FP64 x stage (" + ("VALU" if xs == 0 else "matrix cores") + ")"
+        else:
+            peak, arith = PEAK_MODE_TFLOPS[prec], f"x stage on the precision-mode-{prec} matrix cores"
+        ach, unit, bound = work / (ms * 1e-3) / 1e12, "TFLOP/s", "mfma"
     else:
         ach, peak, unit, bound = work / (ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
-    return {"kernel": kern, "launch_group": f"norder {n}, precision mode {prec}, npw <= {cap}",
-            "bound": bound, "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 4),
-            "ms": round(ms, 4), "algorithmic": work, "traffic": None}, rows
+    out = {"kernel": kern, "launch_group": f"norder {n}, precision mode {prec}, npw <= {cap}",
+           "bound": bound, "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 4),
+           "ms": round(ms, 4), "algorithmic": work, "traffic": None}
+    if arith:
+        out["arithmetic"] = arith
+    return out, rows
 
 
 def headline(probs, steps, warmup, mode, drop_in=True):
