@@ -82,10 +82,10 @@ def stage_model(st, n, nd=1):
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed PMC summary
-    (lines `traffic_bytes <kernel> = <bytes>`), or None."""
-    path = os.path.join(ROOT, "profiles", "c2_boxfgt_fp64_hbm_traffic.md")
+def measured_traffic(kernel, summary="c2_boxfgt_fp64_hbm_traffic.md"):
+    """HBM bytes per launch of `kernel` from a committed PMC summary under
+    profiles/ (lines `traffic_bytes <kernel> = <bytes>`), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         with open(path) as f:
             for line in f:
@@ -223,9 +223,12 @@ def batch_roofline(probs, prof):
         ach, unit, bound = work / (ms * 1e-3) / 1e12, "TFLOP/s", "mfma"
     else:
         ach, peak, unit, bound = work / (ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", "hbm"
+    # HBM bytes of the group per step from the committed PMC summary of
+    # the headline batch (null for a group that summary does not cover)
+    traffic = measured_traffic(f"{kern}/n{n}/prec{prec}/npw{cap}", "headline_batch_hbm_traffic.md")
     out = {"kernel": kern, "launch_group": f"norder {n}, precision mode {prec}, npw <= {cap}",
            "bound": bound, "achieved": round(ach, 3), "peak": peak, "unit": unit, "frac": round(ach / peak, 4),
-           "ms": round(ms, 4), "algorithmic": work, "traffic": None}
+           "ms": round(ms, 4), "algorithmic": work, "traffic": traffic}
     if arith:
         out["arithmetic"] = arith
     return out, rows
