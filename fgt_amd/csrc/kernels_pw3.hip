@@ -542,6 +542,26 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
         ckk[t] = cval[t] ? c - N * cpart[t] : 0;
     }
     double* __restrict__ sffd = reinterpret_cast<double*>(sff);
+    // FP32 with one column tile (norder 8): the table operands of every K
+    // step are converted once for all row tiles (registers: the largest
+    // plane-wave count's steps, uniform guards past npw)
+    constexpr bool HOIST = XS == 2 && NT == 1;
+    constexpr int KSH = HOIST ? kLowpMaxPw / 4 : 1;
+    float hb0[KSH], hb1[KSH];
+    if constexpr (HOIST) {
+        const int ks = (npw + 3) >> 2;
+#pragma unroll
+        for (int st = 0; st < KSH; ++st) {
+            if (st < ks) {
+                const int k1 = 4 * st + lq;
+                const bool kv = k1 < npw;
+                const double2 u = UX[(kv ? k1 : npw - 1) * N + ckk[0]];
+                const bool ok = kv && cval[0];
+                hb0[st] = ok ? bcol(u, cpart[0], 0) : 0.f;
+                hb1[st] = ok ? bcol(u, cpart[0], 1) : 0.f;
+            }
+        }
+    }
     // one row tile (16 frequencies k2) at a time, two accumulator chains per
     // column tile
     for (int mt = 0; mt < MT; ++mt) {
@@ -549,7 +569,34 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
         flt4 acc[NT][2];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = flt4{0.f, 0.f, 0.f, 0.f};
-        if constexpr (XS == 2) {
+        if constexpr (HOIST) {
+            // E rows in chunks of four K steps, the next chunk loaded before
+            // the current one's products (chunks unrolled to the largest
+            // plane-wave count, uniform guards past npw)
+            const int ks = (npw + 3) >> 2;
+            constexpr int NCH = KSH / 4;
+            float2 e[2][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) e[0][j] = Ek[(size_t)min(4 * j + lq, npw - 1) * npw + k2r];
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                if (4 * c < ks) {
+                    if (c + 1 < NCH && 4 * (c + 1) < ks) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            e[(c + 1) & 1][j] = Ek[(size_t)min(4 * (4 * (c + 1) + j) + lq, npw - 1) * npw + k2r];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (4 * c + j < ks) {
+                            acc[0][0] = mfma32(e[c & 1][j].x, hb0[4 * c + j], acc[0][0]);
+                            acc[0][1] = mfma32(e[c & 1][j].y, hb1[4 * c + j], acc[0][1]);
+                        }
+                    }
+                }
+            }
+        } else if constexpr (XS == 2) {
             // FP32: K steps in chunks of KC, the next chunk's operand row and
             // table entries loaded (from valid clamped addresses, zeroed past
             // npw) before the current chunk's products, into two register sets
