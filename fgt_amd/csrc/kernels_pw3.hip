@@ -371,23 +371,38 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     const int rb = 2 * (part * nround / kFusedParts), re = 2 * ((part + 1) * nround / kFusedParts);
     for (int ind = 0; ind < nd; ++ind) {
         ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
+        // norder 8: one wave per group g = (sx, sy) of children; the values
+        // of its (up to two) children -- the z sign tells them apart -- stay
+        // in registers for every round
+        double frg[2][N];
+        bool has0 = false, has1 = false;
+        if constexpr (N * N == 64) {
+            for (int c = 0; c < nk; ++c) {
+                const int sg = ksg[8 * task + c];
+                if ((sg & 3) != wave) continue;
+                const int b = kids[8 * task + c];
+                const double* __restrict__ fb = fvals + ((size_t)b * g.npbox + lane) * nd + ind;
+                if ((sg >> 2) & 1) {
+                    has1 = true;
+#pragma unroll
+                    for (int m3 = 0; m3 < N; ++m3) frg[1][m3] = fb[(size_t)N * N * m3 * nd];
+                } else {
+                    has0 = true;
+#pragma unroll
+                    for (int m3 = 0; m3 < N; ++m3) frg[0][m3] = fb[(size_t)N * N * m3 * nd];
+                }
+            }
+        }
         for (int r = rb; r < re; r += 2) {
             // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
             if constexpr (N * N == 64) {
-                // one wave per group g = (sx, sy): its (up to two) children
                 const int q = wave;
                 double2 H[2];
                 H[0] = H[1] = make_double2(0.0, 0.0);
-                for (int c = 0; c < nk; ++c) {
-                    const int sg = ksg[8 * task + c];
-                    if ((sg & 3) != q) continue;
-                    const int b = kids[8 * task + c];
-                    const int vz = 1 + ((sg >> 2) & 1);
-                    const CTab2 Tz = T + base0 + vz * per;
-                    double fr[N];
 #pragma unroll
-                    for (int m3 = 0; m3 < N; ++m3)
-                        fr[m3] = fvals[((size_t)b * g.npbox + lane + N * N * m3) * nd + ind];
+                for (int z = 0; z < 2; ++z) {
+                    if (!(z ? has1 : has0)) continue;
+                    const CTab2 Tz = T + base0 + (1 + z) * per;
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
                         const int k3 = min(r + s, nh - 1);
@@ -395,8 +410,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 #pragma unroll
                         for (int m3 = 0; m3 < N; ++m3) {
                             const double2 t = Tz[k3 * N + m3];
-                            ar += t.x * fr[m3];
-                            ai += t.y * fr[m3];
+                            ar += t.x * frg[z][m3];
+                            ai += t.y * frg[z][m3];
                         }
                         H[s].x += ar;
                         H[s].y += ai;
