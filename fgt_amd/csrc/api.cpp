@@ -1668,22 +1668,27 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     }
 }
 
-// Device time of f(st) over reps runs (ms per run)
+// Device time of f(st): the fastest of reps runs after a warm-up (a run
+// slowed by other work on the device does not decide a variant)
 template <class F>
 float time_ms(hipStream_t st, int reps, F&& f) {
-    hipEvent_t a, b;
-    hcheck(hipEventCreate(&a), "hipEventCreate");
-    hcheck(hipEventCreate(&b), "hipEventCreate");
+    std::vector<hipEvent_t> ev(reps + 1, nullptr);
+    for (auto& e : ev) hcheck(hipEventCreate(&e), "hipEventCreate");
     f(st);   // warm-up
-    hcheck(hipEventRecord(a, st), "hipEventRecord");
-    for (int r = 0; r < reps; ++r) f(st);
-    hcheck(hipEventRecord(b, st), "hipEventRecord");
-    hcheck(hipEventSynchronize(b), "hipEventSynchronize");
-    float ms = 0;
-    hcheck(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return ms / reps;
+    hcheck(hipEventRecord(ev[0], st), "hipEventRecord");
+    for (int r = 0; r < reps; ++r) {
+        f(st);
+        hcheck(hipEventRecord(ev[r + 1], st), "hipEventRecord");
+    }
+    hcheck(hipEventSynchronize(ev[reps]), "hipEventSynchronize");
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        float ms = 0;
+        hcheck(hipEventElapsedTime(&ms, ev[r], ev[r + 1]), "hipEventElapsedTime");
+        best = std::min(best, ms);
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    return best;
 }
 
 // Picks, per class, the STEP 3 targets per block and the STEP 1 / STEP 5 x

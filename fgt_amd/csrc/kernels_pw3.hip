@@ -557,10 +557,12 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
         ckk[t] = cval[t] ? c - N * cpart[t] : 0;
     }
     double* __restrict__ sffd = reinterpret_cast<double*>(sff);
-    // FP32 with one column tile (norder 8): the table operands of every K
-    // step are converted once for all row tiles (registers: the largest
-    // plane-wave count's steps, uniform guards past npw)
-    constexpr bool HOIST = XS == 2 && NT == 1;
+    // XS 4 (FP32, one column tile: norder 8, plane-wave counts above 32):
+    // the table operands of every K step are converted once for all row
+    // tiles (registers: the largest plane-wave count's steps, uniform guards
+    // past npw); the registers this takes halve the blocks per CU, which
+    // only the LDS of the larger counts limits to the same number anyway
+    constexpr bool HOIST = XS == 4 && NT == 1;
     constexpr int KSH = HOIST ? kLowpMaxPw / 4 : 1;
     float hb0[KSH], hb1[KSH];
     if constexpr (HOIST) {
@@ -800,7 +802,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             // (all loads issued before the FMAs that use them) so enough reads
             // are in flight to cover HBM latency; lanes past npw read a valid
             // duplicate and discard it.
-            if constexpr (XS == 2 || XS == 3) {
+            if constexpr (XS >= 2) {
                 pw2pot_x_lowp<N, XS>(E, UX, npw, nh, r, wave, lane, sff);
             } else if constexpr (XS == 1) {
                 const int k3 = r + wave;
@@ -1063,8 +1065,17 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     const int xs = x5_variant(N, t);
     if (xs == 2) {
         // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores)
-        if (t.prec == 2) launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
-        else launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+        if (t.prec == 2) {
+            launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+            return true;
+        }
+        if constexpr (N == 8) {
+            if (t.npw > 32) {
+                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+                return true;
+            }
+        }
+        launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
         return true;
     }
     if constexpr (N >= 12) {
