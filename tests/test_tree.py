@@ -135,3 +135,36 @@ def test_source_function_rejects_bad_arguments(test_source):
         args.update(kw)
         with pytest.raises(fgt_amd.FgtError):
             fgt_amd.vol_tree_build_fun(2, 8, 0, 0, 1e-6, fun, dp, ip, **args)
+
+
+def test_python_source_function_builds_the_same_tree(test_source):
+    """A Python function as the source (a ctypes callback with the
+    reference's by-reference convention, called from the builder's threads)
+    gives the tree of the compiled source of the same values."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "tree_f2_l1_eta1.npz"))
+    dp = np.array(z["dpars"])
+    ip = np.array(z["ipars"], np.int32)
+    ndim, ng = int(ip[0]), int(ip[1])
+    c = dp.reshape(ng, 2 * ndim + 1)
+
+    proto = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_double))
+
+    def py_source(nd, xyz, dpars, zpars, ipars, f):
+        x = [xyz[k] for k in range(ndim)]
+        acc = 0.0
+        for g in range(ng):
+            rr = 0.0
+            for k in range(ndim):
+                d = x[k] - c[g, k]
+                rr += d * d / c[g, ndim + k]
+            acc += c[g, 2 * ndim] * np.exp(-rr)
+        for q in range(nd[0]):
+            f[q] = acc * (1.0 + q * np.cos(3.0 * x[0] + q))
+
+    cb = proto(py_source)
+    t = fgt_amd.vol_tree_build_fun(ndim, int(z["norder"]), int(z["ipoly"]), int(z["iperiod"]), float(z["eps"]), cb,
+                                   dp, ip, nd=1, iptype=int(z["iptype"]), eta=float(z["eta"]))
+    np.testing.assert_array_equal(t.itree, z["itree"])
+    np.testing.assert_allclose(t.fvals[:t.nused], z["fvals"], rtol=1e-13, atol=1e-13 * np.abs(z["fvals"]).max())
