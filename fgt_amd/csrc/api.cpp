@@ -833,16 +833,9 @@ std::vector<Comp> components_on(int ndim, int nd, int ifpgh, double* pot, double
     return c;
 }
 
-// STEP 6 for a list of output components: on the 3-D register-blocked
-// path the components that share their x and y table kinds go in one pass
-// (their x and y sums are formed once; gnd_tens_prod_to_pghloc,
-// bfgt_local.f:756-920, shares the partial sums the same way), elsewhere
-// one pass per component.
-void launch_direct_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask, const int* tgt, const int* lev,
-                         const int* off, const int* src, const int* code, const int* flags, const double* fvals,
-                         const std::vector<Comp>& comps, hipStream_t st) {
-    std::vector<fgt::ZOut> groups;
-    std::vector<fgt::Kinds> gkd;
+// Output components grouped by their x and y table kinds (gkd[k]), at most
+// three z kinds per group (groups[k]).
+void group_comps(const std::vector<Comp>& comps, std::vector<fgt::ZOut>& groups, std::vector<fgt::Kinds>& gkd) {
     for (const Comp& c : comps) {
         size_t k = 0;
         while (k < groups.size() && !(gkd[k].x == c.kd.x && gkd[k].y == c.kd.y && groups[k].nz < 3)) ++k;
@@ -855,6 +848,19 @@ void launch_direct_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask,
         z.od[z.nz] = c.od;
         ++z.nz;
     }
+}
+
+// STEP 6 for a list of output components: on the 3-D register-blocked
+// path the components that share their x and y table kinds go in one pass
+// (their x and y sums are formed once; gnd_tens_prod_to_pghloc,
+// bfgt_local.f:756-920, shares the partial sums the same way), elsewhere
+// one pass per component.
+void launch_direct_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask, const int* tgt, const int* lev,
+                         const int* off, const int* src, const int* code, const int* flags, const double* fvals,
+                         const std::vector<Comp>& comps, hipStream_t st) {
+    std::vector<fgt::ZOut> groups;
+    std::vector<fgt::Kinds> gkd;
+    group_comps(comps, groups, gkd);
     for (size_t k = 0; k < groups.size(); ++k) {
         if (fgt::launch_direct_3d_z(g, t, ntask, tgt, lev, off, src, code, flags, fvals, gkd[k], groups[k], st)) {
             hcheck(hipGetLastError(), "k_direct3 launch");
@@ -864,6 +870,20 @@ void launch_direct_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask,
             fgt::launch_direct(g, t, ntask, tgt, lev, off, src, code, flags, fvals,
                                fgt::Kinds{gkd[k].x, gkd[k].y, groups[k].kz[q]}, groups[k].od[q], st);
     }
+}
+
+// STEP 5 (3-D task kernel) for a list of output components: the one-leaf
+// tasks of components that share their x and y table kinds run once per
+// group (g3d_pw2pgh, bfgt_pwrouts.f:906-1000, likewise forms each pair of x
+// and y partial sums once for all the z kinds), the sibling tasks once per
+// component.
+void launch_pw2pot_comps(const fgt::Geom& g, const fgt::DevTables& t, int ntask, int nsib, const fgt::PotTask* tasks,
+                         const double2* expn, const std::vector<Comp>& comps, hipStream_t st) {
+    std::vector<fgt::ZOut> groups;
+    std::vector<fgt::Kinds> gkd;
+    group_comps(comps, groups, gkd);
+    for (size_t k = 0; k < groups.size(); ++k)
+        fgt::launch_pw2pot_tasks_z(g, t, ntask, nsib, tasks, expn, gkd[k], groups[k], st);
 }
 
 std::vector<Comp> components(fgt_solver& S, double* pot_override = nullptr) {
@@ -995,19 +1015,17 @@ void execute(fgt_solver& S, hipStream_t st, bool timed, bool fork_near = false,
             }
         }
         mark(4);
-        // STEP 5, once per output component (pot, grad, hess entries)
+        // STEP 5 over the output components (pot, grad, hess entries)
         if (fork) {
             hcheck(hipStreamWaitEvent(st, S.ev_join, 0), "hipStreamWaitEvent");
             mark(7);   // STEP 5 starts here (after the wait for STEP 6)
         }
-        for (const Comp& c : comps) {
-            if (S.fast3d)
-                fgt::launch_pw2pot_tasks(g, t, S.n_s5, S.n_s5_sib, S.d_s5_tasks.as<fgt::PotTask>(),
-                                         ex, c.kd, c.od, st);
-            else
+        if (S.fast3d)
+            launch_pw2pot_comps(g, t, S.n_s5, S.n_s5_sib, S.d_s5_tasks.as<fgt::PotTask>(), ex, comps, st);
+        else
+            for (const Comp& c : comps)
                 fgt::launch_pw2pot(g, t, (int)P.leaf_box.size(), S.d_leaf_box.as<int>(), S.d_leaf_lev.as<int>(),
                                    S.d_leaf_slot.as<int>(), ex, c.kd, c.od, st);
-        }
         mark(5);
     } else {
         for (int i = 1; i <= 5; ++i) mark(i);

@@ -92,6 +92,16 @@ struct Kinds {
     int x, y, z;
 };
 
+// Output components that differ only in their z table kind (up to three:
+// value, first, second derivative) share one STEP 5 or STEP 6 pass: the x
+// and y sums are formed once and each z kind zo.kz[q] contracts them into
+// its own output zo.od[q].
+struct ZOut {
+    int nz;
+    int kz[3];
+    OutDesc od[3];
+};
+
 // One task of the 3-D STEP 5 kernel: the local expansion in `slot`,
 // evaluated at up to four leaves of level `lev` that share the x table
 // variant vx (0 plain table; 1 / 2 table times the + / conjugate
@@ -185,6 +195,11 @@ bool launch_p2pw_3d(const Geom& g, const DevTables& t, int ntask, const int* box
 bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                       const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit,
                       const Multi& mu = Multi{});
+// the same for the components of zo (x and y kinds kd.x, kd.y): the
+// one-leaf tasks share their x and y stages across the z kinds
+bool launch_pw2pot_3d_z(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                        const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s, size_t lds_limit,
+                        const Multi& mu = Multi{});
 bool pw2pot_3d_supported(const Geom& g, const DevTables& t);
 // true when FGT_PW2POT_VALU=1: STEP 5 keeps its x stage on the VALU
 bool pw2pot_valu_only();
@@ -260,6 +275,8 @@ void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box,
 // 3-D STEP 5 over PotTask lists (throws when the shape is not covered)
 void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu = Multi{});
+void launch_pw2pot_tasks_z(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                           const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s);
 // flags (nullable): per entry bit0 = last of its (y,z) table group, bit1 =
 // last of its z group; entries must then be grouped in that order.
 void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
@@ -269,14 +286,6 @@ void launch_direct(const Geom& g, const DevTables& t, int ntask, const int* tgt,
 bool launch_direct_3d(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,
                       const int* off, const int* src, const int* code, const int* flags,
                       const double* fvals, Kinds kd, OutDesc od, hipStream_t s, const Multi& mu = Multi{});
-// Output components that differ only in their z table kind (up to three:
-// value, first, second derivative) share one STEP 6 pass: the x and y sums
-// are formed once and each z kind contracts them into its own output.
-struct ZOut {
-    int nz;
-    int kz[3];
-    OutDesc od[3];
-};
 // 3-D register-blocked path only (norder 8, 12, 16, with flags); false
 // when not covered (nothing launched)
 bool launch_direct_3d_z(const Geom& g, const DevTables& t, int ntask, const int* tgt, const int* lev,

@@ -1126,6 +1126,14 @@ void launch_pw2pot_tasks(const Geom& g, const DevTables& t, int ntask, int nsib,
     check(hipGetLastError(), "k_pw2pot3 launch");
 }
 
+void launch_pw2pot_tasks_z(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                           const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s) {
+    if (ntask <= 0) return;
+    if (!launch_pw2pot_3d_z(g, t, ntask, nsib, tasks, expn, kd, zo, s, t.lds_limit))
+        throw std::runtime_error("3-D STEP 5 kernel not available for this shape");
+    check(hipGetLastError(), "k_pw2pot3 launch");
+}
+
 void launch_pw2pot(const Geom& g, const DevTables& t, int ntask, const int* box, const int* lev,
                    const int* slot, const double2* expn, Kinds kd, OutDesc od, hipStream_t s) {
     if (ntask <= 0) return;

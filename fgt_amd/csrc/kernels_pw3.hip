@@ -732,10 +732,16 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
 // C: most leaves per task (4 for sibling tasks, 1 for a leaf's own
 // expansion; with one leaf the freed registers carry the x stage's next
 // K-step loads while the current one runs).
-template <int N, int XS, int C, typename ET>
+// NZ: output components sharing the x and y table kinds and differing in the
+// z kind zo.kz[q] (one-leaf tasks only: all their variants are 0, so sUr
+// holds the round's rows of NZ <= 3 z kinds in place of the three
+// variants).  The x and y stages run once for all of them, the z stage once
+// per kind, into zo.od[q].
+template <int N, int XS, int C, typename ET, int NZ = 1>
 __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __restrict__ tabS_a, int npw,
                                                 const PotTask* __restrict__ tasks, const ET* __restrict__ expn_a,
-                                                size_t ks, Kinds kd, OutDesc od, Multi mu) {
+                                                size_t ks, Kinds kd, ZOut zo, Multi mu) {
+    static_assert(NZ >= 1 && NZ <= 3 && (NZ == 1 || C == 1), "z kinds are grouped for one-leaf tasks only");
     extern __shared__ double2 sm[];
     double2* sff = sm;
     double2* yres = sff + (size_t)npw * 64;
@@ -752,7 +758,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
         npw = q.g.npw;
         tabS = q.pw2potS;
         expn = static_cast<const ET*>(q.expn);
-        od.p = q.pot;
+        zo.od[0].p = zo.od[1].p = zo.od[2].p = q.pot;
     }
     const int sl = T->slot, l = T->lev, nvy = T->nvy;
     int cbox[C], ccode[C];
@@ -765,7 +771,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     // offsets are wave-uniform so the table reads stay on the scalar path
     const int per = npw * N;
     const int ox = __builtin_amdgcn_readfirstlane((int)(kd.x * ks) + (3 * l + T->vx) * per);
-    const int oz = __builtin_amdgcn_readfirstlane((int)(kd.z * ks) + 3 * l * per);
+    const int oz = __builtin_amdgcn_readfirstlane((int)(zo.kz[0] * ks) + 3 * l * per);
     const CTab2 UX = ctab(tabS) + ox;
     const CTab2 UZ = ctab(tabS) + oz;
     const int tid = threadIdx.x;
@@ -776,12 +782,14 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const bool halve = (npw & 1) != 0;
     // z stage: one thread per output column (zk1, zk2), all kk3 in
     // registers; where the block holds exactly one column per thread and leaf
-    // (N = 8, four leaves) each wave takes one leaf (ZW), else every thread
-    // below N^2 takes all the leaves
+    // (N = 8, four leaves) each wave takes one leaf (ZW), where it holds one
+    // column per thread and z kind (N = 8, NZ kinds) wave q takes kind q
+    // (ZK), else every thread below N^2 takes all the leaves or kinds
     constexpr bool ZW = C > 1 && N * N * C == 256;
-    constexpr int CG = ZW ? 1 : C;
-    const bool zact = ZW || tid < N * N;
-    const int zt = ZW ? lane : tid;
+    constexpr bool ZK = NZ > 1 && N * N == 64;
+    constexpr int CG = (ZW || ZK) ? 1 : C * NZ;
+    const bool zact = ZW || (ZK ? wave < NZ : tid < N * N);
+    const int zt = (ZW || ZK) ? lane : tid;
     const int zk1 = zt % N, zk2 = zt / N;
     const int wbox = ZW ? T->box[wave] : -1, wcode = ZW ? T->code[wave] : 0;
     for (int ind = 0; ind < nd; ++ind) {
@@ -793,10 +801,20 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
         const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
         for (int r = 0; r < nh; r += 4) {
             __syncthreads();
-            // the round's z table rows, all three variants (zero past nh)
-            if (tid < 12 * N) {
-                const int v = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
-                sUr[tid] = (r + s2 < nh) ? UZ[v * per + (r + s2) * N + k] : make_double2(0.0, 0.0);
+            // the round's z table rows, all three variants or (NZ > 1) the
+            // plain rows of each z kind (zero past nh)
+            if constexpr (NZ == 1) {
+                if (tid < 12 * N) {
+                    const int v = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
+                    sUr[tid] = (r + s2 < nh) ? UZ[v * per + (r + s2) * N + k] : make_double2(0.0, 0.0);
+                }
+            } else {
+                if (tid < 4 * N * NZ) {
+                    const int z = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
+                    const int kz = z == 0 ? zo.kz[0] : (z == 1 ? zo.kz[1] : zo.kz[2]);
+                    const CTab2 U = ctab(tabS) + ((long long)kz * ks + 3 * l * per);
+                    sUr[tid] = (r + s2 < nh) ? U[(r + s2) * N + k] : make_double2(0.0, 0.0);
+                }
             }
             // x: wave = slab, lane = k2.  Rows of E are fetched PF at a time
             // (all loads issued before the FMAs that use them) so enough reads
@@ -815,10 +833,11 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                     for (int t = 0; t < 4; ++t) are[t] = aim[t] = dbl4{0.0, 0.0, 0.0, 0.0};
                     const int ksteps = (npw + 3) >> 2;
-                    if constexpr (C == 1) {
+                    if constexpr (C == 1 && NZ == 1) {
                         // the next K-step's operands are loaded (always, from a
                         // valid clamped address) before this step's products,
-                        // into two alternating register sets
+                        // into two alternating register sets (with NZ > 1 the
+                        // z sums take those registers)
                         double2 ua, ub, ea[4], eb[4];
 #define PW2POT_XLOAD(u_, e_, st)                                                             \
     {                                                                                        \
@@ -946,6 +965,27 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                                     G[0][kk3] += u.x * yr - u.y * yi;
                                 }
                             }
+                        } else if constexpr (ZK) {
+                            if (cbox[0] >= 0) {
+                                const double2* __restrict__ uz = sUr + (wave * 4 + s2) * N;
+#pragma unroll
+                                for (int kk3 = 0; kk3 < N; ++kk3) {
+                                    const double2 u = uz[kk3];
+                                    G[0][kk3] += u.x * yr - u.y * yi;
+                                }
+                            }
+                        } else if constexpr (NZ > 1) {
+                            if (cbox[0] >= 0) {
+#pragma unroll
+                                for (int z = 0; z < NZ; ++z) {
+                                    const double2* __restrict__ uz = sUr + (z * 4 + s2) * N;
+#pragma unroll
+                                    for (int kk3 = 0; kk3 < N; ++kk3) {
+                                        const double2 u = uz[kk3];
+                                        G[z][kk3] += u.x * yr - u.y * yi;
+                                    }
+                                }
+                            }
                         } else {
 #pragma unroll
                             for (int c = 0; c < C; ++c) {
@@ -971,11 +1011,18 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             asm volatile("" : "+v"(col));
 #pragma unroll
             for (int c = 0; c < CG; ++c) {
-                const int b = ZW ? wbox : cbox[c];
+                const int b = ZW ? wbox : cbox[C > 1 ? c : 0];
+                // output of kind c (NZ > 1), of this wave's kind (ZK), else
+                // the one output (selected field by field: a whole-struct
+                // select by a run-time index would put zo in scratch memory)
+                const int q = ZK ? wave : (NZ > 1 ? c : 0);
+                double* const op = q == 0 ? zo.od[0].p : (q == 1 ? zo.od[1].p : zo.od[2].p);
+                const int ons = q == 0 ? zo.od[0].ns : (q == 1 ? zo.od[1].ns : zo.od[2].ns);
+                const int ocoff = q == 0 ? zo.od[0].coff : (q == 1 ? zo.od[1].coff : zo.od[2].coff);
                 if (b >= 0) {
-                    double* __restrict__ o = od.p + (size_t)b * g.npbox * od.ns + od.coff + ind;
+                    double* __restrict__ o = op + (size_t)b * g.npbox * ons + ocoff + ind;
 #pragma unroll
-                    for (int kk3 = 0; kk3 < N; ++kk3) o[(size_t)(col + N * N * kk3) * od.ns] += 2.0 * G[c][kk3];
+                    for (int kk3 = 0; kk3 < N; ++kk3) o[(size_t)(col + N * N * kk3) * ons] += 2.0 * G[c][kk3];
                 }
             }
         }
@@ -1029,36 +1076,57 @@ size_t pw2pot3_lds(int n, int npw) {
     return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + 12 * (size_t)n);
 }
 
-template <int N, int XS, int C>
+template <int N, int XS, int C, int NZ = 1>
 void launch_pw2pot3_c(const Geom& g, const DevTables& t, int ntask, const PotTask* tasks, const double2* expn,
-                      Kinds kd, OutDesc od, hipStream_t s, size_t lds, const Multi& mu) {
+                      Kinds kd, const ZOut& zo, hipStream_t s, size_t lds, const Multi& mu) {
     if (ntask <= 0) return;
     if constexpr (XS >= 2) {
-        hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
-                           tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od, mu);
+        hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2, NZ>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
+                           tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, zo, mu);
     } else {
         if (t.expn_f32)
-            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS, t.npw,
-                               tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, od, mu);
+            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, float2, NZ>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS,
+                               t.npw, tasks, reinterpret_cast<const float2*>(expn), t.pw2potS_ks, kd, zo, mu);
         else
-            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, double2>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS,
-                               t.npw, tasks, expn, t.pw2potS_ks, kd, od, mu);
+            hipLaunchKernelGGL((k_pw2pot3<N, XS, C, double2, NZ>), dim3(ntask), dim3(256), lds, s, g, t.pw2potS,
+                               t.npw, tasks, expn, t.pw2potS_ks, kd, zo, mu);
     }
 }
 
-// tasks [0, nsib) hold up to four leaves, [nsib, ntask) one leaf each
+// components zo.kz[a..a+nz) of zo as a group of their own
+ZOut zsub(const ZOut& zo, int a, int nz) {
+    ZOut r{};
+    r.nz = nz;
+    for (int q = 0; q < nz; ++q) {
+        r.kz[q] = zo.kz[a + q];
+        r.od[q] = zo.od[a + q];
+    }
+    return r;
+}
+
+// tasks [0, nsib) hold up to four leaves and run once per z kind; [nsib,
+// ntask) hold one leaf each and run once for all the kinds
 template <int N, int XS>
 void launch_pw2pot3_typed(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                          const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds, const Multi& mu) {
-    launch_pw2pot3_c<N, XS, 4>(g, t, nsib, tasks, expn, kd, od, s, lds, mu);
+                          const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s, size_t lds,
+                          const Multi& mu) {
+    for (int q = 0; q < zo.nz; ++q) launch_pw2pot3_c<N, XS, 4>(g, t, nsib, tasks, expn, kd, zsub(zo, q, 1), s, lds, mu);
     Multi m1 = mu;
     if (m1.tprob) m1.tprob += nsib;
-    launch_pw2pot3_c<N, XS, 1>(g, t, ntask - nsib, tasks + nsib, expn, kd, od, s, lds, m1);
+    const int n1 = ntask - nsib;
+    const PotTask* t1 = tasks + nsib;
+    if (zo.nz == 3) {
+        launch_pw2pot3_c<N, XS, 1, 3>(g, t, n1, t1, expn, kd, zo, s, lds, m1);
+    } else if (zo.nz == 2) {
+        launch_pw2pot3_c<N, XS, 1, 2>(g, t, n1, t1, expn, kd, zo, s, lds, m1);
+    } else {
+        launch_pw2pot3_c<N, XS, 1, 1>(g, t, n1, t1, expn, kd, zo, s, lds, m1);
+    }
 }
 
 template <int N>
 bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                 const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit, const Multi& mu) {
+                 const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s, size_t lds_limit, const Multi& mu) {
     const size_t lds = pw2pot3_lds(N, t.npw);
     if (lds > lds_limit) return false;
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
@@ -1066,25 +1134,25 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     if (xs == 2) {
         // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores)
         if (t.prec == 2) {
-            launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+            launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
             return true;
         }
         if constexpr (N == 8) {
             if (t.npw > 32) {
-                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
                 return true;
             }
         }
-        launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+        launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
         return true;
     }
     if constexpr (N >= 12) {
         if (xs == 1) {
-            launch_pw2pot3_typed<N, 1>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+            launch_pw2pot3_typed<N, 1>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
             return true;
         }
     }
-    launch_pw2pot3_typed<N, 0>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds, mu);
+    launch_pw2pot3_typed<N, 0>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
     return true;
 }
 
@@ -1173,15 +1241,25 @@ bool pw2pot_3d_supported(const Geom& g, const DevTables& t) {
     return lds <= t.lds_limit && lds1 <= t.lds_limit && lds2 <= t.lds_limit;
 }
 
-bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
-                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit, const Multi& mu) {
-    if (g.ndim != 3 || ntask <= 0) return false;
+bool launch_pw2pot_3d_z(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                        const double2* expn, Kinds kd, const ZOut& zo, hipStream_t s, size_t lds_limit,
+                        const Multi& mu) {
+    if (g.ndim != 3 || ntask <= 0 || zo.nz < 1 || zo.nz > 3) return false;
     switch (t.n) {
-        case 8: return try_pw2pot3<8>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
-        case 12: return try_pw2pot3<12>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
-        case 16: return try_pw2pot3<16>(g, t, ntask, nsib, tasks, expn, kd, od, s, lds_limit, mu);
+        case 8: return try_pw2pot3<8>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds_limit, mu);
+        case 12: return try_pw2pot3<12>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds_limit, mu);
+        case 16: return try_pw2pot3<16>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds_limit, mu);
         default: return false;
     }
+}
+
+bool launch_pw2pot_3d(const Geom& g, const DevTables& t, int ntask, int nsib, const PotTask* tasks,
+                      const double2* expn, Kinds kd, OutDesc od, hipStream_t s, size_t lds_limit, const Multi& mu) {
+    ZOut zo{};
+    zo.nz = 1;
+    zo.kz[0] = kd.z;
+    zo.od[0] = od;
+    return launch_pw2pot_3d_z(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds_limit, mu);
 }
 
 }  // namespace fgt
