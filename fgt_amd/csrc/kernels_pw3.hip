@@ -84,18 +84,19 @@ __device__ __forceinline__ float bcol(double2 u, int part, int kpart) {
 // fused form's two x signs).  fa(k2, kk) / fb(kk, k1) return the FP64
 // operands (stage buffer in LDS / 1-D table); fw(k2, k1, v) stores one
 // complex result.  The wave does column tiles ct0, ct0 + ctstep, ... in
-// pairs.  Each lane trades its part with the neighbouring lane (the other
-// part of the same output) so every result is written once as a complex
-// pair: part-0 lanes write rows v = 0, 1 of their four, part-1 lanes 2, 3.
+// pairs (FP32: a last tile without a partner alone).  Each lane trades its part
+// with the neighbouring lane (the other part of the same output) so every
+// result is written once as a complex pair: part-0 lanes write rows v = 0, 1
+// of their four, part-1 lanes 2, 3.
 constexpr int kLowpColGroup = 2;
 
-template <int KN, int XS, class FA, class FB, class FW>
-__device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, FA fa, FB fb, FW fw) {
-    constexpr int CG = kLowpColGroup;
+// column tiles c0, c0 + ctstep, ... (CG of them, all below CT)
+template <int KN, int XS, int CG, class FA, class FB, class FW>
+__device__ __forceinline__ void x1_lowp_group(int npw, int lane, int c0, int ctstep, FA fa, FB fb, FW fw) {
     const int MT = (npw + 15) >> 4;
     const int CT = (2 * npw + 15) >> 4;
     const int li = lane & 15, lq = lane >> 4;
-    for (int c0 = ct0; c0 < CT; c0 += CG * ctstep) {
+    {
         int k1c[CG], part[CG];
         bool cval[CG];
 #pragma unroll
@@ -194,6 +195,19 @@ __device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, 
                     }
             }
         }
+    }
+}
+
+template <int KN, int XS, class FA, class FB, class FW>
+__device__ __forceinline__ void x1_lowp(int npw, int lane, int ct0, int ctstep, FA fa, FB fb, FW fw) {
+    constexpr int CG = kLowpColGroup;
+    const int CT = (2 * npw + 15) >> 4;
+    for (int c0 = ct0; c0 < CT; c0 += CG * ctstep) {
+        // (BF16: the one-tile form's registers would cost a block per CU)
+        if (XS != 2 || c0 + (CG - 1) * ctstep < CT)
+            x1_lowp_group<KN, XS, CG>(npw, lane, c0, ctstep, fa, fb, fw);
+        else
+            x1_lowp_group<KN, XS, 1>(npw, lane, c0, ctstep, fa, fb, fw);
     }
 }
 
