@@ -406,10 +406,13 @@ __global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int n
     if (!ok) e = tot - 1;   // valid duplicate, never stored
     const int r = (int)(e % g.nexp);
     const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    // rows 0-4: the w_x w_y products; rows 5-7: w_z for oz = 0, 1, 2 (read
-    // at a uniform row instead of selected from registers per pair: STEP 3
-    // at a norder-16 periodic problem 0.745 -> 0.696 ms)
-    __shared__ double2 s_fxy[8][kBlock];
+    // rows 0-4: the w_x w_y products (row 4 = 1); row 5: c = w_z(2).  w_z
+    // for oz = 0, 1, 2 is conj(c), 1, c (the table's rows are exactly that),
+    // read at a uniform row (5 with a sign flip, 4, 5) instead of selected
+    // from registers per pair (STEP 3 at a norder-16 periodic problem 0.745
+    // -> 0.696 ms); six rows rather than eight leave room for six blocks per
+    // CU in LDS instead of four
+    __shared__ double2 s_fxy[6][kBlock];
     const int tid = threadIdx.x;
     {
         const double2 a = t.wm2l[2 * npw + k1];
@@ -419,9 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int n
         s_fxy[2][tid] = a;
         s_fxy[3][tid] = b;
         s_fxy[4][tid] = make_double2(1.0, 0.0);
-        s_fxy[5][tid] = t.wm2l[k3];
-        s_fxy[6][tid] = t.wm2l[npw + k3];
-        s_fxy[7][tid] = t.wm2l[2 * npw + k3];
+        s_fxy[5][tid] = t.wm2l[2 * npw + k3];
     }
     const ET* __restrict__ eb = expn + e;
     __shared__ long long s_soff[kM2lBrickMax + 2 * kM2lBatch];
@@ -447,7 +448,9 @@ __global__ void __launch_bounds__(kBlock) k_m2l_brick(Geom g, DevTables t, int n
             if ((mask >> q) & 1) {                                                       \
                 const int ox = (q & 1) + 2 - px, oy = ((q >> 1) & 1) + 2 - py;           \
                 const int oz = ((q >> 2) & 1) + 2 - pz, oxy = ox + 3 * oy;               \
-                const double2 zf = s_fxy[5 + oz][tid];                                   \
+                double2 zf = s_fxy[oz == 1 ? 4 : 5][tid];                                \
+                zf.y = __longlong_as_double(__double_as_longlong(zf.y) ^                 \
+                                            ((long long)(oz == 0) << 63));               \
                 double2 fxy = s_fxy[(kM2lRow >> (3 * oxy)) & 7][tid];                    \
                 fxy.y = __longlong_as_double(__double_as_longlong(fxy.y) ^               \
                                              ((long long)((kM2lConj >> oxy) & 1) << 63)); \
