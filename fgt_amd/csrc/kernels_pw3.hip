@@ -545,19 +545,21 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     }
 }
 
-// x stage of STEP 5 in reduced precision on the matrix cores (wave = slab k3):
+// x stage of STEP 5 in reduced precision on the matrix cores (wave w takes
+// the round's slabs w, w + 4, ... of SR):
 //   ff[k2][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2],
 // rows k2 (row tiles of 16, up to 80 frequencies), columns (kk1, part), K =
 // k1.  XS 2: FP32 operands; XS 3: BF16 hi/lo operands.  The result goes to
-// sff[(k2*4 + slab)*16 + (kk1 ^ (k2 & 15))] (FP64 complex, the layout of the
-// FP64 x stages), where the y and z stages read it.  Lanes past npw read a
-// valid duplicate row (discarded); K entries past npw get zero table values.
-template <int N, int XS>
+// sff[(k2*SR + slab)*KW + (kk1 ^ (k2 & (KW-1)))], KW = 64 / SR (FP64
+// complex; for SR = 4 the layout of the FP64 x stages), where the y and z
+// stages read it.  Lanes past npw read a valid duplicate row (discarded); K
+// entries past npw get zero table values.
+template <int N, int XS, int SR>
 __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const CTab2 UX,
                                               int npw, int nh, int r, int wave, int lane, double2* sff) {
-    const int k3 = r + wave;
-    if (k3 >= nh) return;
-    const float2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+    constexpr int KW = 64 / SR;
+    static_assert(KW >= N, "a (k2, slab) row of sff holds the N values kk1");
+    if (r + wave >= nh) return;
     constexpr int NT = (2 * N + 15) / 16;
     const int MT = (npw + 15) >> 4;
     const int li = lane & 15, lq = lane >> 4;
@@ -593,128 +595,133 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
             }
         }
     }
-    // one row tile (16 frequencies k2) at a time, two accumulator chains per
-    // column tile
-    for (int mt = 0; mt < MT; ++mt) {
-        const int k2r = min(16 * mt + li, npw - 1);
-        flt4 acc[NT][2];
+    for (int sl = wave; sl < SR; sl += 4) {
+        const int k3 = r + sl;
+        if (k3 >= nh) break;
+        const float2* __restrict__ Ek = E + (size_t)k3 * npw * npw;
+        // one row tile (16 frequencies k2) at a time, two accumulator chains per
+        // column tile
+        for (int mt = 0; mt < MT; ++mt) {
+            const int k2r = min(16 * mt + li, npw - 1);
+            flt4 acc[NT][2];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = flt4{0.f, 0.f, 0.f, 0.f};
-        if constexpr (HOIST) {
-            // E rows in chunks of four K steps, the next chunk loaded before
-            // the current one's products (chunks unrolled to the largest
-            // plane-wave count, uniform guards past npw)
-            const int ks = (npw + 3) >> 2;
-            constexpr int NCH = KSH / 4;
-            float2 e[2][4];
+            for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = flt4{0.f, 0.f, 0.f, 0.f};
+            if constexpr (HOIST) {
+                // E rows in chunks of four K steps, the next chunk loaded before
+                // the current one's products (chunks unrolled to the largest
+                // plane-wave count, uniform guards past npw)
+                const int ks = (npw + 3) >> 2;
+                constexpr int NCH = KSH / 4;
+                float2 e[2][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) e[0][j] = Ek[(size_t)min(4 * j + lq, npw - 1) * npw + k2r];
+                for (int j = 0; j < 4; ++j) e[0][j] = Ek[(size_t)min(4 * j + lq, npw - 1) * npw + k2r];
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                if (4 * c < ks) {
-                    if (c + 1 < NCH && 4 * (c + 1) < ks) {
+                for (int c = 0; c < NCH; ++c) {
+                    if (4 * c < ks) {
+                        if (c + 1 < NCH && 4 * (c + 1) < ks) {
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            e[(c + 1) & 1][j] = Ek[(size_t)min(4 * (4 * (c + 1) + j) + lq, npw - 1) * npw + k2r];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
+                            for (int j = 0; j < 4; ++j)
+                                e[(c + 1) & 1][j] = Ek[(size_t)min(4 * (4 * (c + 1) + j) + lq, npw - 1) * npw + k2r];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        if (4 * c + j < ks) {
-                            acc[0][0] = mfma32(e[c & 1][j].x, hb0[4 * c + j], acc[0][0]);
-                            acc[0][1] = mfma32(e[c & 1][j].y, hb1[4 * c + j], acc[0][1]);
+                        for (int j = 0; j < 4; ++j) {
+                            if (4 * c + j < ks) {
+                                acc[0][0] = mfma32(e[c & 1][j].x, hb0[4 * c + j], acc[0][0]);
+                                acc[0][1] = mfma32(e[c & 1][j].y, hb1[4 * c + j], acc[0][1]);
+                            }
                         }
                     }
                 }
-            }
-        } else if constexpr (XS == 2) {
-            // FP32: K steps in chunks of KC, the next chunk's operand row and
-            // table entries loaded (from valid clamped addresses, zeroed past
-            // npw) before the current chunk's products, into two register sets
-            // (two steps per chunk where two column tiles hold the registers)
-            constexpr int KC = NT == 1 ? 4 : 2;
-            const int ks = (npw + 3) >> 2;
-            const int nchunk = (ks + KC - 1) / KC;
-            float2 ea[KC], eb[KC];
-            float ba[NT][KC][2], bb[NT][KC][2];
-            auto load = [&](float2 (&e)[KC], float (&bv)[NT][KC][2], int c) {
+            } else if constexpr (XS == 2) {
+                // FP32: K steps in chunks of KC, the next chunk's operand row and
+                // table entries loaded (from valid clamped addresses, zeroed past
+                // npw) before the current chunk's products, into two register sets
+                // (two steps per chunk where two column tiles hold the registers)
+                constexpr int KC = NT == 1 ? 4 : 2;
+                const int ks = (npw + 3) >> 2;
+                const int nchunk = (ks + KC - 1) / KC;
+                float2 ea[KC], eb[KC];
+                float ba[NT][KC][2], bb[NT][KC][2];
+                auto load = [&](float2 (&e)[KC], float (&bv)[NT][KC][2], int c) {
 #pragma unroll
-                for (int j = 0; j < KC; ++j) {
-                    const int k1 = 4 * (KC * c + j) + lq;
-                    const bool kv = k1 < npw;
-                    const int k1c = kv ? k1 : npw - 1;
-                    e[j] = Ek[(size_t)k1c * npw + k2r];
+                    for (int j = 0; j < KC; ++j) {
+                        const int k1 = 4 * (KC * c + j) + lq;
+                        const bool kv = k1 < npw;
+                        const int k1c = kv ? k1 : npw - 1;
+                        e[j] = Ek[(size_t)k1c * npw + k2r];
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        const double2 u = UX[k1c * N + ckk[t]];
-                        const bool ok = kv && cval[t];
-                        bv[t][j][0] = ok ? bcol(u, cpart[t], 0) : 0.f;
-                        bv[t][j][1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                        for (int t = 0; t < NT; ++t) {
+                            const double2 u = UX[k1c * N + ckk[t]];
+                            const bool ok = kv && cval[t];
+                            bv[t][j][0] = ok ? bcol(u, cpart[t], 0) : 0.f;
+                            bv[t][j][1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                        }
                     }
+                };
+                auto apply = [&](const float2 (&e)[KC], const float (&bv)[NT][KC][2]) {
+#pragma unroll
+                    for (int j = 0; j < KC; ++j)
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) {
+                            acc[t][0] = mfma32(e[j].x, bv[t][j][0], acc[t][0]);
+                            acc[t][1] = mfma32(e[j].y, bv[t][j][1], acc[t][1]);
+                        }
+                };
+                load(ea, ba, 0);
+                for (int c = 0; c < nchunk; c += 2) {
+                    load(eb, bb, min(c + 1, nchunk - 1));
+                    __builtin_amdgcn_sched_barrier(0);
+                    apply(ea, ba);
+                    if (c + 1 >= nchunk) break;
+                    load(ea, ba, min(c + 2, nchunk - 1));
+                    __builtin_amdgcn_sched_barrier(0);
+                    apply(eb, bb);
                 }
-            };
-            auto apply = [&](const float2 (&e)[KC], const float (&bv)[NT][KC][2]) {
-#pragma unroll
-                for (int j = 0; j < KC; ++j)
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        acc[t][0] = mfma32(e[j].x, bv[t][j][0], acc[t][0]);
-                        acc[t][1] = mfma32(e[j].y, bv[t][j][1], acc[t][1]);
-                    }
-            };
-            load(ea, ba, 0);
-            for (int c = 0; c < nchunk; c += 2) {
-                load(eb, bb, min(c + 1, nchunk - 1));
-                __builtin_amdgcn_sched_barrier(0);
-                apply(ea, ba);
-                if (c + 1 >= nchunk) break;
-                load(ea, ba, min(c + 2, nchunk - 1));
-                __builtin_amdgcn_sched_barrier(0);
-                apply(eb, bb);
-            }
-        } else {
-            // K = (k1, part): 16 frequencies per step; lane K entries 8 lq .. 8 lq + 7
-            const int ks = (npw + 15) >> 4;
-            for (int st = 0; st < ks; ++st) {
-                float ax[8];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k1c = min(16 * st + 4 * lq + j, npw - 1);
-                    const float2 e = Ek[(size_t)k1c * npw + k2r];
-                    ax[2 * j] = e.x;
-                    ax[2 * j + 1] = e.y;
-                }
-                bf8 ah, al;
-                split_bf(ax, ah, al);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    float bx[8];
+            } else {
+                // K = (k1, part): 16 frequencies per step; lane K entries 8 lq .. 8 lq + 7
+                const int ks = (npw + 15) >> 4;
+                for (int st = 0; st < ks; ++st) {
+                    float ax[8];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int k1 = 16 * st + 4 * lq + j;
-                        const bool ok = k1 < npw && cval[t];
-                        const double2 u = UX[min(k1, npw - 1) * N + ckk[t]];
-                        bx[2 * j] = ok ? bcol(u, cpart[t], 0) : 0.f;
-                        bx[2 * j + 1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                        const int k1c = min(16 * st + 4 * lq + j, npw - 1);
+                        const float2 e = Ek[(size_t)k1c * npw + k2r];
+                        ax[2 * j] = e.x;
+                        ax[2 * j + 1] = e.y;
                     }
-                    bf8 bh, bl;
-                    split_bf(bx, bh, bl);
-                    acc[t][0] = mfma_bf(ah, bh, acc[t][0]);
-                    acc[t][1] = mfma_bf(ah, bl, acc[t][1]);
-                    acc[t][1] = mfma_bf(al, bh, acc[t][1]);
+                    bf8 ah, al;
+                    split_bf(ax, ah, al);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        float bx[8];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int k1 = 16 * st + 4 * lq + j;
+                            const bool ok = k1 < npw && cval[t];
+                            const double2 u = UX[min(k1, npw - 1) * N + ckk[t]];
+                            bx[2 * j] = ok ? bcol(u, cpart[t], 0) : 0.f;
+                            bx[2 * j + 1] = ok ? bcol(u, cpart[t], 1) : 0.f;
+                        }
+                        bf8 bh, bl;
+                        split_bf(bx, bh, bl);
+                        acc[t][0] = mfma_bf(ah, bh, acc[t][0]);
+                        acc[t][1] = mfma_bf(ah, bl, acc[t][1]);
+                        acc[t][1] = mfma_bf(al, bh, acc[t][1]);
+                    }
                 }
             }
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if (cval[t])
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int k2 = 16 * mt + 4 * lq + v;
+                        if (k2 < npw)
+                            sffd[2 * ((k2 * SR + sl) * KW + (ckk[t] ^ (k2 & (KW - 1)))) + cpart[t]] =
+                                acc[t][0][v] + acc[t][1][v];
+                    }
         }
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-            if (cval[t])
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int k2 = 16 * mt + 4 * lq + v;
-                    if (k2 < npw)
-                        sffd[2 * ((k2 * 4 + wave) * 16 + (ckk[t] ^ (k2 & 15))) + cpart[t]] =
-                            acc[t][0][v] + acc[t][1][v];
-                }
     }
 }
 
@@ -725,14 +732,15 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
 // leaves: for a leaf evaluating its parent's expansion the phase is folded
 // into the tables (STEP 4 folded into STEP 5), and the x variant depends only
 // on the x split sign, so four siblings share one x contraction.
-// Per round of four k3 slabs:
-//   x: ff[k2][s][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2]   (wave = slab)
+// Per round of SR k3 slabs (4; 8 for norder 8 with a matrix-core x stage):
+//   x: ff[k2][s][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2]   (wave = slab,
+//      or slabs w and w + 4)
 //   y: yres[s][kk1][kk2] = sum_k2 Uy[kk2][k2] ff[k2][s][kk1], per y variant
 //   z: G_c[kk3] += Re(Uz_c[kk3][k3] yres[s][zk1][zk2]), per leaf c, with one
 //      thread per output column (zk1, zk2) holding all kk3 in registers.
-// LDS: sff [k2][s][16] (npw*64 complex), yres [s][kk1][16] (1024 complex,
-// entry kk2 at kk2 ^ kk1), sUr [v][s][kk3] (12*N complex: the round's z
-// table rows of the three variants, zero past nh).
+// LDS: sff [k2][s][KW] (npw*64 complex, KW = 64 / SR), yres [s][kk1][16]
+// (1024 complex, entry kk2 at kk2 ^ kk1), sUr [v][s][kk3] (3*SR*N complex:
+// the round's z table rows of the three variants, zero past nh).
 // MF: x stage on the FP64 matrix cores (N >= 12, npw <= 64): per slab the
 // complex product ff[kk1][k2] = sum_k1 U[kk1][k1] E[k1][k2] runs as
 // 16x16x4 tiles (rows kk1, columns 16 k2, K = 4 k1; for N = 12 the rows
@@ -762,6 +770,12 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     double2* sUr = yres + 4 * 16 * 16;
     constexpr int NQ = N / 4;
     constexpr int PF = (N == 12) ? 6 : 8;
+    // slabs per round: 8 for norder 8 with a matrix-core x stage (each wave
+    // computes two slabs, and the y stage's 64 lanes are (kk1, slab) with no
+    // idle half), else 4; KW: kk1 slots per (k2, slab) row of sff, per
+    // (slab, kk1) group of yres
+    constexpr int SR = (N == 8 && XS >= 2) ? 8 : 4;
+    constexpr int KW = 64 / SR;
     const int tix = xcd_task(blockIdx.x, gridDim.x);
     const PotTask* __restrict__ T = tasks + tix;
     const double2* __restrict__ tabS = tabS_a;
@@ -792,7 +806,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int nh = g.pz, nd = g.nd;
-    const int kk1 = lane & 15, s = lane >> 4;
+    const int kk1 = lane & (KW - 1), s = lane / KW;
     const bool halve = (npw & 1) != 0;
     // z stage: one thread per output column (zk1, zk2), all kk3 in
     // registers; where the block holds exactly one column per thread and leaf
@@ -813,18 +827,18 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
             for (int k = 0; k < N; ++k) G[c][k] = 0.0;
         const ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + nd + ind) * g.nexp;
-        for (int r = 0; r < nh; r += 4) {
+        for (int r = 0; r < nh; r += SR) {
             __syncthreads();
             // the round's z table rows, all three variants or (NZ > 1) the
             // plain rows of each z kind (zero past nh)
             if constexpr (NZ == 1) {
-                if (tid < 12 * N) {
-                    const int v = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
+                if (tid < 3 * SR * N) {
+                    const int v = tid / (SR * N), s2 = (tid / N) % SR, k = tid % N;
                     sUr[tid] = (r + s2 < nh) ? UZ[v * per + (r + s2) * N + k] : make_double2(0.0, 0.0);
                 }
             } else {
-                if (tid < 4 * N * NZ) {
-                    const int z = tid / (4 * N), s2 = (tid / N) & 3, k = tid % N;
+                if (tid < SR * N * NZ) {
+                    const int z = tid / (SR * N), s2 = (tid / N) % SR, k = tid % N;
                     const int kz = z == 0 ? zo.kz[0] : (z == 1 ? zo.kz[1] : zo.kz[2]);
                     const CTab2 U = ctab(tabS) + ((long long)kz * ks + 3 * l * per);
                     sUr[tid] = (r + s2 < nh) ? U[(r + s2) * N + k] : make_double2(0.0, 0.0);
@@ -835,7 +849,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             // are in flight to cover HBM latency; lanes past npw read a valid
             // duplicate and discard it.
             if constexpr (XS >= 2) {
-                pw2pot_x_lowp<N, XS>(E, UX, npw, nh, r, wave, lane, sff);
+                pw2pot_x_lowp<N, XS, SR>(E, UX, npw, nh, r, wave, lane, sff);
             } else if constexpr (XS == 1) {
                 const int k3 = r + wave;
                 if (k3 < nh) {
@@ -953,26 +967,26 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
                     for (int k2 = 0; k2 < npw; ++k2) {
-                        const double2 f = sff[(k2 * 4 + s) * 16 + (kk1 ^ (k2 & 15))];
+                        const double2 f = sff[(k2 * SR + s) * KW + (kk1 ^ (k2 & (KW - 1)))];
 #pragma unroll
                         for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
                     }
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) yres[(s * 16 + kk1) * 16 + ((wave * NQ + q) ^ kk1)] = y[q];
+                    for (int q = 0; q < NQ; ++q) yres[(s * KW + kk1) * 16 + ((wave * NQ + q) ^ kk1)] = y[q];
                 }
                 __syncthreads();
                 // z, for the leaves that use this y variant
                 if (zact) {
 #pragma unroll 1
-                    for (int s2 = 0; s2 < 4; ++s2) {
+                    for (int s2 = 0; s2 < SR; ++s2) {
                         const int k3z = r + s2;
                         if (k3z >= nh) break;
-                        const double2 yv = yres[(s2 * 16 + zk1) * 16 + (zk2 ^ zk1)];
+                        const double2 yv = yres[(s2 * KW + zk1) * 16 + (zk2 ^ zk1)];
                         const double w = (halve && k3z == nh - 1) ? 0.5 : 1.0;
                         const double yr = yv.x * w, yi = yv.y * w;
                         if constexpr (ZW) {
                             if (wbox >= 0 && (wcode & 1) == j) {
-                                const double2* __restrict__ uz = sUr + ((wcode >> 1) * 4 + s2) * N;
+                                const double2* __restrict__ uz = sUr + ((wcode >> 1) * SR + s2) * N;
 #pragma unroll
                                 for (int kk3 = 0; kk3 < N; ++kk3) {
                                     const double2 u = uz[kk3];
@@ -981,7 +995,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                             }
                         } else if constexpr (ZK) {
                             if (cbox[0] >= 0) {
-                                const double2* __restrict__ uz = sUr + (wave * 4 + s2) * N;
+                                const double2* __restrict__ uz = sUr + (wave * SR + s2) * N;
 #pragma unroll
                                 for (int kk3 = 0; kk3 < N; ++kk3) {
                                     const double2 u = uz[kk3];
@@ -992,7 +1006,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                             if (cbox[0] >= 0) {
 #pragma unroll
                                 for (int z = 0; z < NZ; ++z) {
-                                    const double2* __restrict__ uz = sUr + (z * 4 + s2) * N;
+                                    const double2* __restrict__ uz = sUr + (z * SR + s2) * N;
 #pragma unroll
                                     for (int kk3 = 0; kk3 < N; ++kk3) {
                                         const double2 u = uz[kk3];
@@ -1004,7 +1018,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                             for (int c = 0; c < C; ++c) {
                                 if (cbox[c] >= 0 && (ccode[c] & 1) == j) {
-                                    const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * 4 + s2) * N;
+                                    const double2* __restrict__ uz = sUr + ((ccode[c] >> 1) * SR + s2) * N;
 #pragma unroll
                                     for (int kk3 = 0; kk3 < N; ++kk3) {
                                         const double2 u = uz[kk3];
@@ -1087,7 +1101,8 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
 }
 
 size_t pw2pot3_lds(int n, int npw) {
-    return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + 12 * (size_t)n);
+    // sff, yres, sUr (three variants of up to eight slabs at norder 8, four above)
+    return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + (n == 8 ? 24 : 12) * (size_t)n);
 }
 
 template <int N, int XS, int C, int NZ = 1>
