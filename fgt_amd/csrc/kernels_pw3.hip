@@ -341,23 +341,27 @@ __global__ void __launch_bounds__(256) k_p2pw3(Geom g, const double2* __restrict
 // eight separate leaf transforms, and no leaf expansion reaches HBM.
 // tabS[l][v][k][m]: child-level table, v = 0 plain, 1 = + phase, 2 = conjugate.
 // Task lists: kids (up to 8 leaf children), ksg (merge sign bits, x = bit0).
-// Rounds of two slabs; wave w: slab w & 1 and half (w >> 1) of the k1 range.
+// Rounds of SR slabs.  SR = 2: wave w takes slab w & 1 and half (w >> 1) of
+// the x stage's k1 range; the y stage's lanes are (m1 < 16, slab, sx).  SR = 4
+// (norder 8, matrix-core x stage, npw <= 48): wave w takes slab w, and the y
+// stage's 64 lanes are (m1 < 8, slab, sx) with none idle.
 // Each parent is split over kFusedParts blocks, each taking a contiguous run
 // of rounds (the parts of one parent are adjacent in launch order and land on
 // one XCD, so the children's values they all read meet in its L2).
-// LDS: sH [4 groups][2 slabs][m2][m1] and sF [2 sx][2 slabs][m1][npw] share
-// one buffer: sH is fully read into registers before sF is written.
+// LDS: sH [4 groups][SR slabs][m2][m1] and sF [2 sx][SR slabs][m1][npw]
+// share one buffer: sH is fully read into registers before sF is written.
 constexpr int kFusedParts = 4;
 
-template <int N, int XS, typename ET>
+template <int N, int XS, typename ET, int SR = 2>
 __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restrict__ tabS_a, int npw,
                                                const int* __restrict__ plev, const int* __restrict__ pslot,
                                                const int* __restrict__ kids, const int* __restrict__ ksg,
                                                const double* __restrict__ fvals_a,
                                                ET* __restrict__ expn_a, Multi mu) {
+    static_assert(SR == 2 || (SR == 4 && N == 8 && XS >= 2), "four-slab rounds: norder 8, matrix-core x stage");
     extern __shared__ double2 sm[];
-    double2* sH = sm;                     // 4*2*N*N
-    double2* sF = sm;                     // 2*2*N*npw (aliases sH)
+    double2* sH = sm;                     // 4*SR*N*N
+    double2* sF = sm;                     // 2*SR*N*npw (aliases sH)
     const int lin = xcd_task(blockIdx.x, gridDim.x);
     const int task = lin / kFusedParts, part = lin % kFusedParts;
     const double2* __restrict__ tabS = tabS_a;
@@ -381,8 +385,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     const int nh = g.pz, nd = g.nd;
     int nk = 0;
     while (nk < 8 && kids[8 * task + nk] >= 0) ++nk;   // leaf children are packed first
-    const int nround = (nh + 1) / 2;
-    const int rb = 2 * (part * nround / kFusedParts), re = 2 * ((part + 1) * nround / kFusedParts);
+    const int nround = (nh + SR - 1) / SR;
+    const int rb = SR * (part * nround / kFusedParts), re = SR * ((part + 1) * nround / kFusedParts);
     for (int ind = 0; ind < nd; ++ind) {
         ET* __restrict__ E = expn + ((size_t)sl * 2 * nd + ind) * g.nexp;
         // norder 8: one wave per group g = (sx, sy) of children; the values
@@ -407,18 +411,19 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                 }
             }
         }
-        for (int r = rb; r < re; r += 2) {
+        for (int r = rb; r < re; r += SR) {
             // z: H[g][s][m2][m1] = sum_{c in g} sum_m3 Tz_c[k3_s][m3] f_c[m1,m2,m3]
             if constexpr (N * N == 64) {
                 const int q = wave;
-                double2 H[2];
-                H[0] = H[1] = make_double2(0.0, 0.0);
+                double2 H[SR];
+#pragma unroll
+                for (int s = 0; s < SR; ++s) H[s] = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int z = 0; z < 2; ++z) {
                     if (!(z ? has1 : has0)) continue;
                     const CTab2 Tz = T + base0 + (1 + z) * per;
 #pragma unroll
-                    for (int s = 0; s < 2; ++s) {
+                    for (int s = 0; s < SR; ++s) {
                         const int k3 = min(r + s, nh - 1);
                         double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -432,7 +437,7 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                     }
                 }
 #pragma unroll
-                for (int s = 0; s < 2; ++s) sH[((q * 2 + s) * N) * N + lane] = H[s];
+                for (int s = 0; s < SR; ++s) sH[((q * SR + s) * N) * N + lane] = H[s];
             } else if (tid < N * N) {
                 double2 H[4][2];
 #pragma unroll
@@ -469,15 +474,16 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
             }
             __syncthreads();
             // y: F[sx][s][m1][k2] = sum_sy sum_m2 Ty^{sy}[k2][m2] H[(sx,sy)][s][m2][m1]
-            // lane = (m1, s in 0..1, sx in 0..1), wave strides k2
+            // lane = (m1 < MS, s < SR, sx in 0..1), wave strides k2
             {
-                const int m1 = lane & 15, s = (lane >> 4) & 1, sx = lane >> 5;
+                constexpr int MS = 32 / SR;
+                const int m1 = lane % MS, s = (lane / MS) % SR, sx = lane / (MS * SR);
                 const bool act = (m1 < N) && (r + s < nh);
                 double2 hp[N], hm[N];
 #pragma unroll
                 for (int m2 = 0; m2 < N; ++m2) {
-                    hp[m2] = act ? sH[(((sx + 0) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
-                    hm[m2] = act ? sH[(((sx + 2) * 2 + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
+                    hp[m2] = act ? sH[(((sx + 0) * SR + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
+                    hm[m2] = act ? sH[(((sx + 2) * SR + s) * N + m2) * N + m1] : make_double2(0.0, 0.0);
                 }
                 const CTab2 Typ = T + base0 + 1 * per;
                 const CTab2 Tym = T + base0 + 2 * per;
@@ -489,24 +495,25 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
                         a0 = zfma(Typ[k2 * N + m2], hp[m2], a0);
                         a1 = zfma(Tym[k2 * N + m2], hm[m2], a1);
                     }
-                    if (act) sF[((sx * 2 + s) * N + m1) * npw + k2] = make_double2(a0.x + a1.x, a0.y + a1.y);
+                    if (act) sF[((sx * SR + s) * N + m1) * npw + k2] = make_double2(a0.x + a1.x, a0.y + a1.y);
                 }
             }
             __syncthreads();
             // x: E[k3][k1][k2] = sum_sx sum_m1 Tx^{sx}[k1][m1] F[sx][s][m1][k2]
             if constexpr (XS >= 2) {
-                // K = (sx, m1); the two waves of a slab take alternate column tiles
-                const int s = wave & 1, hlf = wave >> 1;
+                // K = (sx, m1); SR = 2: the two waves of a slab take alternate
+                // column tiles, SR = 4: one wave per slab takes them all
+                const int s = SR == 4 ? wave : (wave & 1), hlf = SR == 4 ? 0 : wave >> 1;
                 const int k3 = r + s;
                 if (k3 < nh) {
                     ET* __restrict__ Ek = E + (size_t)k3 * npw * npw;
                     const CTab2 Txp = T + base0 + 1 * per;
                     const CTab2 Txm = T + base0 + 2 * per;
                     x1_lowp<2 * N, XS>(
-                        npw, lane, hlf, 2,
+                        npw, lane, hlf, SR == 4 ? 1 : 2,
                         [&](int k2, int kk) {
                             const int sx = kk >= N ? 1 : 0;
-                            return sF[((sx * 2 + s) * N + (kk - N * sx)) * npw + k2];
+                            return sF[((sx * SR + s) * N + (kk - N * sx)) * npw + k2];
                         },
                         [&](int kk, int k1) { return kk >= N ? Txm[k1 * N + kk - N] : Txp[k1 * N + kk]; },
                         [&](int k2, int k1, float2 v) { Ek[(size_t)k1 * npw + k2] = v; });
@@ -1185,12 +1192,30 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     return true;
 }
 
+// slabs per round of the fused STEP 1 kernel: four for norder 8 with the
+// matrix-core x stage while sF (8 n npw complex) leaves room for three blocks
+// per CU, else two
+int p2pwF3_slabs(int n, const DevTables& t) {
+    return (n == 8 && use_lowp1(t) && t.npw <= 48) ? 4 : 2;
+}
+
 template <int N>
 void launch_p2pwF3_typed(const Geom& g, const DevTables& t, unsigned nblk, size_t lds, const int* plev,
                          const int* pslot, const int* kids, const int* ksg, const double* fvals, double2* expn,
                          hipStream_t s, const Multi& mu) {
     if (use_lowp1(t)) {
         float2* e32 = reinterpret_cast<float2*>(expn);
+        if constexpr (N == 8) {
+            if (p2pwF3_slabs(N, t) == 4) {
+                if (t.prec == 2)
+                    hipLaunchKernelGGL((k_p2pwF3<N, 3, float2, 4>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS,
+                                       t.npw, plev, pslot, kids, ksg, fvals, e32, mu);
+                else
+                    hipLaunchKernelGGL((k_p2pwF3<N, 2, float2, 4>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS,
+                                       t.npw, plev, pslot, kids, ksg, fvals, e32, mu);
+                return;
+            }
+        }
         if (t.prec == 2)
             hipLaunchKernelGGL((k_p2pwF3<N, 3, float2>), dim3(nblk), dim3(256), lds, s, g, t.p2pwS, t.npw, plev,
                                pslot, kids, ksg, fvals, e32, mu);
@@ -1250,7 +1275,8 @@ bool p2pw_fused_3d(const Geom& g, const DevTables& t, int ntask, const int* plev
                    const Multi& mu) {
     if (g.ndim != 3 || ntask <= 0) return ntask <= 0;
     const int N = t.n;
-    const size_t lds = sizeof(double2) * std::max(8 * (size_t)N * N, 4 * (size_t)N * t.npw);
+    const size_t sr = (size_t)p2pwF3_slabs(N, t);
+    const size_t lds = sizeof(double2) * std::max(4 * sr * N * N, 2 * sr * N * t.npw);
     if (lds > t.lds_limit) return false;
     if ((long long)ntask * kFusedParts > 0x7fffffffLL) return false;
     const unsigned nblk = (unsigned)(ntask * kFusedParts);
