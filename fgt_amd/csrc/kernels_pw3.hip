@@ -552,6 +552,39 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     }
 }
 
+// XS 4 (FP32, one column tile: norder 8, plane-wave counts above 32): the
+// x table operands of every K step, converted once per task (the table
+// depends only on the task's level, x variant and kind) for all its rounds
+// and row tiles (registers: the largest plane-wave count's steps, uniform
+// guards past npw); the registers this takes halve the blocks per CU, which
+// only the LDS of the larger counts limits to the same number anyway
+template <int N, int XS>
+struct XHoist {
+    static constexpr bool on = XS == 4 && (2 * N + 15) / 16 == 1;
+    static constexpr int KSH = on ? kLowpMaxPw / 4 : 1;
+    float b0[KSH], b1[KSH];
+    __device__ __forceinline__ void load(const CTab2 UX, int npw, int lane) {
+        if constexpr (on) {
+            const int li = lane & 15, lq = lane >> 4;
+            const bool cval = li < 2 * N;
+            const int cpart = li >= N ? 1 : 0;
+            const int ckk = cval ? li - N * cpart : 0;
+            const int ks = (npw + 3) >> 2;
+#pragma unroll
+            for (int st = 0; st < KSH; ++st) {
+                if (st < ks) {
+                    const int k1 = 4 * st + lq;
+                    const bool kv = k1 < npw;
+                    const double2 u = UX[(kv ? k1 : npw - 1) * N + ckk];
+                    const bool ok = kv && cval;
+                    b0[st] = ok ? bcol(u, cpart, 0) : 0.f;
+                    b1[st] = ok ? bcol(u, cpart, 1) : 0.f;
+                }
+            }
+        }
+    }
+};
+
 // x stage of STEP 5 in reduced precision on the matrix cores (wave w takes
 // the round's slabs w, w + 4, ... of SR):
 //   ff[k2][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2],
@@ -563,7 +596,8 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
 // entries past npw get zero table values.
 template <int N, int XS, int SR>
 __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const CTab2 UX,
-                                              int npw, int nh, int r, int wave, int lane, double2* sff) {
+                                              int npw, int nh, int r, int wave, int lane, double2* sff,
+                                              const XHoist<N, XS>& hx) {
     constexpr int KW = 64 / SR;
     static_assert(KW >= N, "a (k2, slab) row of sff holds the N values kk1");
     if (r + wave >= nh) return;
@@ -580,28 +614,8 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
         ckk[t] = cval[t] ? c - N * cpart[t] : 0;
     }
     double* __restrict__ sffd = reinterpret_cast<double*>(sff);
-    // XS 4 (FP32, one column tile: norder 8, plane-wave counts above 32):
-    // the table operands of every K step are converted once for all row
-    // tiles (registers: the largest plane-wave count's steps, uniform guards
-    // past npw); the registers this takes halve the blocks per CU, which
-    // only the LDS of the larger counts limits to the same number anyway
-    constexpr bool HOIST = XS == 4 && NT == 1;
-    constexpr int KSH = HOIST ? kLowpMaxPw / 4 : 1;
-    float hb0[KSH], hb1[KSH];
-    if constexpr (HOIST) {
-        const int ks = (npw + 3) >> 2;
-#pragma unroll
-        for (int st = 0; st < KSH; ++st) {
-            if (st < ks) {
-                const int k1 = 4 * st + lq;
-                const bool kv = k1 < npw;
-                const double2 u = UX[(kv ? k1 : npw - 1) * N + ckk[0]];
-                const bool ok = kv && cval[0];
-                hb0[st] = ok ? bcol(u, cpart[0], 0) : 0.f;
-                hb1[st] = ok ? bcol(u, cpart[0], 1) : 0.f;
-            }
-        }
-    }
+    constexpr bool HOIST = XHoist<N, XS>::on;
+    constexpr int KSH = XHoist<N, XS>::KSH;
     for (int sl = wave; sl < SR; sl += 4) {
         const int k3 = r + sl;
         if (k3 >= nh) break;
@@ -634,8 +648,8 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             if (4 * c + j < ks) {
-                                acc[0][0] = mfma32(e[c & 1][j].x, hb0[4 * c + j], acc[0][0]);
-                                acc[0][1] = mfma32(e[c & 1][j].y, hb1[4 * c + j], acc[0][1]);
+                                acc[0][0] = mfma32(e[c & 1][j].x, hx.b0[4 * c + j], acc[0][0]);
+                                acc[0][1] = mfma32(e[c & 1][j].y, hx.b1[4 * c + j], acc[0][1]);
                             }
                         }
                     }
@@ -827,6 +841,9 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
     const int zt = (ZW || ZK) ? lane : tid;
     const int zk1 = zt % N, zk2 = zt / N;
     const int wbox = ZW ? T->box[wave] : -1, wcode = ZW ? T->code[wave] : 0;
+    // x-stage table operands of the task (XS 4 only)
+    XHoist<N, XS> hx;
+    hx.load(UX, npw, lane);
     for (int ind = 0; ind < nd; ++ind) {
         double G[CG][N];
 #pragma unroll
@@ -856,7 +873,7 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
             // are in flight to cover HBM latency; lanes past npw read a valid
             // duplicate and discard it.
             if constexpr (XS >= 2) {
-                pw2pot_x_lowp<N, XS, SR>(E, UX, npw, nh, r, wave, lane, sff);
+                pw2pot_x_lowp<N, XS, SR>(E, UX, npw, nh, r, wave, lane, sff, hx);
             } else if constexpr (XS == 1) {
                 const int k3 = r + wave;
                 if (k3 < nh) {
