@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace fgt {
 
@@ -590,13 +591,13 @@ struct XHoist {
 //   ff[k2][kk1] = sum_k1 Ux[kk1][k1] E[k3][k1][k2],
 // rows k2 (row tiles of 16, up to 80 frequencies), columns (kk1, part), K =
 // k1.  XS 2: FP32 operands; XS 3: BF16 hi/lo operands.  The result goes to
-// sff[(k2*SR + slab)*KW + (kk1 ^ (k2 & (KW-1)))], KW = 64 / SR (FP64
-// complex; for SR = 4 the layout of the FP64 x stages), where the y and z
-// stages read it.  Lanes past npw read a valid duplicate row (discarded); K
+// sff[(k2*SR + slab)*KW + (kk1 ^ (k2 & (KW-1)))], KW = 64 / SR (FP32
+// complex: the two accumulator chains' FP32 sum, stored exactly; for SR = 4
+// the layout of the FP64 x stages), where the y stage reads it.  Lanes past npw read a valid duplicate row (discarded); K
 // entries past npw get zero table values.
 template <int N, int XS, int SR>
 __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, const CTab2 UX,
-                                              int npw, int nh, int r, int wave, int lane, double2* sff,
+                                              int npw, int nh, int r, int wave, int lane, float2* sff,
                                               const XHoist<N, XS>& hx) {
     constexpr int KW = 64 / SR;
     static_assert(KW >= N, "a (k2, slab) row of sff holds the N values kk1");
@@ -613,7 +614,7 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
         cpart[t] = c >= N ? 1 : 0;
         ckk[t] = cval[t] ? c - N * cpart[t] : 0;
     }
-    double* __restrict__ sffd = reinterpret_cast<double*>(sff);
+    float* __restrict__ sffd = reinterpret_cast<float*>(sff);
     constexpr bool HOIST = XHoist<N, XS>::on;
     constexpr int KSH = XHoist<N, XS>::KSH;
     for (int sl = wave; sl < SR; sl += 4) {
@@ -759,7 +760,9 @@ __device__ __forceinline__ void pw2pot_x_lowp(const float2* __restrict__ E, cons
 //   y: yres[s][kk1][kk2] = sum_k2 Uy[kk2][k2] ff[k2][s][kk1], per y variant
 //   z: G_c[kk3] += Re(Uz_c[kk3][k3] yres[s][zk1][zk2]), per leaf c, with one
 //      thread per output column (zk1, zk2) holding all kk3 in registers.
-// LDS: sff [k2][s][KW] (npw*64 complex, KW = 64 / SR), yres [s][kk1][16]
+// LDS: sff [k2][s][KW] (npw*64 complex, KW = 64 / SR; FP32 after a
+// reduced-precision x stage, whose FP32 results it holds exactly, else
+// FP64), yres [s][kk1][16]
 // (1024 complex, entry kk2 at kk2 ^ kk1), sUr [v][s][kk3] (3*SR*N complex:
 // the round's z table rows of the three variants, zero past nh).
 // MF: x stage on the FP64 matrix cores (N >= 12, npw <= 64): per slab the
@@ -786,8 +789,9 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
                                                 size_t ks, Kinds kd, ZOut zo, Multi mu) {
     static_assert(NZ >= 1 && NZ <= 3 && (NZ == 1 || C == 1), "z kinds are grouped for one-leaf tasks only");
     extern __shared__ double2 sm[];
-    double2* sff = sm;
-    double2* yres = sff + (size_t)npw * 64;
+    using SF = typename std::conditional<(XS >= 2), float2, double2>::type;
+    SF* sff = reinterpret_cast<SF*>(sm);
+    double2* yres = reinterpret_cast<double2*>(sff + (size_t)npw * 64);
     double2* sUr = yres + 4 * 16 * 16;
     constexpr int NQ = N / 4;
     constexpr int PF = (N == 12) ? 6 : 8;
@@ -991,7 +995,8 @@ __global__ void __launch_bounds__(256, 2) k_pw2pot3(Geom g, const double2* __res
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) y[q] = make_double2(0.0, 0.0);
                     for (int k2 = 0; k2 < npw; ++k2) {
-                        const double2 f = sff[(k2 * SR + s) * KW + (kk1 ^ (k2 & (KW - 1)))];
+                        const SF fs = sff[(k2 * SR + s) * KW + (kk1 ^ (k2 & (KW - 1)))];
+                        const double2 f = make_double2(fs.x, fs.y);
 #pragma unroll
                         for (int q = 0; q < NQ; ++q) y[q] = zfma(UY[k2 * N + wave * NQ + q], f, y[q]);
                     }
@@ -1124,9 +1129,11 @@ bool try_p2pw3(const Geom& g, const DevTables& t, int ntask, const int* box, con
     return true;
 }
 
-size_t pw2pot3_lds(int n, int npw) {
+// xs: the x-stage variant (>= 2: reduced precision, FP32 sff)
+size_t pw2pot3_lds(int n, int npw, int xs = 0) {
     // sff, yres, sUr (three variants of up to eight slabs at norder 8, four above)
-    return sizeof(double2) * ((size_t)npw * 64 + 4 * 16 * 16 + (n == 8 ? 24 : 12) * (size_t)n);
+    return (xs >= 2 ? sizeof(float2) : sizeof(double2)) * (size_t)npw * 64 +
+           sizeof(double2) * (4 * 16 * 16 + (n == 8 ? 24 : 12) * (size_t)n);
 }
 
 template <int N, int XS, int C, int NZ = 1>
@@ -1185,18 +1192,19 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
     if (3 * t.pw2potS_ks > 0x7fffffffULL) return false;
     const int xs = x5_variant(N, t);
     if (xs == 2) {
-        // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores)
+        // reduced-precision x stage (FP32 or BF16 hi/lo matrix cores; FP32 sff)
+        const size_t lds32 = pw2pot3_lds(N, t.npw, 2);
         if (t.prec == 2) {
-            launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
+            launch_pw2pot3_typed<N, 3>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
             return true;
         }
         if constexpr (N == 8) {
             if (t.npw > 32) {
-                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
+                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
                 return true;
             }
         }
-        launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds, mu);
+        launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
         return true;
     }
     if constexpr (N >= 12) {
