@@ -1616,7 +1616,10 @@ void batch_chain_objects(fgt_batch& B) {
 }
 
 void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
-    hipStream_t st = B.streams[0], sn = B.streams[1];
+    // a profiled execute runs everything on one stream (near field and
+    // loose problems included), so each per-stage event pair times its
+    // launches alone
+    hipStream_t st = B.streams[0], sn = prof ? B.streams[0] : B.streams[1];
     const size_t nc = B.classes.size();
     if (prof) {
         prof->ev.assign(12 * nc, nullptr);
@@ -1626,8 +1629,6 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
         throw std::logic_error("batch chain streams not set up");
     std::vector<hipStream_t> ch{st};
     ch.insert(ch.end(), B.chains.begin(), B.chains.end());
-    // a profiled execute keeps every class on one chain, so the per-stage
-    // events time each launch alone
     const std::vector<int> chain = prof ? std::vector<int>(nc, 0) : class_chains(B);
     auto mark = [&](size_t c, int step, int end, hipStream_t s) {
         if (prof) hcheck(hipEventRecord(prof->ev[12 * c + 2 * step + end], s), "hipEventRecord");
@@ -1649,7 +1650,7 @@ void batch_enqueue(fgt_batch& B, StageEvents* prof = nullptr) {
     const size_t nside = B.streams.size() - 2;
     for (size_t i = 0; i < B.loose.size(); ++i) {
         fgt_solver& S = *B.solvers[B.loose[i]];
-        execute(S, B.streams[2 + i % nside], false, false, nullptr, true);
+        execute(S, prof ? st : B.streams[2 + i % nside], false, false, nullptr, true);
     }
     for (size_t c = 0; c < nc; ++c) {
         const BClass& C = *B.classes[c];

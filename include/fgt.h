@@ -228,8 +228,9 @@ int fgt_batch_class_info(const fgt_batch *b, int k, int *info);
 /* batch problem indices of class k (info[4] of them) */
 int fgt_batch_class_members(const fgt_batch *b, int k, int *members);
 /* One eager execute with HIP events around every stage of every class (the
- * reference's per-step timeinfo, boxfgt.f:1419-1432); then
- * fgt_batch_class_stage_ms gives class k's STEP 1..6 device milliseconds. */
+ * reference's per-step timeinfo, boxfgt.f:1419-1432), all on one stream so
+ * each stage is timed alone; then fgt_batch_class_stage_ms gives class k's
+ * STEP 1..6 device milliseconds. */
 int fgt_batch_profile(fgt_batch *b);
 int fgt_batch_class_stage_ms(const fgt_batch *b, int k, double *ms);
 int fgt_batch_get_pot(fgt_batch *b, int i, double *pot, int accumulate);
