@@ -672,13 +672,18 @@ __global__ void __launch_bounds__(kBlock) k_m2l_sep(Geom g, DevTables t, int nbr
 }
 
 // ---------------------------------------------------------------- STEP 4
+// Each thread takes kSplitEPT elements (all their loads issued before the
+// stores): a quarter of the blocks of one element per thread, where most
+// blocks of non-first entries only exit.
+constexpr int kSplitEPT = 4;
+
 template <typename ET>
 __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, int nentry, const int* __restrict__ pc_parent,
                                                   const int* __restrict__ pc_child,
                                                   const int* __restrict__ pc_sign, int klev,
                                                   ET* __restrict__ expn_a, Multi mu) {
     // the entries of one parent are consecutive: the block of the first
-    // reads the parent's element once and writes every child of the run;
+    // reads the parent's elements once and writes every child of the run;
     // the blocks of the others have nothing to do
     const int entry = blockIdx.y;
     auto same = [&](int a, int b) {
@@ -693,22 +698,35 @@ __global__ void __launch_bounds__(kBlock) k_split(Geom g, DevTables t, int nentr
         expn = static_cast<ET*>(q.expn);
         w1 = q.wms + (size_t)(mu.tklev[entry] - 1) * q.g.npw;
     }
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)g.nd * g.nexp;
-    if (e >= tot) return;
-    const int r = (int)(e % g.nexp);
-    const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
-    const double2 v = ldE(expn[((size_t)pc_parent[entry] * 2 + 1) * tot + e]);
-    // ms_phase per child from the two x-y products (a conjugate is exact)
-    const double2 wx = g.px > 1 ? w1[k1] : make_double2(1.0, 0.0);
-    const double2 wy = g.py > 1 ? w1[k2] : make_double2(1.0, 0.0);
-    const double2 wz = w1[k3];
-    const double2 pa = cmul(wx, wy), pb = cmul(conjd(wx), wy);
+    const long long e0 = (long long)blockIdx.x * kSplitEPT * blockDim.x + threadIdx.x;
+    if (e0 >= tot) return;
+    const ET* __restrict__ src = expn + ((size_t)pc_parent[entry] * 2 + 1) * tot;
+    double2 v[kSplitEPT], pa[kSplitEPT], pb[kSplitEPT], wz[kSplitEPT];
+#pragma unroll
+    for (int k = 0; k < kSplitEPT; ++k) {
+        const long long e = e0 + (long long)k * blockDim.x;
+        const long long ec = e < tot ? e : tot - 1;   // valid duplicate, never stored
+        v[k] = ldE(src[ec]);
+        const int r = (int)(ec % g.nexp);
+        const int k2 = r % g.py, k1 = (r / g.py) % g.px, k3 = r / (g.py * g.px);
+        // ms_phase per child from the two x-y products (a conjugate is exact)
+        const double2 wx = g.px > 1 ? w1[k1] : make_double2(1.0, 0.0);
+        const double2 wy = g.py > 1 ? w1[k2] : make_double2(1.0, 0.0);
+        wz[k] = w1[k3];
+        pa[k] = cmul(wx, wy);
+        pb[k] = cmul(conjd(wx), wy);
+    }
     for (int j = entry; j < nentry && (j == entry || same(entry, j)); ++j) {
         const int sg = pc_sign[j];
-        const double2 xy = (sg & 1) ? ((sg & 2) ? conjd(pa) : pb) : ((sg & 2) ? conjd(pb) : pa);
-        const double2 ph = cmul(xy, (sg & 4) ? conjd(wz) : wz);
-        stE(expn + ((size_t)pc_child[j] * 2 + 1) * tot + e, cmul(v, ph));
+        ET* __restrict__ dst = expn + ((size_t)pc_child[j] * 2 + 1) * tot;
+#pragma unroll
+        for (int k = 0; k < kSplitEPT; ++k) {
+            const long long e = e0 + (long long)k * blockDim.x;
+            const double2 xy = (sg & 1) ? ((sg & 2) ? conjd(pa[k]) : pb[k]) : ((sg & 2) ? conjd(pb[k]) : pa[k]);
+            const double2 ph = cmul(xy, (sg & 4) ? conjd(wz[k]) : wz[k]);
+            if (e < tot) stE(dst + e, cmul(v[k], ph));
+        }
     }
 }
 
@@ -1108,7 +1126,7 @@ void launch_split(const Geom& g, const DevTables& t, int nentry, const int* pc_p
                   const Multi& mu) {
     for (int a = 0; a < nentry; a += kMaxGridY) {
         const int cnt = std::min(kMaxGridY, nentry - a);
-        const dim3 grid(blocks_for((long long)g.nd * g.nexp), cnt);
+        const dim3 grid(blocks_for(((long long)g.nd * g.nexp + kSplitEPT - 1) / kSplitEPT), cnt);
         Multi ma = mu;
         if (ma.pp) ma.tprob += a, ma.tklev += a;
         if (t.expn_f32)
