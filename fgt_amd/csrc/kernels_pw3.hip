@@ -553,12 +553,12 @@ __global__ void __launch_bounds__(256) k_p2pwF3(Geom g, const double2* __restric
     }
 }
 
-// XS 4 (FP32, one column tile: norder 8, plane-wave counts above 32): the
-// x table operands of every K step, converted once per task (the table
-// depends only on the task's level, x variant and kind) for all its rounds
-// and row tiles (registers: the largest plane-wave count's steps, uniform
-// guards past npw); the registers this takes halve the blocks per CU, which
-// only the LDS of the larger counts limits to the same number anyway
+// XS 4 (FP32, one column tile: norder 8): the x table operands of every K
+// step, converted once per task (the table depends only on the task's
+// level, x variant and kind) for all its rounds and row tiles (registers:
+// the largest plane-wave count's steps, uniform guards past npw; 147 VGPRs,
+// three waves per SIMD, against four for XS 2, which converts them per row
+// tile: STEP 5 of 21 norder-8, npw <= 32 problems 1.80 -> 1.47 ms)
 template <int N, int XS>
 struct XHoist {
     static constexpr bool on = XS == 4 && (2 * N + 15) / 16 == 1;
@@ -1199,10 +1199,8 @@ bool try_pw2pot3(const Geom& g, const DevTables& t, int ntask, int nsib, const P
             return true;
         }
         if constexpr (N == 8) {
-            if (t.npw > 32) {
-                launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
-                return true;
-            }
+            launch_pw2pot3_typed<N, 4>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
+            return true;
         }
         launch_pw2pot3_typed<N, 2>(g, t, ntask, nsib, tasks, expn, kd, zo, s, lds32, mu);
         return true;
